@@ -1,0 +1,147 @@
+/* C-ABI of libpibronic_hip.so: the MI355X (gfx950) kernels behind Pibronic's per-block
+ * PIMC estimator.
+ *
+ * The reference has no FFI: its boundary is the Python function API of
+ * pibronic/pimc/pimc.py, whose module-level kernels take the precomputed model constants
+ * (`model.const`, a TemperatureDependentClass, and `state_shift`) and mutate preallocated
+ * block buffers. This header is that seam as plain C: the Python host (pibronic_amd.pimc)
+ * keeps the reference's classes and computes the same constants with NumPy, then hands
+ * them over once per run; each block is one or three calls.
+ *
+ * Conventions
+ *  - All arrays are C-order float64 unless stated. Pointers named *host* are read during
+ *    the call and may be freed afterwards; pointers named *dev* are device memory that the
+ *    caller owns (for example torch tensors on cuda:0) and must stay valid until the work
+ *    queued on `stream` has finished.
+ *  - `stream` is a hipStream_t passed as void*; NULL means the default stream.
+ *  - Every function returns PIB_OK (0) or a negative PIB_ERR_* code; the message is
+ *    available from pib_last_error(ctx). Nothing throws or aborts across the ABI.
+ *  - A context is bound to one device and is not re-entrant: calls on one context must not
+ *    overlap. Use one context per host thread.
+ */
+#ifndef PIBRONIC_HIP_H
+#define PIBRONIC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIB_ABI_VERSION 1
+
+#define PIB_OK 0
+#define PIB_ERR_ARG (-1)         /* invalid argument (reference: AssertionError)        */
+#define PIB_ERR_HIP (-2)         /* HIP runtime error                                   */
+#define PIB_ERR_STATE (-3)       /* required constants not set                          */
+#define PIB_ERR_UNSUPPORTED (-4) /* surface count has no compiled kernel               */
+
+/* slots of pib_set_omatrix_set */
+#define PIB_SET_RHO 0       /* sampling model at tau          (data.rho.const)        */
+#define PIB_SET_VIB 1       /* coupled model at tau           (data.vib.const)        */
+#define PIB_SET_VIB_PLUS 2  /* coupled model at tau_plus      (data.vib.const_plus)   */
+#define PIB_SET_VIB_MINUS 3 /* coupled model at tau_minus     (data.vib.const_minus)  */
+
+/* flags of pib_eval_block */
+#define PIB_EVAL_PM 1      /* also g(beta +- dbeta): block_compute_pm                     */
+#define PIB_EVAL_NOSCALE 2 /* skip the S12 scaling (S_p = 1), as the explicit golden test */
+
+/* rows of the per-sample output matrix written by pib_eval_block */
+#define PIB_OUT_RHO 0  /* s_rho = rho / prod_p S_p             (BoxResult.scaled_rho)       */
+#define PIB_OUT_G 1    /* s_g   = g   / prod_p S_p             (BoxResult.scaled_g)         */
+#define PIB_OUT_GP 2   /* s_gP                                 (BoxResultPM.scaled_gofr_plus)  */
+#define PIB_OUT_GM 3   /* s_gM                                 (BoxResultPM.scaled_gofr_minus) */
+#define PIB_OUT_LOGS 4 /* sum_p log S_p (no reference counterpart: the dropped scale)      */
+#define PIB_NUM_OUT 5
+
+/* entries of the block-sum vector written by pib_block_sums */
+#define PIB_SUM_N 0     /* samples                                  */
+#define PIB_SUM_R 1     /* sum r, r = g/rho                         */
+#define PIB_SUM_R2 2    /* sum r^2                                  */
+#define PIB_SUM_ABSR 3  /* sum |r|                                  */
+#define PIB_SUM_NEG 4   /* number of r < 0                          */
+#define PIB_SUM_D1 5    /* sum d1, d1 = (g+ - g-)/(2 dbeta rho)     */
+#define PIB_SUM_D1SQ 6  /* sum d1^2                                 */
+#define PIB_SUM_D2 7    /* sum d2, d2 = (g+ - 2g + g-)/(dbeta^2 rho) */
+#define PIB_SUM_D2SQ 8  /* sum d2^2                                 */
+#define PIB_SUM_RD1 9   /* sum r d1                                 */
+#define PIB_SUM_RD2 10  /* sum r d2                                 */
+#define PIB_SUM_D1D2 11 /* sum d1 d2                                */
+#define PIB_SUM_RP 12   /* sum g+/rho                               */
+#define PIB_SUM_RM 13   /* sum g-/rho                               */
+#define PIB_NUM_SUMS 14
+
+typedef struct pib_ctx pib_ctx;
+
+/* Library / context lifetime. No reference counterpart (the reference is in-process
+ * NumPy); the context holds the per-run constants that BoxData.preprocess builds
+ * (pibronic/pimc/pimc.py:647-692). */
+int pib_abi_version(void);
+int pib_ctx_create(int device, pib_ctx** out);
+int pib_ctx_destroy(pib_ctx* ctx);
+const char* pib_last_error(const pib_ctx* ctx);
+
+/* Residual diabatic coupling of the coupled model after folding:
+ * energy_host (A*A) with the diagonal zeroed, linear_host (N*A*A) with the surface
+ * diagonal zeroed, quadratic_host (N*N*A*A, may be NULL for none). Replaces the model
+ * state read by diagonalize_coupling_matrix (pibronic/pimc/pimc.py:1139-1160) as left by
+ * ModelVibronic.finish_folding_in_terms (:208-218). */
+int pib_set_coupling(pib_ctx* ctx, int A, int N, const double* energy_host,
+                     const double* linear_host, const double* quadratic_host);
+
+/* One set of O-matrix constants: coth_host, csch_host (N), prefactor_host (A_set) and the
+ * model's state shift state_shift_host (A_set*N). Replaces the TemperatureDependentClass
+ * fields cothAN/cschAN/omatrix_prefactor (pibronic/pimc/pimc.py:59-83) and the
+ * `state_shift` argument of build_o_matrix (:1087). The reference broadcasts coth and csch
+ * over surfaces; they are passed once per mode. */
+int pib_set_omatrix_set(pib_ctx* ctx, int slot, int A_set, int N, const double* coth_host,
+                        const double* csch_host, const double* prefactor_host,
+                        const double* state_shift_host);
+
+/* Bead count P (>= 3, pibronic/pimc/pimc.py:683) and the base tau = beta/P used for
+ * M = exp(-tau V) in every ring product (build_numerator, :1183; quirk Q1). */
+int pib_set_beads(pib_ctx* ctx, int P, double tau);
+
+/* Lanes cooperating on one sample (power of two <= 64), 0 = automatic. Tuning only. */
+int pib_set_lanes_per_sample(pib_ctx* ctx, int lanes);
+int pib_get_lanes_per_sample(pib_ctx* ctx, int64_t B);
+
+/* Evaluate one block of B samples from bead coordinates q_dev (B*N*P, the reference's
+ * transformed qTensor[:, 0] of pibronic/pimc/pimc.py:618-624). Writes PIB_NUM_OUT rows of
+ * out_stride (>= B) doubles to out_dev: the S12-scaled rho and g (and g+- with
+ * PIB_EVAL_PM) of block_compute / block_compute_pm (:1341-1385, :1388-1462), computed by
+ * build_o_matrix, build_scaling_factors, scale_o_matrices, build_denominator,
+ * diagonalize_coupling_matrix and build_numerator (:1062-1213) fused into one launch.
+ * debug_dev, if not NULL, receives per (sample, bead) records of
+ * [O_rho (A_rho), O_vib (A), S_p, M (A*A)] (unscaled O), B*P records. */
+int pib_eval_block(pib_ctx* ctx, const double* q_dev, int64_t B, int flags, double* out_dev,
+                   int64_t out_stride, double* debug_dev, void* stream);
+
+/* Sampling constants of the harmonic rho: surface weights weights_host (A_rho, normalised,
+ * ModelSampling.compute_weight_for_each_state, pibronic/pimc/pimc.py:341-351), standard
+ * deviations sigma_host (A_rho*N*P, :401-404), state shift (A_rho*N, :372-381) and the
+ * circulant eigenvectors circ_eigvects_host (P*P, row p = bead, :684-689). */
+int pib_set_sampler(pib_ctx* ctx, int A_rho, int N, int P, const double* weights_host,
+                    const double* sigma_host, const double* state_shift_host,
+                    const double* circ_eigvects_host);
+
+/* Draw B samples of bead coordinates into q_dev (B*N*P) and, optionally, their surface
+ * sources into src_dev (B int32). Replaces draw_sample + transform_sampled_coordinates
+ * (pibronic/pimc/pimc.py:595-600, 326-334, 613-631). Counter-based: sample i of the call
+ * uses the Philox counter of global sample index offset+i under key `seed`, so blocks are
+ * independent streams and any split of a run gives the same samples. */
+int pib_sample_block(pib_ctx* ctx, uint64_t seed, int64_t offset, int64_t B, double* q_dev,
+                     int32_t* src_dev, void* stream);
+
+/* Reduce a block's per-sample outputs (as written by pib_eval_block) into PIB_NUM_SUMS
+ * doubles at sums_dev, in a fixed order (deterministic). The per-sample terms are those of
+ * calculate_basic_property_terms (pibronic/stats/stats.py:38-55); with pm == 0 the
+ * derivative entries are zero. */
+int pib_block_sums(pib_ctx* ctx, const double* out_dev, int64_t out_stride, int64_t B, int pm,
+                   double delta_beta, double* sums_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIBRONIC_HIP_H */

@@ -1,0 +1,178 @@
+"""ctypes binding of libpibronic_hip.so (the C-ABI declared in include/pibronic_hip.h).
+
+The library is the only compute path of this package: there is no NumPy or PyTorch
+fallback. If the shared object is missing or cannot be loaded, `load()` raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB_NAME = "libpibronic_hip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+ABI_VERSION = 1
+
+PIB_OK = 0
+PIB_ERR_ARG = -1
+PIB_ERR_HIP = -2
+PIB_ERR_STATE = -3
+PIB_ERR_UNSUPPORTED = -4
+
+SET_RHO, SET_VIB, SET_VIB_PLUS, SET_VIB_MINUS = 0, 1, 2, 3
+EVAL_PM, EVAL_NOSCALE = 1, 2
+OUT_RHO, OUT_G, OUT_GP, OUT_GM, OUT_LOGS = 0, 1, 2, 3, 4
+NUM_OUT = 5
+(SUM_N, SUM_R, SUM_R2, SUM_ABSR, SUM_NEG, SUM_D1, SUM_D1SQ, SUM_D2, SUM_D2SQ,
+ SUM_RD1, SUM_RD2, SUM_D1D2, SUM_RP, SUM_RM) = range(14)
+NUM_SUMS = 14
+
+# exported symbol -> (restype, argtypes); must match include/pibronic_hip.h
+_c_int, _c_i64, _c_u64, _c_dbl, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double, ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    "pib_abi_version": (_c_int, []),
+    "pib_ctx_create": (_c_int, [_c_int, ctypes.POINTER(_vp)]),
+    "pib_ctx_destroy": (_c_int, [_vp]),
+    "pib_last_error": (ctypes.c_char_p, [_vp]),
+    "pib_set_coupling": (_c_int, [_vp, _c_int, _c_int, _dp, _dp, _dp]),
+    "pib_set_omatrix_set": (_c_int, [_vp, _c_int, _c_int, _c_int, _dp, _dp, _dp, _dp]),
+    "pib_set_beads": (_c_int, [_vp, _c_int, _c_dbl]),
+    "pib_set_lanes_per_sample": (_c_int, [_vp, _c_int]),
+    "pib_get_lanes_per_sample": (_c_int, [_vp, _c_i64]),
+    "pib_eval_block": (_c_int, [_vp, _vp, _c_i64, _c_int, _vp, _c_i64, _vp, _vp]),
+    "pib_set_sampler": (_c_int, [_vp, _c_int, _c_int, _c_int, _dp, _dp, _dp, _dp]),
+    "pib_sample_block": (_c_int, [_vp, _c_u64, _c_i64, _c_i64, _vp, _vp, _vp]),
+    "pib_block_sums": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_int, _c_dbl, _vp, _vp]),
+}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    """Raised for a non-zero status from the C-ABI (message from pib_last_error)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+        self.msg = msg
+
+
+def load(path=None):
+    """Load the shared library once and attach the ctypes signatures. Raises when absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(f"{p} not found: build it with `python -m pibronic_amd.build`")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.pib_abi_version() != ABI_VERSION:
+        raise ImportError(f"{p}: ABI version {lib.pib_abi_version()} != {ABI_VERSION}")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _dptr(a):
+    """Host float64 C-contiguous array -> double*; None -> NULL."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def _host(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _devptr(t):
+    """torch tensor (device) or integer address -> void* value."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class Context:
+    """One pib_ctx on one device; thin checked wrappers around every entry point."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        rc = self.lib.pib_ctx_create(int(device), ctypes.byref(h))
+        if rc != PIB_OK:
+            raise NativeError(rc, f"pib_ctx_create(device={device}) failed")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pib_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != PIB_OK:
+            raise NativeError(rc, self.lib.pib_last_error(self.h).decode())
+
+    # ------------------------------------------------------------------ constants
+    def set_coupling(self, energy, linear, quadratic=None):
+        E, L, Q = _host(energy), _host(linear), _host(quadratic)
+        A, N = E.shape[0], L.shape[0]
+        self._check(self.lib.pib_set_coupling(self.h, A, N, _dptr(E), _dptr(L), _dptr(Q)))
+
+    def set_omatrix_set(self, slot, coth, csch, prefactor, state_shift):
+        c, s, f, d = _host(coth), _host(csch), _host(prefactor), _host(state_shift)
+        self._check(self.lib.pib_set_omatrix_set(self.h, slot, f.shape[0], c.shape[0],
+                                                 _dptr(c), _dptr(s), _dptr(f), _dptr(d)))
+
+    def set_beads(self, P, tau):
+        self._check(self.lib.pib_set_beads(self.h, int(P), float(tau)))
+
+    def set_lanes_per_sample(self, lanes):
+        self._check(self.lib.pib_set_lanes_per_sample(self.h, int(lanes)))
+
+    def lanes_per_sample(self, B):
+        r = self.lib.pib_get_lanes_per_sample(self.h, int(B))
+        if r < 0:
+            self._check(r)
+        return r
+
+    def set_sampler(self, weights, sigma, state_shift, circ_eigvects):
+        w, s, d, U = _host(weights), _host(sigma), _host(state_shift), _host(circ_eigvects)
+        A_rho, N, P = s.shape
+        self._check(self.lib.pib_set_sampler(self.h, A_rho, N, P, _dptr(w), _dptr(s),
+                                             _dptr(d), _dptr(U)))
+
+    # ------------------------------------------------------------------ per block
+    def eval_block(self, q, B, flags, out, out_stride, debug=None, stream=None):
+        self._check(self.lib.pib_eval_block(self.h, _devptr(q), int(B), int(flags), _devptr(out),
+                                            int(out_stride), _devptr(debug), _stream(stream)))
+
+    def sample_block(self, seed, offset, B, q, src=None, stream=None):
+        self._check(self.lib.pib_sample_block(self.h, int(seed) & (2**64 - 1), int(offset), int(B),
+                                              _devptr(q), _devptr(src), _stream(stream)))
+
+    def block_sums(self, out, out_stride, B, pm, delta_beta, sums, stream=None):
+        self._check(self.lib.pib_block_sums(self.h, _devptr(out), int(out_stride), int(B),
+                                            int(bool(pm)), float(delta_beta), _devptr(sums),
+                                            _stream(stream)))

@@ -1,0 +1,398 @@
+// C-ABI layer: context, constant upload, argument checks and kernel dispatch.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pibronic_hip.h"
+#include "pib_common.hpp"
+#include "pib_kernels.hpp"
+
+using namespace pib;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct pib_ctx {
+  int device = 0;
+  int num_cu = 256;
+  std::string err;
+  // coupling
+  int A = 0, N = 0;
+  DevBuf energy, linear, qdiag, pair, qpair;
+  int npair = 0;
+  bool has_qdiag = false;
+  bool coupling_set = false;
+  // O sets
+  int set_A[NUM_SETS] = {0, 0, 0, 0};
+  int set_N[NUM_SETS] = {0, 0, 0, 0};
+  DevBuf coth[NUM_SETS], csch[NUM_SETS], pref[NUM_SETS], shift[NUM_SETS];
+  // beads
+  int P = 0;
+  double tau = 0.0;
+  int lanes = 0;
+  // sampler
+  int s_A = 0, s_N = 0, s_P = 0;
+  DevBuf s_cdf, s_sigma, s_shift, s_Ut;
+  bool sampler_set = false;
+  // reduction scratch
+  DevBuf partials;
+  int nparts = 0;
+};
+
+namespace {
+
+int fail(pib_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(pib_ctx* c, hipError_t e, const char* what) {
+  return fail(c, PIB_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define PIB_HIP(ctx, call)                                   \
+  do {                                                       \
+    hipError_t e_ = (call);                                  \
+    if (e_ != hipSuccess) return hip_fail((ctx), e_, #call); \
+  } while (0)
+
+// (Re)allocate a device buffer and copy `bytes` from host. Synchronises the device first so
+// that no queued kernel can still read the previous contents.
+int upload(pib_ctx* c, DevBuf& b, const void* host, size_t bytes) {
+  PIB_HIP(c, hipSetDevice(c->device));
+  if (b.p && b.bytes < bytes) {
+    PIB_HIP(c, hipDeviceSynchronize());
+    PIB_HIP(c, hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (!b.p && bytes) {
+    PIB_HIP(c, hipMalloc(&b.p, bytes));
+    b.bytes = bytes;
+  }
+  if (bytes) {
+    PIB_HIP(c, hipDeviceSynchronize());
+    PIB_HIP(c, hipMemcpy(b.p, host, bytes, hipMemcpyHostToDevice));
+  }
+  return PIB_OK;
+}
+
+void release(DevBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+bool finite_all(const double* v, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return false;
+  return true;
+}
+
+int auto_lanes(const pib_ctx* c, int64_t B) {
+  // enough lanes in flight to cover every SIMD with a few waves; never more lanes per
+  // sample than beads, so that each lane owns at least one bead
+  const int64_t target = (int64_t)c->num_cu * 4 * kWave * 2;
+  int maxg = 1;
+  while (maxg < 64 && maxg * 2 <= c->P) maxg *= 2;
+  int g = 1;
+  while (g < maxg && B * g < target) g *= 2;
+  return g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pib_abi_version(void) { return PIB_ABI_VERSION; }
+
+int pib_ctx_create(int device, pib_ctx** out) {
+  if (!out) return PIB_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return PIB_ERR_HIP;
+  pib_ctx* c = new pib_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return PIB_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    c->num_cu = prop.multiProcessorCount;
+  *out = c;
+  return PIB_OK;
+}
+
+int pib_ctx_destroy(pib_ctx* c) {
+  if (!c) return PIB_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();
+  release(c->energy);
+  release(c->linear);
+  release(c->qdiag);
+  release(c->pair);
+  release(c->qpair);
+  for (int s = 0; s < NUM_SETS; ++s) {
+    release(c->coth[s]);
+    release(c->csch[s]);
+    release(c->pref[s]);
+    release(c->shift[s]);
+  }
+  release(c->s_cdf);
+  release(c->s_sigma);
+  release(c->s_shift);
+  release(c->s_Ut);
+  release(c->partials);
+  delete c;
+  return PIB_OK;
+}
+
+const char* pib_last_error(const pib_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int pib_set_coupling(pib_ctx* c, int A, int N, const double* E, const double* L,
+                     const double* Q) {
+  if (!c) return PIB_ERR_ARG;
+  if (A < 1 || N < 1 || !E || !L) return fail(c, PIB_ERR_ARG, "set_coupling: bad A/N or null array");
+  const size_t AA = (size_t)A * A;
+  if (!finite_all(E, AA) || !finite_all(L, (size_t)N * AA) || (Q && !finite_all(Q, (size_t)N * N * AA)))
+    return fail(c, PIB_ERR_ARG, "set_coupling: non-finite model parameter");
+  // V must be symmetric in surfaces (pibronic/pimc/pimc.py:1169)
+  for (int j = 0; j <= N; ++j) {
+    const double* X = (j == N) ? E : L + (size_t)j * AA;
+    for (int b = 0; b < A; ++b)
+      for (int d = 0; d < b; ++d)
+        if (std::fabs(X[b * A + d] - X[d * A + b]) > 1e-8 + 1e-5 * std::fabs(X[d * A + b]))
+          return fail(c, PIB_ERR_ARG, "set_coupling: coupling matrix is not symmetric in surfaces");
+  }
+  int r;
+  if ((r = upload(c, c->energy, E, AA * sizeof(double)))) return r;
+  if ((r = upload(c, c->linear, L, (size_t)N * AA * sizeof(double)))) return r;
+  std::vector<double> qd((size_t)N * AA, 0.0), qp;
+  std::vector<int> pairs;
+  bool any_diag = false;
+  if (Q) {
+    for (int j = 0; j < N; ++j)
+      for (int k = 0; k < N; ++k) {
+        const double* blk = Q + ((size_t)j * N + k) * AA;
+        bool nz = false;
+        for (size_t i = 0; i < AA; ++i) nz |= (blk[i] != 0.0);
+        if (!nz) continue;
+        if (j == k) {
+          any_diag = true;
+          for (size_t i = 0; i < AA; ++i) qd[(size_t)j * AA + i] = 0.5 * blk[i];
+        } else {
+          pairs.push_back(j);
+          pairs.push_back(k);
+          for (size_t i = 0; i < AA; ++i) qp.push_back(0.5 * blk[i]);
+        }
+      }
+  }
+  c->has_qdiag = any_diag;
+  if (any_diag && (r = upload(c, c->qdiag, qd.data(), qd.size() * sizeof(double)))) return r;
+  c->npair = (int)(pairs.size() / 2);
+  if (c->npair) {
+    if ((r = upload(c, c->pair, pairs.data(), pairs.size() * sizeof(int)))) return r;
+    if ((r = upload(c, c->qpair, qp.data(), qp.size() * sizeof(double)))) return r;
+  }
+  c->A = A;
+  c->N = N;
+  c->coupling_set = true;
+  return PIB_OK;
+}
+
+int pib_set_omatrix_set(pib_ctx* c, int slot, int A_set, int N, const double* coth,
+                        const double* csch, const double* pref, const double* shift) {
+  if (!c) return PIB_ERR_ARG;
+  if (slot < 0 || slot >= NUM_SETS) return fail(c, PIB_ERR_ARG, "set_omatrix_set: bad slot");
+  if (A_set < 1 || N < 1 || !coth || !csch || !pref || !shift)
+    return fail(c, PIB_ERR_ARG, "set_omatrix_set: bad size or null array");
+  if (!finite_all(coth, N) || !finite_all(csch, N) || !finite_all(pref, A_set) ||
+      !finite_all(shift, (size_t)A_set * N))
+    return fail(c, PIB_ERR_ARG, "set_omatrix_set: non-finite constant");
+  int r;
+  if ((r = upload(c, c->coth[slot], coth, N * sizeof(double)))) return r;
+  if ((r = upload(c, c->csch[slot], csch, N * sizeof(double)))) return r;
+  if ((r = upload(c, c->pref[slot], pref, A_set * sizeof(double)))) return r;
+  if ((r = upload(c, c->shift[slot], shift, (size_t)A_set * N * sizeof(double)))) return r;
+  c->set_A[slot] = A_set;
+  c->set_N[slot] = N;
+  return PIB_OK;
+}
+
+int pib_set_beads(pib_ctx* c, int P, double tau) {
+  if (!c) return PIB_ERR_ARG;
+  if (P < 3) return fail(c, PIB_ERR_ARG, "circulant matrix requires 3 or more beads");
+  if (!std::isfinite(tau) || tau <= 0.0) return fail(c, PIB_ERR_ARG, "set_beads: tau must be > 0");
+  c->P = P;
+  c->tau = tau;
+  return PIB_OK;
+}
+
+int pib_set_lanes_per_sample(pib_ctx* c, int lanes) {
+  if (!c) return PIB_ERR_ARG;
+  if (lanes != 0 && (lanes < 1 || lanes > 64 || (lanes & (lanes - 1))))
+    return fail(c, PIB_ERR_ARG, "lanes per sample must be 0 or a power of two <= 64");
+  c->lanes = lanes;
+  return PIB_OK;
+}
+
+int pib_get_lanes_per_sample(pib_ctx* c, int64_t B) {
+  if (!c) return PIB_ERR_ARG;
+  if (c->lanes) return c->lanes;
+  if (c->P < 3) return fail(c, PIB_ERR_STATE, "beads not set");
+  return auto_lanes(c, B);
+}
+
+int pib_eval_block(pib_ctx* c, const double* q, int64_t B, int flags, double* out,
+                   int64_t out_stride, double* dbg, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  const bool pm = (flags & PIB_EVAL_PM) != 0;
+  if (flags & ~(PIB_EVAL_PM | PIB_EVAL_NOSCALE)) return fail(c, PIB_ERR_ARG, "eval_block: unknown flag");
+  if (!c->coupling_set || c->P < 3) return fail(c, PIB_ERR_STATE, "eval_block: coupling or beads not set");
+  const int need = pm ? 4 : 2;
+  for (int s = 0; s < need; ++s)
+    if (!c->set_A[s]) return fail(c, PIB_ERR_STATE, "eval_block: O-matrix set " + std::to_string(s) + " not set");
+  for (int s = 1; s < need; ++s)
+    if (c->set_A[s] != c->A || c->set_N[s] != c->N)
+      return fail(c, PIB_ERR_ARG, "eval_block: coupled-model O-matrix set has the wrong surface/mode count");
+  if (c->set_N[0] != c->N) return fail(c, PIB_ERR_ARG, "eval_block: sampling model has a different number of modes");
+  const int AR = c->set_A[SET_RHO];
+  if (!eval_supported(c->A, AR))
+    return fail(c, PIB_ERR_UNSUPPORTED, "eval_block: no FP64 kernel for A=" + std::to_string(c->A) +
+                                             ", A_rho=" + std::to_string(AR));
+  if (B < 0 || out_stride < B) return fail(c, PIB_ERR_ARG, "eval_block: B < 0 or out_stride < B");
+  if (B == 0) return PIB_OK;
+  if (!q || !out) return fail(c, PIB_ERR_ARG, "eval_block: null q or out");
+  EvalArgs a{};
+  a.q = q;
+  a.B = B;
+  a.N = c->N;
+  a.P = c->P;
+  a.G = c->lanes ? c->lanes : auto_lanes(c, B);
+  a.K = (c->P + a.G - 1) / a.G;
+  a.noscale = (flags & PIB_EVAL_NOSCALE) ? 1 : 0;
+  a.tau = c->tau;
+  for (int s = 0; s < NUM_SETS; ++s) {
+    a.set[s].coth = (const double*)c->coth[s].p;
+    a.set[s].csch = (const double*)c->csch[s].p;
+    a.set[s].pref = (const double*)c->pref[s].p;
+    a.set[s].shift = (const double*)c->shift[s].p;
+    if (s >= need) a.set[s] = a.set[SET_VIB];
+  }
+  a.energy = (const double*)c->energy.p;
+  a.linear = (const double*)c->linear.p;
+  a.qdiag = c->has_qdiag ? (const double*)c->qdiag.p : nullptr;
+  a.npair = c->npair;
+  a.pair = (const int*)c->pair.p;
+  a.qpair = (const double*)c->qpair.p;
+  a.out = out;
+  a.out_stride = out_stride;
+  a.dbg = dbg;
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_eval_fp64(c->A, AR, pm ? 1 : 0, a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "eval_block launch");
+  return PIB_OK;
+}
+
+int pib_set_sampler(pib_ctx* c, int A_rho, int N, int P, const double* w, const double* sigma,
+                    const double* shift, const double* U) {
+  if (!c) return PIB_ERR_ARG;
+  if (A_rho < 1 || N < 1 || P < 3 || !w || !sigma || !shift || !U)
+    return fail(c, PIB_ERR_ARG, "set_sampler: bad size or null array");
+  if (P > 4096) return fail(c, PIB_ERR_ARG, "set_sampler: P > 4096 not supported");
+  if (!finite_all(w, A_rho) || !finite_all(sigma, (size_t)A_rho * N * P) ||
+      !finite_all(shift, (size_t)A_rho * N) || !finite_all(U, (size_t)P * P))
+    return fail(c, PIB_ERR_ARG, "set_sampler: non-finite constant");
+  std::vector<double> cdf(A_rho);
+  double tot = 0.0;
+  for (int a = 0; a < A_rho; ++a) {
+    if (w[a] < 0.0) return fail(c, PIB_ERR_ARG, "set_sampler: negative surface weight");
+    tot += w[a];
+  }
+  if (!(tot > 0.0)) return fail(c, PIB_ERR_ARG, "set_sampler: surface weights sum to zero");
+  double run = 0.0;
+  for (int a = 0; a < A_rho; ++a) {
+    run += w[a];
+    cdf[a] = run / tot;
+  }
+  cdf[A_rho - 1] = 1.0;
+  std::vector<double> Ut((size_t)P * P);
+  for (int p = 0; p < P; ++p)
+    for (int k = 0; k < P; ++k) Ut[(size_t)k * P + p] = U[(size_t)p * P + k];
+  int r;
+  if ((r = upload(c, c->s_cdf, cdf.data(), cdf.size() * sizeof(double)))) return r;
+  if ((r = upload(c, c->s_sigma, sigma, (size_t)A_rho * N * P * sizeof(double)))) return r;
+  if ((r = upload(c, c->s_shift, shift, (size_t)A_rho * N * sizeof(double)))) return r;
+  if ((r = upload(c, c->s_Ut, Ut.data(), Ut.size() * sizeof(double)))) return r;
+  c->s_A = A_rho;
+  c->s_N = N;
+  c->s_P = P;
+  c->sampler_set = true;
+  return PIB_OK;
+}
+
+int pib_sample_block(pib_ctx* c, uint64_t seed, int64_t offset, int64_t B, double* q,
+                     int32_t* src, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (!c->sampler_set) return fail(c, PIB_ERR_STATE, "sample_block: sampler not set");
+  if (B < 0 || offset < 0) return fail(c, PIB_ERR_ARG, "sample_block: negative B or offset");
+  if (B == 0) return PIB_OK;
+  if (!q) return fail(c, PIB_ERR_ARG, "sample_block: null q");
+  SampleArgs a{};
+  a.B = B;
+  a.N = c->s_N;
+  a.P = c->s_P;
+  a.A_rho = c->s_A;
+  a.offset = offset;
+  a.key0 = (uint32_t)seed;
+  a.key1 = (uint32_t)(seed >> 32);
+  a.cdf = (const double*)c->s_cdf.p;
+  a.sigma = (const double*)c->s_sigma.p;
+  a.shift = (const double*)c->s_shift.p;
+  a.Ut = (const double*)c->s_Ut.p;
+  a.q = q;
+  a.src = src;
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_sample_fp64(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "sample_block launch");
+  return PIB_OK;
+}
+
+int pib_block_sums(pib_ctx* c, const double* out, int64_t out_stride, int64_t B, int pm,
+                   double delta_beta, double* sums, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (!out || !sums || B < 1 || out_stride < B)
+    return fail(c, PIB_ERR_ARG, "block_sums: null pointer, B < 1 or out_stride < B");
+  if (pm && !(delta_beta > 0.0)) return fail(c, PIB_ERR_ARG, "block_sums: delta_beta must be > 0");
+  int nparts = (int)std::min<int64_t>((B + 255) / 256, 1024);
+  if (nparts < 1) nparts = 1;
+  if (c->nparts < nparts) {
+    std::vector<double> zero((size_t)nparts * NUM_SUMS, 0.0);
+    int r = upload(c, c->partials, zero.data(), zero.size() * sizeof(double));
+    if (r) return r;
+    c->nparts = nparts;
+  }
+  SumArgs a{};
+  a.out = out;
+  a.out_stride = out_stride;
+  a.B = B;
+  a.pm = pm ? 1 : 0;
+  a.delta_beta = pm ? delta_beta : 1.0;
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_block_sums(a, (double*)c->partials.p, nparts, sums, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "block_sums launch");
+  return PIB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// Shared device helpers for the PIMC kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pib {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------------------
+// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw, "Parallel random
+// numbers: as easy as 1, 2, 3", SC'11). Written from the published round function and
+// constants; 10 rounds, key bumped by the Weyl constants between rounds.
+// ---------------------------------------------------------------------------------------
+struct u32x4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ void mulhilo(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  hi = (uint32_t)(p >> 32);
+  lo = (uint32_t)p;
+}
+
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 ctr, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0, lo0, hi1, lo1;
+    mulhilo(M0, ctr.x, hi0, lo0);
+    mulhilo(M1, ctr.z, hi1, lo1);
+    u32x4 n;
+    n.x = hi1 ^ ctr.y ^ k0;
+    n.y = lo1;
+    n.z = hi0 ^ ctr.w ^ k1;
+    n.w = lo0;
+    ctr = n;
+    k0 += W0;
+    k1 += W1;
+  }
+  return ctr;
+}
+
+// 53-bit uniform in [0, 1) from two 32-bit words.
+__device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
+  const uint64_t m = ((uint64_t)(hi >> 5) << 26) | (uint64_t)(lo >> 6);
+  return (double)m * 0x1.0p-53;
+}
+
+// ---------------------------------------------------------------------------------------
+// Cross-lane moves of doubles within a wave (ds_bpermute based; width 64).
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double shfl_down_d(double v, int delta) {
+  return __shfl_down(v, (unsigned int)delta, kWave);
+}
+
+__device__ __forceinline__ double shfl_xor_d(double v, int mask) {
+  return __shfl_xor(v, mask, kWave);
+}
+
+}  // namespace pib

@@ -1,0 +1,423 @@
+// FP64 per-sample evaluation of the PIMC estimator (one launch per block of samples).
+//
+// Replaces the body of block_compute / block_compute_pm after the coordinate transform
+// (pibronic/pimc/pimc.py:1366-1381, 1421-1449): build_o_matrix (:1087-1129),
+// build_scaling_factors + scale_o_matrices (:1062-1084), build_denominator (:1132-1136),
+// diagonalize_coupling_matrix (:1139-1174) and build_numerator (:1177-1213), fused.
+//
+// Mapping (gfx950, wave64): G lanes cooperate on one sample, 64/G samples per wave. Lane g
+// owns the contiguous bead chunk [g*K, min((g+1)*K, P)). Per bead it forms the diagonal
+// O-factors of every temperature set, the residual coupling V in registers, diagonalises V
+// with a cyclic Jacobi sweep (per lane, A x A in registers), builds M = exp(-tau V) and
+// multiplies M * diag(O_vib / S_p) into its running ring product. The G partial products are
+// then combined by a log2(G)-level ordered tree of A x A products over cross-lane moves
+// (chunk 0 * chunk 1 * ... keeps the reference's left-to-right order).
+//
+// Numerics: the file is compiled with -ffp-contract=off so the O-matrix exponent is the
+// reference's expression term by term (coth*(x1^2 + x2^2) - ((2*csch)*x1)*x2, summed over
+// modes in order), which matters because that expression cancels when tau*omega is small.
+// Linear algebra (V, Jacobi, M, ring) uses explicit fma() where it helps accuracy.
+#include "pib_common.hpp"
+#include "pib_kernels.hpp"
+
+namespace pib {
+
+// ---------------------------------------------------------------------------------------
+// Cyclic Jacobi eigen-decomposition of a symmetric A x A matrix held in registers.
+// Only the upper triangle a[i][j] (i <= j) is read and written. On exit the diagonal holds
+// the eigenvalues and the columns of u the eigenvectors. Rotation per Rutishauser's form
+// (Golub & Van Loan, Matrix Computations, 4th ed., Alg. 8.5.2 / 8.5.3).
+// ---------------------------------------------------------------------------------------
+template <int A>
+__device__ __forceinline__ double& UT(double (&a)[A][A], int i, int j) {
+  return i <= j ? a[i][j] : a[j][i];
+}
+
+template <int A>
+__device__ __forceinline__ void jacobi_rotate(double (&a)[A][A], double (&u)[A][A], int p, int q) {
+  const double apq = a[p][q];
+  if (apq != 0.0) {
+    const double app = a[p][p], aqq = a[q][q];
+    const double theta = (aqq - app) / (2.0 * apq);
+    double t = 1.0 / (fabs(theta) + sqrt(fma(theta, theta, 1.0)));
+    t = theta < 0.0 ? -t : t;
+    const double c = 1.0 / sqrt(fma(t, t, 1.0));
+    const double s = t * c;
+    const double tr = s / (1.0 + c);
+    a[p][p] = fma(-t, apq, app);
+    a[q][q] = fma(t, apq, aqq);
+    a[p][q] = 0.0;
+#pragma unroll
+    for (int r = 0; r < A; ++r) {
+      if (r == p || r == q) continue;
+      const double g = UT<A>(a, r, p), h = UT<A>(a, r, q);
+      UT<A>(a, r, p) = g - s * fma(g, tr, h);
+      UT<A>(a, r, q) = h + s * fma(-h, tr, g);
+    }
+#pragma unroll
+    for (int r = 0; r < A; ++r) {
+      const double g = u[r][p], h = u[r][q];
+      u[r][p] = g - s * fma(g, tr, h);
+      u[r][q] = h + s * fma(-h, tr, g);
+    }
+  }
+}
+
+template <int A>
+__device__ __forceinline__ void jacobi_eigh(double (&a)[A][A], double (&u)[A][A]) {
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int j = 0; j < A; ++j) u[i][j] = (i == j) ? 1.0 : 0.0;
+  if (A == 1) return;
+  double fro2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < A; ++i) {
+    fro2 = fma(a[i][i], a[i][i], fro2);
+#pragma unroll
+    for (int j = i + 1; j < A; ++j) fro2 = fma(2.0 * a[i][j], a[i][j], fro2);
+  }
+  // relative off-diagonal norm 1e-15: the residual perturbs M = exp(-tau V) by about
+  // tau*|V|*1e-15, far below the parity bound; quadratic convergence reaches it in a few sweeps
+  const double tol2 = 1e-30 * fro2;
+  constexpr int kMaxSweeps = (A <= 4) ? 12 : 20;
+  for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
+    double off = 0.0;
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int j = i + 1; j < A; ++j) off = fma(a[i][j], a[i][j], off);
+    const bool done = !(off > tol2);
+    if (__all(done)) break;
+    if (!done) {
+#pragma unroll
+      for (int p = 0; p < A - 1; ++p)
+#pragma unroll
+        for (int q = p + 1; q < A; ++q) jacobi_rotate<A>(a, u, p, q);
+    }
+  }
+}
+
+// O-matrix exponent term of one mode, in the reference's evaluation order
+// (pibronic/pimc/pimc.py:1106): coth*(x1**2 + x2**2) - 2.*csch*x1*x2.
+__device__ __forceinline__ double oterm(double coth, double csch2, double x1, double x2) {
+  return coth * (x1 * x1 + x2 * x2) - (csch2 * x1) * x2;
+}
+
+template <int A, int AR, bool PM>
+__global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
+  constexpr bool SEP = (AR != A);  // qTempTensor path (pibronic/pimc/pimc.py:1094-1096)
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int G = args.G, K = args.K, N = args.N, P = args.P;
+  const int gl = lane & (G - 1);
+  const int64_t sample = wave * (kWave / G) + lane / G;
+  const bool live = sample < args.B;
+  const double* __restrict__ qs = args.q + (live ? sample : 0) * (int64_t)N * P;
+
+  const OSet sr = args.set[SET_RHO], sv = args.set[SET_VIB];
+  const OSet sp = args.set[SET_VIB_PLUS], sm = args.set[SET_VIB_MINUS];
+  const double* __restrict__ Ev = args.energy;
+  const double* __restrict__ Lv = args.linear;
+  const double* __restrict__ Qd = args.qdiag;
+
+  double Ng[A][A], Np[A][A], Nm[A][A];
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = 0; k < A; ++k) {
+      Ng[i][k] = (i == k) ? 1.0 : 0.0;
+      if (PM) { Np[i][k] = Ng[i][k]; Nm[i][k] = Ng[i][k]; }
+    }
+  double rp[AR];
+#pragma unroll
+  for (int a = 0; a < AR; ++a) rp[a] = 1.0;
+  double ls = 0.0;
+
+  const int pbeg = gl * K;
+  const int pend = live ? min(pbeg + K, P) : pbeg;
+  for (int p = pbeg; p < pend; ++p) {
+    const int pn = (p + 1 == P) ? 0 : p + 1;
+    double er[AR], ev[A], ep[A], em[A];
+    double vq[A][A], vl[A][A];
+    for (int j = 0; j < N; ++j) {
+      const double x0 = qs[j * P + p], xn = qs[j * P + pn];
+      // sampling model: q -= d_rho; O_rho; q += d_rho (pimc.py:1091-1127)
+      double q1[A], q1n[A];
+      {
+        const double c = sr.coth[j], s2 = 2.0 * sr.csch[j];
+#pragma unroll
+        for (int a = 0; a < AR; ++a) {
+          const double d = sr.shift[a * N + j];
+          const double x1 = x0 - d, x2 = xn - d;
+          const double t = oterm(c, s2, x1, x2);
+          er[a] = (j == 0) ? t : er[a] + t;
+          if (!SEP) { q1[a] = x1 + d; q1n[a] = x2 + d; }
+        }
+        if (SEP) {
+#pragma unroll
+          for (int a = 0; a < A; ++a) { q1[a] = x0; q1n[a] = xn; }
+        }
+      }
+      // coupled model at tau (and tau +- dbeta/P): q -= d_vib; O; q += d_vib
+      double q2[A];
+      {
+        const double c = sv.coth[j], s2 = 2.0 * sv.csch[j];
+        double cP = 0, sP2 = 0, cM = 0, sM2 = 0;
+        if (PM) { cP = sp.coth[j]; sP2 = 2.0 * sp.csch[j]; cM = sm.coth[j]; sM2 = 2.0 * sm.csch[j]; }
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+          const double d = sv.shift[a * N + j];
+          const double y1 = q1[a] - d, y2 = q1n[a] - d;
+          const double t = oterm(c, s2, y1, y2);
+          ev[a] = (j == 0) ? t : ev[a] + t;
+          q2[a] = y1 + d;
+          if (PM) {
+            const double q2n = y2 + d;
+            const double z1 = q2[a] - d, z2 = q2n - d;
+            const double tp = oterm(cP, sP2, z1, z2);
+            ep[a] = (j == 0) ? tp : ep[a] + tp;
+            const double w1 = (z1 + d) - d, w2 = (z2 + d) - d;
+            const double tm = oterm(cM, sM2, w1, w2);
+            em[a] = (j == 0) ? tm : em[a] + tm;
+          }
+        }
+      }
+      // residual coupling, lower triangle b >= c (eigh UPLO='L'):
+      // V[b][c] = sum_j (q_c*Qh[j,j,b,c])*q_b + sum_j L[j,b,c]*q_b  (pimc.py:1147-1160)
+#pragma unroll
+      for (int b = 0; b < A; ++b)
+#pragma unroll
+        for (int c = 0; c <= b; ++c) {
+          const double lin = Lv[(j * A + b) * A + c] * q2[b];
+          vl[b][c] = (j == 0) ? lin : vl[b][c] + lin;
+          const double qd = Qd ? (q2[c] * Qd[(j * A + b) * A + c]) * q2[b] : 0.0;
+          vq[b][c] = (j == 0) ? qd : vq[b][c] + qd;
+        }
+    }
+    // off-diagonal quadratic mode pairs (j != k): recompute the V-stage coordinate copies
+    for (int t = 0; t < args.npair; ++t) {
+      const int jj = args.pair[2 * t], kk = args.pair[2 * t + 1];
+      double qj[A], qk[A];
+#pragma unroll
+      for (int a = 0; a < A; ++a) {
+        double vj = qs[jj * P + p], vk = qs[kk * P + p];
+        if (!SEP) {
+          const double dj = sr.shift[a * N + jj], dk = sr.shift[a * N + kk];
+          vj = (vj - dj) + dj;
+          vk = (vk - dk) + dk;
+        }
+        const double ej = sv.shift[a * N + jj], ek = sv.shift[a * N + kk];
+        qj[a] = (vj - ej) + ej;
+        qk[a] = (vk - ek) + ek;
+      }
+      const double* Qh = args.qpair + (int64_t)t * A * A;
+#pragma unroll
+      for (int b = 0; b < A; ++b)
+#pragma unroll
+        for (int c = 0; c <= b; ++c) vq[b][c] += (qk[c] * Qh[b * A + c]) * qj[b];
+    }
+
+    // O-factors, per-bead scaling S_p = max over both O's (pimc.py:1076-1084)
+    double Or[AR], Ov[A], Op[A], Om[A];
+    double S = 0.0;
+#pragma unroll
+    for (int a = 0; a < AR; ++a) {
+      Or[a] = exp(-0.5 * er[a]) * sr.pref[a];
+      S = fmax(S, Or[a]);
+    }
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      Ov[a] = exp(-0.5 * ev[a]) * sv.pref[a];
+      S = fmax(S, Ov[a]);
+      if (PM) {
+        Op[a] = exp(-0.5 * ep[a]) * sp.pref[a];
+        Om[a] = exp(-0.5 * em[a]) * sm.pref[a];
+      }
+    }
+    if (args.noscale) S = 1.0;
+
+    // coupling matrix -> Jacobi -> M = U exp(-tau lambda) U^T (pimc.py:1171-1187)
+    double a_[A][A], u[A][A];
+#pragma unroll
+    for (int b = 0; b < A; ++b)
+#pragma unroll
+      for (int c = 0; c <= b; ++c) a_[c][b] = (vq[b][c] + vl[b][c]) + Ev[b * A + c];
+    jacobi_eigh<A>(a_, u);
+    double ex[A];
+#pragma unroll
+    for (int d = 0; d < A; ++d) ex[d] = exp(-args.tau * a_[d][d]);
+    double M[A][A];
+#pragma unroll
+    for (int c = 0; c < A; ++c)
+#pragma unroll
+      for (int e = 0; e <= c; ++e) {
+        double s = (u[c][0] * ex[0]) * u[e][0];
+#pragma unroll
+        for (int d = 1; d < A; ++d) s = fma(u[c][d] * ex[d], u[e][d], s);
+        M[c][e] = s;
+        M[e][c] = s;
+      }
+
+    if (args.dbg) {
+      const int rec = eval_debug_record(A, AR);
+      double* dr = args.dbg + ((int64_t)sample * P + p) * rec;
+#pragma unroll
+      for (int a = 0; a < AR; ++a) dr[a] = Or[a];
+#pragma unroll
+      for (int a = 0; a < A; ++a) dr[AR + a] = Ov[a];
+      dr[AR + A] = S;
+#pragma unroll
+      for (int c = 0; c < A; ++c)
+#pragma unroll
+        for (int e = 0; e < A; ++e) dr[AR + A + 1 + c * A + e] = M[c][e];
+    }
+
+    // scale (pimc.py:1062-1066, 1445, 1448), then accumulate rho and the ring products
+#pragma unroll
+    for (int a = 0; a < AR; ++a) rp[a] *= Or[a] / S;
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      Ov[a] = Ov[a] / S;
+      if (PM) { Op[a] = Op[a] / S; Om[a] = Om[a] / S; }
+    }
+    ls += log(S);
+#pragma unroll
+    for (int i = 0; i < A; ++i) {
+      double rg[A], rpl[A], rmi[A];
+#pragma unroll
+      for (int c = 0; c < A; ++c) {
+        rg[c] = Ng[i][c];
+        if (PM) { rpl[c] = Np[i][c]; rmi[c] = Nm[i][c]; }
+      }
+#pragma unroll
+      for (int k = 0; k < A; ++k) {
+        double s = rg[0] * M[0][k];
+#pragma unroll
+        for (int c = 1; c < A; ++c) s = fma(rg[c], M[c][k], s);
+        Ng[i][k] = s * Ov[k];
+        if (PM) {
+          double sp_ = rpl[0] * M[0][k], sm_ = rmi[0] * M[0][k];
+#pragma unroll
+          for (int c = 1; c < A; ++c) {
+            sp_ = fma(rpl[c], M[c][k], sp_);
+            sm_ = fma(rmi[c], M[c][k], sm_);
+          }
+          Np[i][k] = sp_ * Op[k];
+          Nm[i][k] = sm_ * Om[k];
+        }
+      }
+    }
+  }
+
+  // ordered tree over the G lanes of a sample: lane g (g % 2s == 0) absorbs lane g + s
+  for (int s = 1; s < G; s <<= 1) {
+    const bool leader = (gl & (2 * s - 1)) == 0;
+    double o[A][A];
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int k = 0; k < A; ++k) o[i][k] = shfl_down_d(Ng[i][k], s);
+    if (leader) {
+#pragma unroll
+      for (int i = 0; i < A; ++i) {
+        double r[A];
+#pragma unroll
+        for (int c = 0; c < A; ++c) r[c] = Ng[i][c];
+#pragma unroll
+        for (int k = 0; k < A; ++k) {
+          double acc = r[0] * o[0][k];
+#pragma unroll
+          for (int c = 1; c < A; ++c) acc = fma(r[c], o[c][k], acc);
+          Ng[i][k] = acc;
+        }
+      }
+    }
+    if (PM) {
+#pragma unroll
+      for (int w = 0; w < 2; ++w) {
+        double (&X)[A][A] = (w == 0) ? Np : Nm;
+#pragma unroll
+        for (int i = 0; i < A; ++i)
+#pragma unroll
+          for (int k = 0; k < A; ++k) o[i][k] = shfl_down_d(X[i][k], s);
+        if (leader) {
+#pragma unroll
+          for (int i = 0; i < A; ++i) {
+            double r[A];
+#pragma unroll
+            for (int c = 0; c < A; ++c) r[c] = X[i][c];
+#pragma unroll
+            for (int k = 0; k < A; ++k) {
+              double acc = r[0] * o[0][k];
+#pragma unroll
+              for (int c = 1; c < A; ++c) acc = fma(r[c], o[c][k], acc);
+              X[i][k] = acc;
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < AR; ++a) {
+      const double t = shfl_down_d(rp[a], s);
+      if (leader) rp[a] *= t;
+    }
+    const double tl = shfl_down_d(ls, s);
+    if (leader) ls += tl;
+  }
+
+  if (live && gl == 0) {
+    const int64_t st = args.out_stride;
+    double rho = rp[0];
+#pragma unroll
+    for (int a = 1; a < AR; ++a) rho += rp[a];
+    double g = Ng[0][0];
+#pragma unroll
+    for (int i = 1; i < A; ++i) g += Ng[i][i];
+    args.out[OUT_RHO * st + sample] = rho;
+    args.out[OUT_G * st + sample] = g;
+    if (PM) {
+      double gp = Np[0][0], gm = Nm[0][0];
+#pragma unroll
+      for (int i = 1; i < A; ++i) { gp += Np[i][i]; gm += Nm[i][i]; }
+      args.out[OUT_GP * st + sample] = gp;
+      args.out[OUT_GM * st + sample] = gm;
+    }
+    args.out[OUT_LOGS * st + sample] = ls;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+template <int A, int AR>
+static hipError_t launch_a(int pm, const EvalArgs& a, hipStream_t s) {
+  const int spw = kWave / a.G;
+  const int64_t waves = (a.B + spw - 1) / spw;
+  const int threads = 256;
+  const int64_t blocks = (waves * kWave + threads - 1) / threads;
+  if (pm)
+    hipLaunchKernelGGL((k_eval_fp64<A, AR, true>), dim3((unsigned)blocks), dim3(threads), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_eval_fp64<A, AR, false>), dim3((unsigned)blocks), dim3(threads), 0, s, a);
+  return hipGetLastError();
+}
+
+#define PIB_EVAL_CASES(X) \
+  X(1, 1) X(2, 2) X(3, 3) X(4, 4) X(5, 5) X(6, 6) X(7, 7) X(8, 8) \
+  X(2, 1) X(2, 4) X(2, 8) X(4, 2) X(4, 8)
+
+bool eval_supported(int A, int AR) {
+#define PIB_CASE(a_, ar_) if (A == a_ && AR == ar_) return true;
+  PIB_EVAL_CASES(PIB_CASE)
+#undef PIB_CASE
+  return false;
+}
+
+hipError_t launch_eval_fp64(int A, int AR, int pm, const EvalArgs& a, hipStream_t s) {
+#define PIB_CASE(a_, ar_) if (A == a_ && AR == ar_) return launch_a<a_, ar_>(pm, a, s);
+  PIB_EVAL_CASES(PIB_CASE)
+#undef PIB_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace pib

@@ -1,0 +1,77 @@
+// Kernel argument blocks and host-side launchers shared by the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pib {
+
+// One TemperatureDependentClass (pibronic/pimc/pimc.py:59-89) plus the state shift of the
+// model it belongs to. coth/csch are per mode (the reference broadcasts them over surfaces).
+struct OSet {
+  const double* coth;   // N
+  const double* csch;   // N
+  const double* pref;   // A_set, O-matrix prefactor F_a
+  const double* shift;  // A_set * N, state shift d_aj
+};
+
+enum : int { SET_RHO = 0, SET_VIB = 1, SET_VIB_PLUS = 2, SET_VIB_MINUS = 3, NUM_SETS = 4 };
+
+// Output rows of the per-sample result matrix (each row has out_stride entries).
+enum : int { OUT_RHO = 0, OUT_G = 1, OUT_GP = 2, OUT_GM = 3, OUT_LOGS = 4, NUM_OUT = 5 };
+
+struct EvalArgs {
+  const double* q;      // (B, N, P) bead coordinates R, sample-major
+  int64_t B;
+  int N, P;
+  int G;                // lanes per sample (power of two, divides 64)
+  int K;                // beads per lane = ceil(P / G)
+  int noscale;          // 1: skip the S12 scaling (S_p = 1), as the reference's explicit test
+  double tau;           // M-matrix step (always the base tau, quirk Q1)
+  OSet set[NUM_SETS];
+  const double* energy;   // A*A, diagonal zeroed
+  const double* linear;   // N*A*A, diagonal zeroed
+  const double* qdiag;    // N*A*A = 0.5*Q[j,j] or nullptr
+  int npair;              // off-diagonal quadratic mode pairs (j != k)
+  const int* pair;        // 2*npair (j, k)
+  const double* qpair;    // npair*A*A = 0.5*Q[j,k]
+  double* out;            // NUM_OUT rows of out_stride doubles
+  int64_t out_stride;
+  double* dbg;            // optional per-(sample, bead) dump, see eval_debug_record()
+};
+
+__host__ __device__ inline int eval_debug_record(int A, int AR) { return AR + A + 1 + A * A; }
+
+struct SampleArgs {
+  int64_t B;
+  int N, P, A_rho;
+  int64_t offset;         // global index of this block's first sample (Philox counter)
+  uint32_t key0, key1;    // Philox key (seed)
+  const double* cdf;      // A_rho cumulative surface weights
+  const double* sigma;    // A_rho * N * P standard deviations of the normal-mode draws
+  const double* shift;    // A_rho * N sampling-model state shift
+  const double* Ut;       // P * P circulant eigenvectors, transposed: Ut[k*P + p] = U[p][k]
+  double* q;              // (B, N, P) output
+  int32_t* src;           // (B,) surface sources, optional
+};
+
+enum : int {
+  SUM_N = 0, SUM_R, SUM_R2, SUM_ABSR, SUM_NEG, SUM_D1, SUM_D1SQ, SUM_D2, SUM_D2SQ,
+  SUM_RD1, SUM_RD2, SUM_D1D2, SUM_RP, SUM_RM, NUM_SUMS
+};
+
+struct SumArgs {
+  const double* out;
+  int64_t out_stride;
+  int64_t B;
+  int pm;
+  double delta_beta;
+};
+
+// launchers (return hipError_t of the launch)
+hipError_t launch_eval_fp64(int A, int AR, int pm, const EvalArgs& a, hipStream_t s);
+hipError_t launch_sample_fp64(const SampleArgs& a, hipStream_t s);
+hipError_t launch_block_sums(const SumArgs& a, double* partials, int nparts, double* sums,
+                             hipStream_t s);
+bool eval_supported(int A, int AR);
+
+}  // namespace pib
