@@ -1,0 +1,211 @@
+"""Benchmark of the PIMC block estimator on one MI355X.
+
+Workload (default): A=4 surfaces, N=8 modes, P=64 beads, T=300 K, seeded synthetic model
+(linear + mode-diagonal quadratic couplings), blocks of 1e5 samples, FP64. One step is one
+block through the hot path: device bead sampling, the fused evaluation of g and rho
+(block_compute's per-block body) and the on-device block sums. Inputs and outputs stay
+in HBM; the per-sample results are not copied to the host inside the timed region.
+
+Prints one JSON line. `value` is effective samples per second (Kish ESS of w = |g/rho|
+over the timed samples, divided by the timed wall time); samples*beads/s is reported
+beside it. The CPU baseline is the FP64 NumPy restatement of the reference algorithm
+(the oracle), one process per host core, timed on a bounded sample before the GPU is
+touched.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix peaks coincide)
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_flops(A, N, P, n_quad_pairs, pm=False):
+    """Algorithmic flops per sample, split by kernel (counting conventions: transform
+    2 N P^2; V P A^2 (2N + 2Nq + 1); eig 9 A^3 per bead; M 2A^3 + A^2 per bead; O-factors
+    A (7N + 1) per bead per set; ring 2A^3 + A^2 per bead per product; rho and scaling
+    5 P A)."""
+    sets = 4 if pm else 2
+    prods = 3 if pm else 1
+    sample = 2 * N * P * P
+    ev = (P * A * A * (2 * N + 2 * n_quad_pairs + 1) + P * 9 * A ** 3 + P * (2 * A ** 3 + A * A)
+          + P * sets * A * (7 * N + 1) + P * prods * (2 * A ** 3 + A * A) + 5 * P * A)
+    return {"sample": sample, "eval": ev, "total": sample + ev}
+
+
+# --------------------------------------------------------------------------------------
+# CPU baseline (oracle = FP64 NumPy restatement of the reference algorithm)
+# --------------------------------------------------------------------------------------
+def _cpu_worker(args):
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    vib_path, rho_path, T, P, B, seconds, seed = args
+    import numpy as np
+    from oracle import pimc_oracle as po
+    A, N, E, w, L, Q = po.read_model_json(vib_path)
+    Ar, Nr, Er, wr, Lr, _ = po.read_model_json(rho_path, diagonal=True)
+    s = po.Setup((E, w, L, Q), (Er, wr, Lr), T, P)
+    rng = np.random.default_rng(seed)
+    n, t0 = 0, time.perf_counter()
+    sw, sw2 = 0.0, 0.0
+    while True:
+        q, _ = s.draw_q(rng, B)
+        r = po.evaluate(s, q)
+        ratio = np.abs(r["s_g"] / r["s_rho"])
+        sw += ratio.sum()
+        sw2 += (ratio * ratio).sum()
+        n += B
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            return n, el, sw, sw2
+
+
+def cpu_baseline(vib_path, rho_path, T, P, seconds, cores, B=500):
+    import multiprocessing as mp
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        os.environ[k] = "1"
+    ctx = mp.get_context("spawn")
+    jobs = [(vib_path, rho_path, T, P, B, seconds, 1000 + i) for i in range(cores)]
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    sbs = sum(n * P / el for n, el, _, _ in res)
+    one = res[0][0] * P / res[0][1]
+    n_tot = sum(r[0] for r in res)
+    sw = sum(r[2] for r in res)
+    sw2 = sum(r[3] for r in res)
+    import numpy as np
+    return dict(samples_beads_per_s=sbs, one_core=one, samples=n_tot, ess_frac=(sw * sw / sw2) / n_tot,
+                seconds=max(r[1] for r in res), numpy=np.__version__)
+
+
+# --------------------------------------------------------------------------------------
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--block", type=int, default=100_000)
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per sample (0 = automatic)")
+    ap.add_argument("--pm", type=int, default=0, help="1: block_compute_pm (g at beta +- dbeta)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-cores", type=int, default=0, help="0 = min(16, os.cpu_count())")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=2024)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not "
+                                   "supported (samples shard trivially; see DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    from pibronic_amd import pimc, synthetic, _native
+    name = "A4_N8_P64_T300"
+    cfg = synthetic.WORKLOADS[name]
+    A, N, P, T = cfg["A"], cfg["N"], cfg["P"], cfg["T"]
+    B = args.block
+    pm = bool(args.pm)
+    tmp = tempfile.mkdtemp(prefix="pib_bench_")
+    FS = synthetic.write_workload(tmp, name)
+
+    cpu = None
+    if not args.no_cpu:
+        cores = args.cpu_cores or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(FS.path_vib_model, FS.path_rho_model, T, P, args.cpu_seconds, cores)
+        cpu["cores"] = cores
+
+    import torch
+    data = (pimc.BoxDataPM if pm else pimc.BoxData).from_FileStructure(FS)
+    data.samples, data.beads, data.temperature = B * (args.steps + args.warmup), P, T
+    data.block_size, data.blocks = B, args.steps + args.warmup
+    data.seed = args.seed
+    data.preprocess()
+    ctx = data.context()
+    if args.lanes:
+        ctx.set_lanes_per_sample(args.lanes)
+    lanes = ctx.lanes_per_sample(B)
+    dev = data.device_buffers(B)
+    stream = torch.cuda.current_stream()
+    K, W = args.steps, args.warmup
+    sums = torch.zeros((K + W, _native.NUM_SUMS), dtype=torch.float64, device="cuda")
+    flags = _native.EVAL_PM if pm else 0
+    ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+
+    def step(i, timed):
+        ctx.sample_block(args.seed, i * B, B, dev["q"], None, stream)
+        if timed:
+            ev0[i - W].record(stream)
+        ctx.eval_block(dev["q"], B, flags, dev["out"], dev["out"].shape[1], None, stream)
+        if timed:
+            ev1[i - W].record(stream)
+        ctx.block_sums(dev["out"], dev["out"].shape[1], B, pm, data.delta_beta, sums[i], stream)
+
+    for i in range(W):
+        step(i, False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(W, W + K):
+        step(i, True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    print(f"[bench] {K} steps of {B} samples in {elapsed:.4f} s", file=sys.stderr, flush=True)
+
+    s = sums[W:].sum(dim=0).cpu().numpy()
+    n = s[_native.SUM_N]
+    ess = s[_native.SUM_ABSR] ** 2 / s[_native.SUM_R2]
+    mean_w = s[_native.SUM_ABSR] / n
+    var_w = s[_native.SUM_R2] / n - mean_w ** 2
+    cv2 = var_w / mean_w ** 2
+    n_needed = cv2 / 1e-6
+    samples_per_s = n / elapsed
+    eval_ms = sum(a.elapsed_time(b) for a, b in zip(ev0, ev1)) / K
+    nq = int(np.count_nonzero(np.abs(data.vib.quadratic).sum(axis=(2, 3))))
+    fl = algorithmic_flops(A, N, P, nq, pm)
+    achieved = fl["eval"] * B / (eval_ms * 1e-3) / 1e12
+    line = {
+        "metric": "effective samples/sec at fixed Z-ratio std error; learned ρ vs harmonic ρ",
+        "value": ess / elapsed,
+        "unit": "effective samples/s (harmonic rho)",
+        "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": elapsed / K * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded model, device Philox samples)",
+        "config": {"workload": f"{name}: A={A} N={N} P={P} T={T:g}K block={B}"
+                               + (" +-dbeta" if pm else ""),
+                   "samples_per_step": B, "lanes_per_sample": lanes},
+        "samples_beads_per_s": n * P / elapsed,
+        "samples_per_s": samples_per_s,
+        "ess_fraction": ess / n,
+        "negative_fraction": s[_native.SUM_NEG] / n,
+        "z_ratio_rel_err_1e-3_samples": n_needed,
+        "time_to_1e-3_s": n_needed / samples_per_s,
+        "eval_kernel_ms": eval_ms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_eval_fp64", "flops_per_sample": fl["eval"]},
+        "cpu_baseline": None,
+    }
+    if cpu is not None:
+        line["cpu_baseline"] = {
+            "value": cpu["samples_beads_per_s"], "unit": "samples*beads/s",
+            "cores": cpu["cores"], "kind": "port",
+            "sample": f"oracle (FP64 NumPy restatement, per-(sample,bead) dot loop) on the same "
+                      f"model, {cpu['samples']} samples in blocks of 500 over {cpu['seconds']:.1f} s, "
+                      f"one process per core, numpy {cpu['numpy']}",
+            "one_core_value": cpu["one_core"],
+            "effective_samples_per_s": cpu["samples_beads_per_s"] / P * cpu["ess_frac"],
+            "gpu_over_cpu": (n * P / elapsed) / cpu["samples_beads_per_s"],
+        }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
