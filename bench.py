@@ -40,6 +40,33 @@ def algorithmic_flops(A, N, P, n_quad_pairs, pm=False):
     return {"sample": sample, "eval": ev, "total": sample + ev}
 
 
+def profiled_traffic(path, kernel_prefix):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 summary: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch), each divided by what the same counter reads for a plain
+    256 MiB device copy in the same run (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x on MI355X).
+    Returns (bytes, detail) or (None, reason)."""
+    if not os.path.exists(path):
+        return None, "no profile"
+    rows = {}
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0].startswith("`"):
+            try:
+                rows[(cells[0].strip("`"), cells[1])] = (float(cells[2]), float(cells[3]))
+            except ValueError:
+                pass
+    copy = [v for (k, g), v in rows.items() if k.startswith("__amd_rocclr_copyBuffer")]
+    kern = [v for (k, g), v in rows.items() if k.startswith(kernel_prefix)]
+    if not copy or not kern:
+        return None, "kernel or calibration copy missing from profile"
+    ref_kib = 256 * 1024.0
+    f_fetch, f_write = copy[0][0] / ref_kib, copy[0][1] / ref_kib
+    fetch, write = kern[0][0] / f_fetch, kern[0][1] / f_write
+    return (fetch + write) * 1024.0, {"fetch_kib_raw": kern[0][0], "write_kib_raw": kern[0][1],
+                                      "fetch_factor": f_fetch, "write_factor": f_write,
+                                      "profile": os.path.relpath(path, ROOT)}
+
+
 # --------------------------------------------------------------------------------------
 # CPU baseline (oracle = FP64 NumPy restatement of the reference algorithm)
 # --------------------------------------------------------------------------------------
@@ -97,6 +124,9 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0, help="0 = min(16, os.cpu_count())")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the timed region, copy a buffer of this many MiB 3 times "
+                         "(calibrates profiler byte counters against a known transfer)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -158,6 +188,15 @@ def main():
     elapsed = time.perf_counter() - t0
     print(f"[bench] {K} steps of {B} samples in {elapsed:.4f} s", file=sys.stderr, flush=True)
 
+    if args.calib_copy_mib:
+        nel = args.calib_copy_mib * (1 << 20) // 8
+        src_t = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst_t = torch.empty_like(src_t)
+        for _ in range(3):
+            dst_t.copy_(src_t)
+        torch.cuda.synchronize()
+        del src_t, dst_t
+
     s = sums[W:].sum(dim=0).cpu().numpy()
     n = s[_native.SUM_N]
     ess = s[_native.SUM_ABSR] ** 2 / s[_native.SUM_R2]
@@ -170,6 +209,11 @@ def main():
     nq = int(np.count_nonzero(np.abs(data.vib.quadratic).sum(axis=(2, 3))))
     fl = algorithmic_flops(A, N, P, nq, pm)
     achieved = fl["eval"] * B / (eval_ms * 1e-3) / 1e12
+    traffic, tdetail = None, "profile only for the default workload"
+    if not pm and B == 100_000:
+        traffic, tdetail = profiled_traffic(
+            os.path.join(ROOT, "profiles", f"k_eval_fp64_A{A}_N{N}_P{P}_B{B}.md"),
+            f"void pib::k_eval_fp64<{A}, {A}, false, false>")
     line = {
         "metric": "effective samples/sec at fixed Z-ratio std error; learned ρ vs harmonic ρ",
         "value": ess / elapsed,
@@ -189,8 +233,10 @@ def main():
         "time_to_1e-3_s": n_needed / samples_per_s,
         "eval_kernel_ms": eval_ms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_eval_fp64", "flops_per_sample": fl["eval"]},
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "k_eval_fp64", "flops_per_sample": fl["eval"],
+                     "algorithmic_bytes": B * N * P * 8 + B * (5 if pm else 3) * 8,
+                     "traffic_detail": tdetail},
         "cpu_baseline": None,
     }
     if cpu is not None:

@@ -118,12 +118,14 @@ int pib_eval_block(pib_ctx* ctx, const double* q_dev, int64_t B, int flags, doub
                    int64_t out_stride, double* debug_dev, void* stream);
 
 /* Sampling constants of the harmonic rho: surface weights weights_host (A_rho, normalised,
- * ModelSampling.compute_weight_for_each_state, pibronic/pimc/pimc.py:341-351), standard
- * deviations sigma_host (A_rho*N*P, :401-404), state shift (A_rho*N, :372-381) and the
- * circulant eigenvectors circ_eigvects_host (P*P, row p = bead, :684-689). */
+ * ModelSampling.compute_weight_for_each_state, pibronic/pimc/pimc.py:341-351), the
+ * sampling model's coth_host and csch_host (N, its TemperatureDependentClass at tau) and its
+ * state shift (A_rho*N, :372-381). Together with P they define the Gaussian of
+ * compute_sampling_constants (:398-406): per mode, precision 2 coth I - csch C with C the
+ * ring adjacency matrix of the circulant basis (:681-689). The library factors it once. */
 int pib_set_sampler(pib_ctx* ctx, int A_rho, int N, int P, const double* weights_host,
-                    const double* sigma_host, const double* state_shift_host,
-                    const double* circ_eigvects_host);
+                    const double* coth_host, const double* csch_host,
+                    const double* state_shift_host);
 
 /* Draw B samples of bead coordinates into q_dev (B*N*P) and, optionally, their surface
  * sources into src_dev (B int32). Replaces draw_sample + transform_sampled_coordinates

@@ -157,11 +157,11 @@ class Context:
             self._check(r)
         return r
 
-    def set_sampler(self, weights, sigma, state_shift, circ_eigvects):
-        w, s, d, U = _host(weights), _host(sigma), _host(state_shift), _host(circ_eigvects)
-        A_rho, N, P = s.shape
-        self._check(self.lib.pib_set_sampler(self.h, A_rho, N, P, _dptr(w), _dptr(s),
-                                             _dptr(d), _dptr(U)))
+    def set_sampler(self, P, weights, coth, csch, state_shift):
+        w, c, s, d = _host(weights), _host(coth), _host(csch), _host(state_shift)
+        A_rho, N = d.shape
+        self._check(self.lib.pib_set_sampler(self.h, A_rho, N, int(P), _dptr(w), _dptr(c),
+                                             _dptr(s), _dptr(d)))
 
     # ------------------------------------------------------------------ per block
     def eval_block(self, q, B, flags, out, out_stride, debug=None, stream=None):

@@ -42,7 +42,7 @@ struct pib_ctx {
   int lanes = 0;
   // sampler
   int s_A = 0, s_N = 0, s_P = 0;
-  DevBuf s_cdf, s_sigma, s_shift, s_Ut;
+  DevBuf s_cdf, s_chol, s_shift;
   bool sampler_set = false;
   // reduction scratch
   DevBuf partials;
@@ -100,9 +100,10 @@ bool finite_all(const double* v, size_t n) {
 }
 
 int auto_lanes(const pib_ctx* c, int64_t B) {
-  // enough lanes in flight to cover every SIMD with a few waves; never more lanes per
-  // sample than beads, so that each lane owns at least one bead
-  const int64_t target = (int64_t)c->num_cu * 4 * kWave * 2;
+  // enough lanes in flight for ~24 waves per SIMD over the launch (measured on MI355X at
+  // A=4, N=8, P=64: 16 lanes per sample for blocks of 1e5 samples is the fastest split);
+  // never more lanes per sample than beads, so that each lane owns at least one bead
+  const int64_t target = (int64_t)c->num_cu * 4 * kWave * 24;
   int maxg = 1;
   while (maxg < 64 && maxg * 2 <= c->P) maxg *= 2;
   int g = 1;
@@ -150,9 +151,8 @@ int pib_ctx_destroy(pib_ctx* c) {
     release(c->shift[s]);
   }
   release(c->s_cdf);
-  release(c->s_sigma);
+  release(c->s_chol);
   release(c->s_shift);
-  release(c->s_Ut);
   release(c->partials);
   delete c;
   return PIB_OK;
@@ -305,14 +305,14 @@ int pib_eval_block(pib_ctx* c, const double* q, int64_t B, int flags, double* ou
   return PIB_OK;
 }
 
-int pib_set_sampler(pib_ctx* c, int A_rho, int N, int P, const double* w, const double* sigma,
-                    const double* shift, const double* U) {
+int pib_set_sampler(pib_ctx* c, int A_rho, int N, int P, const double* w, const double* coth,
+                    const double* csch, const double* shift) {
   if (!c) return PIB_ERR_ARG;
-  if (A_rho < 1 || N < 1 || P < 3 || !w || !sigma || !shift || !U)
+  if (A_rho < 1 || N < 1 || P < 3 || !w || !coth || !csch || !shift)
     return fail(c, PIB_ERR_ARG, "set_sampler: bad size or null array");
   if (P > 4096) return fail(c, PIB_ERR_ARG, "set_sampler: P > 4096 not supported");
-  if (!finite_all(w, A_rho) || !finite_all(sigma, (size_t)A_rho * N * P) ||
-      !finite_all(shift, (size_t)A_rho * N) || !finite_all(U, (size_t)P * P))
+  if (!finite_all(w, A_rho) || !finite_all(coth, N) || !finite_all(csch, N) ||
+      !finite_all(shift, (size_t)A_rho * N))
     return fail(c, PIB_ERR_ARG, "set_sampler: non-finite constant");
   std::vector<double> cdf(A_rho);
   double tot = 0.0;
@@ -327,14 +327,37 @@ int pib_set_sampler(pib_ctx* c, int A_rho, int N, int P, const double* w, const 
     cdf[a] = run / tot;
   }
   cdf[A_rho - 1] = 1.0;
-  std::vector<double> Ut((size_t)P * P);
-  for (int p = 0; p < P; ++p)
-    for (int k = 0; k < P; ++k) Ut[(size_t)k * P + p] = U[(size_t)p * P + k];
+  // Cholesky factor of the cyclic tridiagonal precision matrix of each mode:
+  // Lambda = alpha I + beta C, alpha = 2 coth, beta = -csch, C = ring adjacency (ones at
+  // +-1 mod P). Non-zeros of L: diagonal, subdiagonal (k+1, k) for k <= P-3, last row.
+  std::vector<double> chol((size_t)N * 3 * P, 0.0);
+  for (int j = 0; j < N; ++j) {
+    const double alpha = 2.0 * coth[j], beta = -csch[j];
+    double* dinv = chol.data() + (size_t)j * 3 * P;
+    double* sub = dinv + P;
+    double* last = sub + P;
+    std::vector<double> diag(P);
+    double lastsq = 0.0;
+    for (int k = 0; k < P - 1; ++k) {
+      const double ps = k >= 1 ? sub[k - 1] : 0.0;
+      const double dd = alpha - ps * ps;
+      if (!(dd > 0.0)) return fail(c, PIB_ERR_ARG, "set_sampler: precision matrix is not positive definite");
+      diag[k] = std::sqrt(dd);
+      if (k <= P - 3) sub[k] = beta / diag[k];
+      const double lam = (k == 0 || k == P - 2) ? beta : 0.0;
+      const double pl = k >= 1 ? last[k - 1] * sub[k - 1] : 0.0;
+      last[k] = (lam - pl) / diag[k];
+      lastsq += last[k] * last[k];
+    }
+    const double dl = alpha - lastsq;
+    if (!(dl > 0.0)) return fail(c, PIB_ERR_ARG, "set_sampler: precision matrix is not positive definite");
+    diag[P - 1] = std::sqrt(dl);
+    for (int k = 0; k < P; ++k) dinv[k] = 1.0 / diag[k];
+  }
   int r;
   if ((r = upload(c, c->s_cdf, cdf.data(), cdf.size() * sizeof(double)))) return r;
-  if ((r = upload(c, c->s_sigma, sigma, (size_t)A_rho * N * P * sizeof(double)))) return r;
+  if ((r = upload(c, c->s_chol, chol.data(), chol.size() * sizeof(double)))) return r;
   if ((r = upload(c, c->s_shift, shift, (size_t)A_rho * N * sizeof(double)))) return r;
-  if ((r = upload(c, c->s_Ut, Ut.data(), Ut.size() * sizeof(double)))) return r;
   c->s_A = A_rho;
   c->s_N = N;
   c->s_P = P;
@@ -358,9 +381,8 @@ int pib_sample_block(pib_ctx* c, uint64_t seed, int64_t offset, int64_t B, doubl
   a.key0 = (uint32_t)seed;
   a.key1 = (uint32_t)(seed >> 32);
   a.cdf = (const double*)c->s_cdf.p;
-  a.sigma = (const double*)c->s_sigma.p;
+  a.chol = (const double*)c->s_chol.p;
   a.shift = (const double*)c->s_shift.p;
-  a.Ut = (const double*)c->s_Ut.p;
   a.q = q;
   a.src = src;
   PIB_HIP(c, hipSetDevice(c->device));

@@ -35,40 +35,43 @@ __device__ __forceinline__ double& UT(double (&a)[A][A], int i, int j) {
 
 template <int A>
 __device__ __forceinline__ void jacobi_rotate(double (&a)[A][A], double (&u)[A][A], int p, int q) {
+  // t = tan(phi) of the rotation annihilating a_pq: the root of smaller magnitude of
+  // t^2 + 2 t (a_qq - a_pp)/(2 a_pq) - 1 = 0, written with one division and one sqrt.
+  // Branch-free: a_pq == 0 gives t = 0, c = 1, s = 0, an exact no-op.
   const double apq = a[p][q];
-  if (apq != 0.0) {
-    const double app = a[p][p], aqq = a[q][q];
-    const double theta = (aqq - app) / (2.0 * apq);
-    double t = 1.0 / (fabs(theta) + sqrt(fma(theta, theta, 1.0)));
-    t = theta < 0.0 ? -t : t;
-    const double c = 1.0 / sqrt(fma(t, t, 1.0));
-    const double s = t * c;
-    const double tr = s / (1.0 + c);
-    a[p][p] = fma(-t, apq, app);
-    a[q][q] = fma(t, apq, aqq);
-    a[p][q] = 0.0;
+  const double d = a[q][q] - a[p][p];
+  const double h = 2.0 * apq;
+  const double r = sqrt(fma(d, d, h * h));
+  const double den = fabs(d) + r;
+  double t = den > 0.0 ? h / den : 0.0;
+  t = d < 0.0 ? -t : t;
+  const double c = rsqrt(fma(t, t, 1.0));
+  const double s = t * c;
+  a[p][p] = fma(-t, apq, a[p][p]);
+  a[q][q] = fma(t, apq, a[q][q]);
+  a[p][q] = 0.0;
 #pragma unroll
-    for (int r = 0; r < A; ++r) {
-      if (r == p || r == q) continue;
-      const double g = UT<A>(a, r, p), h = UT<A>(a, r, q);
-      UT<A>(a, r, p) = g - s * fma(g, tr, h);
-      UT<A>(a, r, q) = h + s * fma(-h, tr, g);
-    }
+  for (int r_ = 0; r_ < A; ++r_) {
+    if (r_ == p || r_ == q) continue;
+    const double g = UT<A>(a, r_, p), hh = UT<A>(a, r_, q);
+    UT<A>(a, r_, p) = fma(c, g, -s * hh);
+    UT<A>(a, r_, q) = fma(s, g, c * hh);
+  }
 #pragma unroll
-    for (int r = 0; r < A; ++r) {
-      const double g = u[r][p], h = u[r][q];
-      u[r][p] = g - s * fma(g, tr, h);
-      u[r][q] = h + s * fma(-h, tr, g);
-    }
+  for (int r_ = 0; r_ < A; ++r_) {
+    const double g = u[r_][p], hh = u[r_][q];
+    u[r_][p] = fma(c, g, -s * hh);
+    u[r_][q] = fma(s, g, c * hh);
   }
 }
 
+// Symmetric eigen-decomposition by cyclic Jacobi sweeps, started from the orthogonal basis
+// held in u (identity, or the previous bead's eigenvectors: adjacent beads of a ring polymer
+// have close coupling matrices, so the rotated matrix u^T V u is nearly diagonal and fewer
+// sweeps are needed). On entry a holds V (upper triangle); on exit the diagonal of a holds
+// the eigenvalues and V = u diag(a) u^T.
 template <int A>
-__device__ __forceinline__ void jacobi_eigh(double (&a)[A][A], double (&u)[A][A]) {
-#pragma unroll
-  for (int i = 0; i < A; ++i)
-#pragma unroll
-    for (int j = 0; j < A; ++j) u[i][j] = (i == j) ? 1.0 : 0.0;
+__device__ __forceinline__ void jacobi_eigh(double (&a)[A][A], double (&u)[A][A], bool warm) {
   if (A == 1) return;
   double fro2 = 0.0;
 #pragma unroll
@@ -76,6 +79,28 @@ __device__ __forceinline__ void jacobi_eigh(double (&a)[A][A], double (&u)[A][A]
     fro2 = fma(a[i][i], a[i][i], fro2);
 #pragma unroll
     for (int j = i + 1; j < A; ++j) fro2 = fma(2.0 * a[i][j], a[i][j], fro2);
+  }
+  if (warm) {
+    // a <- u^T a u  (a symmetric; only the upper triangle is formed)
+    double t[A][A];
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int k = 0; k < A; ++k) {
+        double acc = UT<A>(a, i, 0) * u[0][k];
+#pragma unroll
+        for (int m = 1; m < A; ++m) acc = fma(UT<A>(a, i, m), u[m][k], acc);
+        t[i][k] = acc;  // (V u)[i][k]
+      }
+#pragma unroll
+    for (int k = 0; k < A; ++k)
+#pragma unroll
+      for (int l = k; l < A; ++l) {
+        double acc = u[0][k] * t[0][l];
+#pragma unroll
+        for (int m = 1; m < A; ++m) acc = fma(u[m][k], t[m][l], acc);
+        a[k][l] = acc;
+      }
   }
   // relative off-diagonal norm 1e-15: the residual perturbs M = exp(-tau V) by about
   // tau*|V|*1e-15, far below the parity bound; quadratic convergence reaches it in a few sweeps
@@ -87,14 +112,13 @@ __device__ __forceinline__ void jacobi_eigh(double (&a)[A][A], double (&u)[A][A]
     for (int i = 0; i < A; ++i)
 #pragma unroll
       for (int j = i + 1; j < A; ++j) off = fma(a[i][j], a[i][j], off);
-    const bool done = !(off > tol2);
-    if (__all(done)) break;
-    if (!done) {
+    // the sweep count is uniform over the wave: converged lanes keep rotating (harmlessly)
+    // until every lane of the wave has converged, which keeps the sweep branch-free
+    if (__all(!(off > tol2))) break;
 #pragma unroll
-      for (int p = 0; p < A - 1; ++p)
+    for (int p = 0; p < A - 1; ++p)
 #pragma unroll
-        for (int q = p + 1; q < A; ++q) jacobi_rotate<A>(a, u, p, q);
-    }
+      for (int q = p + 1; q < A; ++q) jacobi_rotate<A>(a, u, p, q);
   }
 }
 
@@ -104,7 +128,7 @@ __device__ __forceinline__ double oterm(double coth, double csch2, double x1, do
   return coth * (x1 * x1 + x2 * x2) - (csch2 * x1) * x2;
 }
 
-template <int A, int AR, bool PM>
+template <int A, int AR, bool PM, bool DBG>
 __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
   constexpr bool SEP = (AR != A);  // qTempTensor path (pibronic/pimc/pimc.py:1094-1096)
   const int lane = threadIdx.x & (kWave - 1);
@@ -115,11 +139,22 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
   const bool live = sample < args.B;
   const double* __restrict__ qs = args.q + (live ? sample : 0) * (int64_t)N * P;
 
-  const OSet sr = args.set[SET_RHO], sv = args.set[SET_VIB];
-  const OSet sp = args.set[SET_VIB_PLUS], sm = args.set[SET_VIB_MINUS];
+  // read-only constants: restrict-qualified so the compiler can keep them in scalar loads
+  struct RO {
+    const double* __restrict__ coth;
+    const double* __restrict__ csch;
+    const double* __restrict__ pref;
+    const double* __restrict__ shift;
+  };
+  const RO sr{args.set[SET_RHO].coth, args.set[SET_RHO].csch, args.set[SET_RHO].pref, args.set[SET_RHO].shift};
+  const RO sv{args.set[SET_VIB].coth, args.set[SET_VIB].csch, args.set[SET_VIB].pref, args.set[SET_VIB].shift};
+  const RO sp{args.set[SET_VIB_PLUS].coth, args.set[SET_VIB_PLUS].csch, args.set[SET_VIB_PLUS].pref, args.set[SET_VIB_PLUS].shift};
+  const RO sm{args.set[SET_VIB_MINUS].coth, args.set[SET_VIB_MINUS].csch, args.set[SET_VIB_MINUS].pref, args.set[SET_VIB_MINUS].shift};
   const double* __restrict__ Ev = args.energy;
   const double* __restrict__ Lv = args.linear;
   const double* __restrict__ Qd = args.qdiag;
+  const int* __restrict__ pairs = args.pair;
+  const double* __restrict__ qpair = args.qpair;
 
   double Ng[A][A], Np[A][A], Nm[A][A];
 #pragma unroll
@@ -132,14 +167,22 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
   double rp[AR];
 #pragma unroll
   for (int a = 0; a < AR; ++a) rp[a] = 1.0;
-  double ls = 0.0;
+  // prod_p S_p kept as mantissa * 2^exponent; one log per lane at the end
+  double smant = 1.0;
+  int sexp = 0;
+  // eigenvectors of the previous bead: warm start of the next Jacobi
+  double u[A][A];
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = 0; k < A; ++k) u[i][k] = (i == k) ? 1.0 : 0.0;
 
   const int pbeg = gl * K;
   const int pend = live ? min(pbeg + K, P) : pbeg;
   for (int p = pbeg; p < pend; ++p) {
     const int pn = (p + 1 == P) ? 0 : p + 1;
     double er[AR], ev[A], ep[A], em[A];
-    double vq[A][A], vl[A][A];
+    double v[A][A];
     for (int j = 0; j < N; ++j) {
       const double x0 = qs[j * P + p], xn = qs[j * P + pn];
       // sampling model: q -= d_rho; O_rho; q += d_rho (pimc.py:1091-1127)
@@ -189,15 +232,14 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
       for (int b = 0; b < A; ++b)
 #pragma unroll
         for (int c = 0; c <= b; ++c) {
-          const double lin = Lv[(j * A + b) * A + c] * q2[b];
-          vl[b][c] = (j == 0) ? lin : vl[b][c] + lin;
-          const double qd = Qd ? (q2[c] * Qd[(j * A + b) * A + c]) * q2[b] : 0.0;
-          vq[b][c] = (j == 0) ? qd : vq[b][c] + qd;
+          double term = Lv[(j * A + b) * A + c] * q2[b];
+          if (Qd) term = fma(q2[c] * Qd[(j * A + b) * A + c], q2[b], term);
+          v[b][c] = (j == 0) ? term : v[b][c] + term;
         }
     }
     // off-diagonal quadratic mode pairs (j != k): recompute the V-stage coordinate copies
     for (int t = 0; t < args.npair; ++t) {
-      const int jj = args.pair[2 * t], kk = args.pair[2 * t + 1];
+      const int jj = pairs[2 * t], kk = pairs[2 * t + 1];
       double qj[A], qk[A];
 #pragma unroll
       for (int a = 0; a < A; ++a) {
@@ -211,11 +253,11 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
         qj[a] = (vj - ej) + ej;
         qk[a] = (vk - ek) + ek;
       }
-      const double* Qh = args.qpair + (int64_t)t * A * A;
+      const double* Qh = qpair + (int64_t)t * A * A;
 #pragma unroll
       for (int b = 0; b < A; ++b)
 #pragma unroll
-        for (int c = 0; c <= b; ++c) vq[b][c] += (qk[c] * Qh[b * A + c]) * qj[b];
+        for (int c = 0; c <= b; ++c) v[b][c] = fma(qk[c] * Qh[b * A + c], qj[b], v[b][c]);
     }
 
     // O-factors, per-bead scaling S_p = max over both O's (pimc.py:1076-1084)
@@ -238,12 +280,12 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
     if (args.noscale) S = 1.0;
 
     // coupling matrix -> Jacobi -> M = U exp(-tau lambda) U^T (pimc.py:1171-1187)
-    double a_[A][A], u[A][A];
+    double a_[A][A];
 #pragma unroll
     for (int b = 0; b < A; ++b)
 #pragma unroll
-      for (int c = 0; c <= b; ++c) a_[c][b] = (vq[b][c] + vl[b][c]) + Ev[b * A + c];
-    jacobi_eigh<A>(a_, u);
+      for (int c = 0; c <= b; ++c) a_[c][b] = v[b][c] + Ev[b * A + c];
+    jacobi_eigh<A>(a_, u, p != pbeg);
     double ex[A];
 #pragma unroll
     for (int d = 0; d < A; ++d) ex[d] = exp(-args.tau * a_[d][d]);
@@ -259,7 +301,7 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
         M[e][c] = s;
       }
 
-    if (args.dbg) {
+    if (DBG) {
       const int rec = eval_debug_record(A, AR);
       double* dr = args.dbg + ((int64_t)sample * P + p) * rec;
 #pragma unroll
@@ -274,14 +316,19 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
     }
 
     // scale (pimc.py:1062-1066, 1445, 1448), then accumulate rho and the ring products
+    const double inv = 1.0 / S;
 #pragma unroll
-    for (int a = 0; a < AR; ++a) rp[a] *= Or[a] / S;
+    for (int a = 0; a < AR; ++a) rp[a] *= Or[a] * inv;
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-      Ov[a] = Ov[a] / S;
-      if (PM) { Op[a] = Op[a] / S; Om[a] = Om[a] / S; }
+      Ov[a] *= inv;
+      if (PM) { Op[a] *= inv; Om[a] *= inv; }
     }
-    ls += log(S);
+    {
+      int e;
+      smant = frexp(smant * S, &e);
+      sexp += e;
+    }
 #pragma unroll
     for (int i = 0; i < A; ++i) {
       double rg[A], rpl[A], rmi[A];
@@ -309,6 +356,8 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
       }
     }
   }
+
+  double ls = log(smant) + (double)sexp * 0.69314718055994530942;
 
   // ordered tree over the G lanes of a sample: lane g (g % 2s == 0) absorbs lane g + s
   for (int s = 1; s < G; s <<= 1) {
@@ -394,11 +443,16 @@ static hipError_t launch_a(int pm, const EvalArgs& a, hipStream_t s) {
   const int spw = kWave / a.G;
   const int64_t waves = (a.B + spw - 1) / spw;
   const int threads = 256;
-  const int64_t blocks = (waves * kWave + threads - 1) / threads;
-  if (pm)
-    hipLaunchKernelGGL((k_eval_fp64<A, AR, true>), dim3((unsigned)blocks), dim3(threads), 0, s, a);
+  const dim3 grid((unsigned)((waves * kWave + threads - 1) / threads));
+  const bool dbg = a.dbg != nullptr;
+  if (pm && dbg)
+    hipLaunchKernelGGL((k_eval_fp64<A, AR, true, true>), grid, dim3(threads), 0, s, a);
+  else if (pm)
+    hipLaunchKernelGGL((k_eval_fp64<A, AR, true, false>), grid, dim3(threads), 0, s, a);
+  else if (dbg)
+    hipLaunchKernelGGL((k_eval_fp64<A, AR, false, true>), grid, dim3(threads), 0, s, a);
   else
-    hipLaunchKernelGGL((k_eval_fp64<A, AR, false>), dim3((unsigned)blocks), dim3(threads), 0, s, a);
+    hipLaunchKernelGGL((k_eval_fp64<A, AR, false, false>), grid, dim3(threads), 0, s, a);
   return hipGetLastError();
 }
 

@@ -47,9 +47,9 @@ struct SampleArgs {
   int64_t offset;         // global index of this block's first sample (Philox counter)
   uint32_t key0, key1;    // Philox key (seed)
   const double* cdf;      // A_rho cumulative surface weights
-  const double* sigma;    // A_rho * N * P standard deviations of the normal-mode draws
+  const double* chol;     // N * 3P: per mode 1/diag(L), subdiagonal of L, last row of L, where
+                          // L L^T = 2 coth_j I - csch_j C (cyclic tridiagonal Cholesky)
   const double* shift;    // A_rho * N sampling-model state shift
-  const double* Ut;       // P * P circulant eigenvectors, transposed: Ut[k*P + p] = U[p][k]
   double* q;              // (B, N, P) output
   int32_t* src;           // (B,) surface sources, optional
 };

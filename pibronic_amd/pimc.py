@@ -371,8 +371,8 @@ class BoxData:
                 ctx.set_omatrix_set(slot, c.cothAN[0], c.cschAN[0], c.omatrix_prefactor,
                                     vib.state_shift)
         ctx.set_beads(self.beads, self.tau)
-        ctx.set_sampler(rho.state_weight, rho.standard_deviation, rho.state_shift,
-                        self.circulant_eigvects)
+        ctx.set_sampler(self.beads, rho.state_weight, rho.const.cothAN[0], rho.const.cschAN[0],
+                        rho.state_shift)
         self._ctx = ctx
         return ctx
 
