@@ -33,19 +33,44 @@ __device__ __forceinline__ double& UT(double (&a)[A][A], int i, int j) {
   return i <= j ? a[i][j] : a[j][i];
 }
 
+// 1/x and 1/sqrt(x) from the hardware approximations refined by two Newton steps each
+// (relative error ~1 ulp). Used only inside Jacobi rotations, whose orthogonality depends on
+// c^2 + s^2 = 1 (kept to ~1 ulp), not on t being correctly rounded.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-(x * y), y, 1.0);  // 1 - x y^2
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
 template <int A>
 __device__ __forceinline__ void jacobi_rotate(double (&a)[A][A], double (&u)[A][A], int p, int q) {
   // t = tan(phi) of the rotation annihilating a_pq: the root of smaller magnitude of
-  // t^2 + 2 t (a_qq - a_pp)/(2 a_pq) - 1 = 0, written with one division and one sqrt.
-  // Branch-free: a_pq == 0 gives t = 0, c = 1, s = 0, an exact no-op.
+  // t^2 + 2 t (a_qq - a_pp)/(2 a_pq) - 1 = 0, i.e. t = sgn(d) h / (|d| + sqrt(d^2 + h^2)),
+  // d = a_qq - a_pp, h = 2 a_pq; |t| <= 1. Branch-free: a_pq == 0 gives t = 0, c = 1, s = 0,
+  // an exact no-op.
   const double apq = a[p][q];
   const double d = a[q][q] - a[p][p];
   const double h = 2.0 * apq;
-  const double r = sqrt(fma(d, d, h * h));
+  // safe arguments instead of branches: x == 0 gives r = 0 * rsq(1) = 0, and den == 0
+  // implies h == 0, so t = 0 * rcp(1) = 0
+  const double x = fma(d, d, h * h);
+  const double r = x * rsq_nr(x > 0.0 ? x : 1.0);
   const double den = fabs(d) + r;
-  double t = den > 0.0 ? h / den : 0.0;
+  double t = h * rcp_nr(den > 0.0 ? den : 1.0);
   t = d < 0.0 ? -t : t;
-  const double c = rsqrt(fma(t, t, 1.0));
+  const double c = rsq_nr(fma(t, t, 1.0));
   const double s = t * c;
   a[p][p] = fma(-t, apq, a[p][p]);
   a[q][q] = fma(t, apq, a[q][q]);
@@ -65,13 +90,17 @@ __device__ __forceinline__ void jacobi_rotate(double (&a)[A][A], double (&u)[A][
   }
 }
 
-// Symmetric eigen-decomposition by cyclic Jacobi sweeps, started from the orthogonal basis
-// held in u (identity, or the previous bead's eigenvectors: adjacent beads of a ring polymer
-// have close coupling matrices, so the rotated matrix u^T V u is nearly diagonal and fewer
-// sweeps are needed). On entry a holds V (upper triangle); on exit the diagonal of a holds
-// the eigenvalues and V = u diag(a) u^T.
+// Symmetric eigen-decomposition by cyclic Jacobi sweeps. On entry a holds V (upper
+// triangle); on exit its diagonal holds the eigenvalues and V = u diag(a) u^T. (A warm start
+// from the previous bead's eigenvectors saves about one sweep on average, but it keeps a
+// second A x A basis live across the mode loop; on gfx950 the lost occupancy costs more than
+// the sweep, so every bead starts from the identity.)
 template <int A>
-__device__ __forceinline__ void jacobi_eigh(double (&a)[A][A], double (&u)[A][A], bool warm) {
+__device__ __forceinline__ void jacobi_eigh(double (&a)[A][A], double (&u)[A][A]) {
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = 0; k < A; ++k) u[i][k] = (i == k) ? 1.0 : 0.0;
   if (A == 1) return;
   double fro2 = 0.0;
 #pragma unroll
@@ -80,31 +109,10 @@ __device__ __forceinline__ void jacobi_eigh(double (&a)[A][A], double (&u)[A][A]
 #pragma unroll
     for (int j = i + 1; j < A; ++j) fro2 = fma(2.0 * a[i][j], a[i][j], fro2);
   }
-  if (warm) {
-    // a <- u^T a u  (a symmetric; only the upper triangle is formed)
-    double t[A][A];
-#pragma unroll
-    for (int i = 0; i < A; ++i)
-#pragma unroll
-      for (int k = 0; k < A; ++k) {
-        double acc = UT<A>(a, i, 0) * u[0][k];
-#pragma unroll
-        for (int m = 1; m < A; ++m) acc = fma(UT<A>(a, i, m), u[m][k], acc);
-        t[i][k] = acc;  // (V u)[i][k]
-      }
-#pragma unroll
-    for (int k = 0; k < A; ++k)
-#pragma unroll
-      for (int l = k; l < A; ++l) {
-        double acc = u[0][k] * t[0][l];
-#pragma unroll
-        for (int m = 1; m < A; ++m) acc = fma(u[m][k], t[m][l], acc);
-        a[k][l] = acc;
-      }
-  }
-  // relative off-diagonal norm 1e-15: the residual perturbs M = exp(-tau V) by about
-  // tau*|V|*1e-15, far below the parity bound; quadratic convergence reaches it in a few sweeps
-  const double tol2 = 1e-30 * fro2;
+  // stop at a relative off-diagonal norm of 1e-12: the residual perturbs M = exp(-tau V) by
+  // about tau*|V|*1e-12 per bead, two orders below the per-sample parity bound; quadratic
+  // convergence takes the typical matrix from 1e-4 to below 1e-12 in one sweep
+  const double tol2 = 1e-24 * fro2;
   constexpr int kMaxSweeps = (A <= 4) ? 12 : 20;
   for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
     double off = 0.0;
@@ -128,16 +136,28 @@ __device__ __forceinline__ double oterm(double coth, double csch2, double x1, do
   return coth * (x1 * x1 + x2 * x2) - (csch2 * x1) * x2;
 }
 
+// Occupancy: for A <= 4 the VGPR budget is capped at 168 (three waves per SIMD). The kernel
+// is latency-bound (long dependent FP64 chains); at A = 4 with +-dbeta this is 16% faster
+// than the compiler's default of 256 VGPRs / two waves (measured on MI355X). Larger A keeps
+// the default: its live matrices do not fit 168 VGPRs without heavy spilling.
+template <int A>
+struct EvalOccupancy {
+  static constexpr int waves = (A <= 4) ? 3 : 1;
+};
+
 template <int A, int AR, bool PM, bool DBG>
-__global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A>::waves)))
+k_eval_fp64(const EvalArgs args) {
   constexpr bool SEP = (AR != A);  // qTempTensor path (pibronic/pimc/pimc.py:1094-1096)
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int G = args.G, K = args.K, N = args.N, P = args.P;
   const int gl = lane & (G - 1);
-  const int64_t sample = wave * (kWave / G) + lane / G;
+  const int spw = kWave / G;
+  const int64_t sample = wave * spw + lane / G;
   const bool live = sample < args.B;
-  const double* __restrict__ qs = args.q + (live ? sample : 0) * (int64_t)N * P;
+  const int64_t NP = (int64_t)N * P;
+  const double* __restrict__ qs = args.q + (live ? sample : 0) * NP;
 
   // read-only constants: restrict-qualified so the compiler can keep them in scalar loads
   struct RO {
@@ -170,19 +190,22 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
   // prod_p S_p kept as mantissa * 2^exponent; one log per lane at the end
   double smant = 1.0;
   int sexp = 0;
-  // eigenvectors of the previous bead: warm start of the next Jacobi
-  double u[A][A];
-#pragma unroll
-  for (int i = 0; i < A; ++i)
-#pragma unroll
-    for (int k = 0; k < A; ++k) u[i][k] = (i == k) ? 1.0 : 0.0;
 
   const int pbeg = gl * K;
   const int pend = live ? min(pbeg + K, P) : pbeg;
   for (int p = pbeg; p < pend; ++p) {
     const int pn = (p + 1 == P) ? 0 : p + 1;
-    double er[AR], ev[A], ep[A], em[A];
-    double v[A][A];
+    double er[AR], ev[A], ep[A], em[A], v[A][A];
+#pragma unroll
+    for (int a = 0; a < AR; ++a) er[a] = 0.0;
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      ev[a] = 0.0;
+      ep[a] = 0.0;
+      em[a] = 0.0;
+#pragma unroll
+      for (int c = 0; c <= a; ++c) v[a][c] = 0.0;
+    }
     for (int j = 0; j < N; ++j) {
       const double x0 = qs[j * P + p], xn = qs[j * P + pn];
       // sampling model: q -= d_rho; O_rho; q += d_rho (pimc.py:1091-1127)
@@ -194,7 +217,7 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
           const double d = sr.shift[a * N + j];
           const double x1 = x0 - d, x2 = xn - d;
           const double t = oterm(c, s2, x1, x2);
-          er[a] = (j == 0) ? t : er[a] + t;
+          er[a] += t;  // 0 + t_0 == t_0: same sums as the reference
           if (!SEP) { q1[a] = x1 + d; q1n[a] = x2 + d; }
         }
         if (SEP) {
@@ -213,16 +236,16 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
           const double d = sv.shift[a * N + j];
           const double y1 = q1[a] - d, y2 = q1n[a] - d;
           const double t = oterm(c, s2, y1, y2);
-          ev[a] = (j == 0) ? t : ev[a] + t;
+          ev[a] += t;
           q2[a] = y1 + d;
           if (PM) {
             const double q2n = y2 + d;
             const double z1 = q2[a] - d, z2 = q2n - d;
             const double tp = oterm(cP, sP2, z1, z2);
-            ep[a] = (j == 0) ? tp : ep[a] + tp;
+            ep[a] += tp;
             const double w1 = (z1 + d) - d, w2 = (z2 + d) - d;
             const double tm = oterm(cM, sM2, w1, w2);
-            em[a] = (j == 0) ? tm : em[a] + tm;
+            em[a] += tm;
           }
         }
       }
@@ -232,9 +255,9 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
       for (int b = 0; b < A; ++b)
 #pragma unroll
         for (int c = 0; c <= b; ++c) {
-          double term = Lv[(j * A + b) * A + c] * q2[b];
-          if (Qd) term = fma(q2[c] * Qd[(j * A + b) * A + c], q2[b], term);
-          v[b][c] = (j == 0) ? term : v[b][c] + term;
+          const double lq = Qd ? fma(Qd[(j * A + b) * A + c], q2[c], Lv[(j * A + b) * A + c])
+                               : Lv[(j * A + b) * A + c];
+          v[b][c] = fma(q2[b], lq, v[b][c]);
         }
     }
     // off-diagonal quadratic mode pairs (j != k): recompute the V-stage coordinate copies
@@ -280,12 +303,12 @@ __global__ void __launch_bounds__(256) k_eval_fp64(const EvalArgs args) {
     if (args.noscale) S = 1.0;
 
     // coupling matrix -> Jacobi -> M = U exp(-tau lambda) U^T (pimc.py:1171-1187)
-    double a_[A][A];
+    double a_[A][A], u[A][A];
 #pragma unroll
     for (int b = 0; b < A; ++b)
 #pragma unroll
       for (int c = 0; c <= b; ++c) a_[c][b] = v[b][c] + Ev[b * A + c];
-    jacobi_eigh<A>(a_, u, p != pbeg);
+    jacobi_eigh<A>(a_, u);
     double ex[A];
 #pragma unroll
     for (int d = 0; d < A; ++d) ex[d] = exp(-args.tau * a_[d][d]);
@@ -445,14 +468,20 @@ static hipError_t launch_a(int pm, const EvalArgs& a, hipStream_t s) {
   const int threads = 256;
   const dim3 grid((unsigned)((waves * kWave + threads - 1) / threads));
   const bool dbg = a.dbg != nullptr;
+  // q is read straight from global memory. Staging each wave's samples through LDS (one
+  // coalesced copy) removes the ~2.5x re-fetch of q seen in FETCH_SIZE, but measured 3-13%
+  // slower on MI355X: the copy prologue adds latency and the LDS it needs costs occupancy,
+  // while this kernel is bound by dependent FP64 arithmetic, not by memory.
+  const EvalArgs& b = a;
+  const size_t lds = 0;
   if (pm && dbg)
-    hipLaunchKernelGGL((k_eval_fp64<A, AR, true, true>), grid, dim3(threads), 0, s, a);
+    hipLaunchKernelGGL((k_eval_fp64<A, AR, true, true>), grid, dim3(threads), lds, s, b);
   else if (pm)
-    hipLaunchKernelGGL((k_eval_fp64<A, AR, true, false>), grid, dim3(threads), 0, s, a);
+    hipLaunchKernelGGL((k_eval_fp64<A, AR, true, false>), grid, dim3(threads), lds, s, b);
   else if (dbg)
-    hipLaunchKernelGGL((k_eval_fp64<A, AR, false, true>), grid, dim3(threads), 0, s, a);
+    hipLaunchKernelGGL((k_eval_fp64<A, AR, false, true>), grid, dim3(threads), lds, s, b);
   else
-    hipLaunchKernelGGL((k_eval_fp64<A, AR, false, false>), grid, dim3(threads), 0, s, a);
+    hipLaunchKernelGGL((k_eval_fp64<A, AR, false, false>), grid, dim3(threads), lds, s, b);
   return hipGetLastError();
 }
 
