@@ -210,8 +210,9 @@ class ModelSampling(ModelClass):
         self.standard_deviation = np.sqrt(1. / self.inverse_covariance)
 
     def precompute(self, data):
-        """pimc.py:409-424"""
+        """pimc.py:409-424 (E~_a = E_a + delta_a is kept: finish_folding_in_terms blanks E)."""
         self.compute_linear_displacement()
+        self.tilde_energy = self.energy + self.delta_weight
         self.compute_weight_for_each_state()
         self.initialize_TDP_object()
         self.finish_folding_in_terms()
