@@ -142,6 +142,68 @@ int pib_sample_block(pib_ctx* ctx, uint64_t seed, int64_t offset, int64_t B, dou
 int pib_block_sums(pib_ctx* ctx, const double* out_dev, int64_t out_stride, int64_t B, int pm,
                    double delta_beta, double* sums_dev, void* stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Learned proposal (affine-coupling flow over bead normal-mode coordinates). No reference
+ * counterpart: the reference's nearest hooks are the ML data generators
+ * pibronic/pimc/pimc.py:1216-1338 (g and rho at externally chosen R). These entry points are
+ * the building blocks a flow needs on the device; pibronic_amd.flow composes them.
+ * Device pointers throughout; float32 unless stated.
+ * ------------------------------------------------------------------------------------- */
+
+/* whitening modes of pib_flow_whiten */
+#define PIB_FLOW_WHITEN 0   /* R (double, B*N*P) -> zp (float, B*N*P), Z = (R - m) U / sigma  */
+#define PIB_FLOW_UNWHITEN 1 /* zp (float) -> R (double) = m + (Z sigma) U^T                   */
+#define PIB_FLOW_GRAD 2     /* dR (float) -> dzp (float) = (dR U) sigma: dL/dZ from dL/dR     */
+
+/* C = op(A) op(B) (+ bias[N]) (relu) (* [mask > 0]) (+ C if accumulate), FP32 inputs and
+ * accumulation on MFMA. M, N multiples of 64, K a multiple of 16. */
+int pib_gemm_f32(pib_ctx* ctx, int trans_a, int trans_b, int64_t M, int64_t N, int64_t K,
+                 const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                 int64_t ldc, const float* bias, int relu, const float* mask, int64_t ldmask,
+                 int accumulate, void* stream);
+
+/* Normal-mode basis of the flow: U_host (P*P, the circulant eigenvectors, row p = bead),
+ * sigma_host (N*P, harmonic standard deviations per mode and normal mode) and m_host (N, the
+ * centre subtracted before rotating). P must be a multiple of 64 (MFMA tiles of the
+ * whitening product; the coupling halves are the even-k and odd-k normal modes). */
+int pib_flow_set_basis(pib_ctx* ctx, int N, int P, const double* U_host,
+                       const double* sigma_host, const double* m_host);
+
+/* Whitening transforms between R and the flow's permuted coordinates (modes above);
+ * work_dev: 2*B*N*P floats of scratch. B*N a multiple of 64. */
+int pib_flow_whiten(pib_ctx* ctx, int mode, int64_t B, const void* in_dev, float* work_dev,
+                    void* out_dev, void* stream);
+
+/* One coupling layer on z (B*D, in place on the half at b_off) from the MLP output o
+ * (B*D: raw scales | shifts); logdet (double, B) += sum s (forward) or -= sum s (inverse). */
+int pib_flow_coupling(pib_ctx* ctx, int inverse, int64_t B, int D, int a_off, int b_off,
+                      float smax, float* z_dev, const float* o_dev, double* logdet_dev,
+                      void* stream);
+
+/* Backward of an inverse coupling layer for L = -sum_b c_b (log rho(R0_b) + logdet_b):
+ * y = saved layer input, dz = dL/d(output) (B-half overwritten with dL/d(input)),
+ * dout = dL/d(o). */
+int pib_flow_coupling_inverse_backward(pib_ctx* ctx, int64_t B, int D, int a_off, int b_off,
+                                       float smax, const float* y_dev, const float* o_dev,
+                                       float* dz_dev, float* dout_dev, const double* c_dev,
+                                       void* stream);
+
+/* log rho(R) of the sampling model (unscaled, double, B) and, if dR_dev is not NULL,
+ * dR = -c * d log rho / dR (float, B*N*P; c_dev NULL means c = 1). Uses the PIB_SET_RHO
+ * constants and P. Same function as build_o_matrix + build_denominator
+ * (pibronic/pimc/pimc.py:1087-1136) in log form. */
+int pib_flow_logrho(pib_ctx* ctx, int64_t B, const double* R_dev, const double* c_dev,
+                    float* dR_dev, double* logrho_dev, void* stream);
+
+/* out[j] = sum_i X[i*ldx + j] for a M x N matrix. */
+int pib_colsum_f32(pib_ctx* ctx, int64_t M, int64_t N, const float* X_dev, int64_t ldx,
+                   float* out_dev, void* stream);
+
+/* One Adam step (Kingma & Ba) on n parameters; step counts from 1. */
+int pib_adam_f32(pib_ctx* ctx, int64_t n, float* p_dev, const float* g_dev, float* m_dev,
+                 float* v_dev, float lr, float beta1, float beta2, float eps, int step,
+                 void* stream);
+
 #ifdef __cplusplus
 }
 #endif

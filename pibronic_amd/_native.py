@@ -44,7 +44,21 @@ SIGNATURES = {
     "pib_set_sampler": (_c_int, [_vp, _c_int, _c_int, _c_int, _dp, _dp, _dp, _dp]),
     "pib_sample_block": (_c_int, [_vp, _c_u64, _c_i64, _c_i64, _vp, _vp, _vp]),
     "pib_block_sums": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_int, _c_dbl, _vp, _vp]),
+    "pib_gemm_f32": (_c_int, [_vp, _c_int, _c_int, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp,
+                              _c_i64, _vp, _c_i64, _vp, _c_int, _vp, _c_i64, _c_int, _vp]),
+    "pib_flow_set_basis": (_c_int, [_vp, _c_int, _c_int, _dp, _dp, _dp]),
+    "pib_flow_whiten": (_c_int, [_vp, _c_int, _c_i64, _vp, _vp, _vp, _vp]),
+    "pib_flow_coupling": (_c_int, [_vp, _c_int, _c_i64, _c_int, _c_int, _c_int, ctypes.c_float,
+                                   _vp, _vp, _vp, _vp]),
+    "pib_flow_coupling_inverse_backward": (_c_int, [_vp, _c_i64, _c_int, _c_int, _c_int,
+                                                    ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pib_flow_logrho": (_c_int, [_vp, _c_i64, _vp, _vp, _vp, _vp, _vp]),
+    "pib_colsum_f32": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "pib_adam_f32": (_c_int, [_vp, _c_i64, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_float, ctypes.c_float, _c_int, _vp]),
 }
+
+FLOW_WHITEN, FLOW_UNWHITEN, FLOW_GRAD = 0, 1, 2
 
 _lib = None
 
@@ -176,3 +190,43 @@ class Context:
         self._check(self.lib.pib_block_sums(self.h, _devptr(out), int(out_stride), int(B),
                                             int(bool(pm)), float(delta_beta), _devptr(sums),
                                             _stream(stream)))
+
+    # ------------------------------------------------------------------ learned proposal
+    def gemm_f32(self, A, B, C, M, N, K, lda, ldb, ldc, trans_a=False, trans_b=False, bias=None,
+                 relu=False, mask=None, ldmask=0, accumulate=False, stream=None):
+        self._check(self.lib.pib_gemm_f32(self.h, int(trans_a), int(trans_b), int(M), int(N), int(K),
+                                          _devptr(A), int(lda), _devptr(B), int(ldb), _devptr(C),
+                                          int(ldc), _devptr(bias), int(bool(relu)), _devptr(mask),
+                                          int(ldmask), int(bool(accumulate)), _stream(stream)))
+
+    def flow_set_basis(self, U, sigma, m):
+        Uh, sh, mh = _host(U), _host(sigma), _host(m)
+        N, P = sh.shape
+        self._check(self.lib.pib_flow_set_basis(self.h, N, P, _dptr(Uh), _dptr(sh), _dptr(mh)))
+
+    def flow_whiten(self, mode, B, inp, work, out, stream=None):
+        self._check(self.lib.pib_flow_whiten(self.h, int(mode), int(B), _devptr(inp), _devptr(work),
+                                             _devptr(out), _stream(stream)))
+
+    def flow_coupling(self, inverse, B, D, a_off, b_off, smax, z, o, logdet, stream=None):
+        self._check(self.lib.pib_flow_coupling(self.h, int(bool(inverse)), int(B), int(D), int(a_off),
+                                               int(b_off), float(smax), _devptr(z), _devptr(o),
+                                               _devptr(logdet), _stream(stream)))
+
+    def flow_coupling_inverse_backward(self, B, D, a_off, b_off, smax, y, o, dz, dout, c, stream=None):
+        self._check(self.lib.pib_flow_coupling_inverse_backward(
+            self.h, int(B), int(D), int(a_off), int(b_off), float(smax), _devptr(y), _devptr(o),
+            _devptr(dz), _devptr(dout), _devptr(c), _stream(stream)))
+
+    def flow_logrho(self, B, R, c=None, dR=None, logrho=None, stream=None):
+        self._check(self.lib.pib_flow_logrho(self.h, int(B), _devptr(R), _devptr(c), _devptr(dR),
+                                             _devptr(logrho), _stream(stream)))
+
+    def colsum_f32(self, M, N, X, ldx, out, stream=None):
+        self._check(self.lib.pib_colsum_f32(self.h, int(M), int(N), _devptr(X), int(ldx), _devptr(out),
+                                            _stream(stream)))
+
+    def adam_f32(self, n, p, g, m, v, lr, beta1, beta2, eps, step, stream=None):
+        self._check(self.lib.pib_adam_f32(self.h, int(n), _devptr(p), _devptr(g), _devptr(m), _devptr(v),
+                                          float(lr), float(beta1), float(beta2), float(eps), int(step),
+                                          _stream(stream)))

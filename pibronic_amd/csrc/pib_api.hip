@@ -10,6 +10,7 @@
 #include "../../include/pibronic_hip.h"
 #include "pib_common.hpp"
 #include "pib_kernels.hpp"
+#include "pib_flow.hpp"
 
 using namespace pib;
 
@@ -47,6 +48,9 @@ struct pib_ctx {
   // reduction scratch
   DevBuf partials;
   int nparts = 0;
+  // flow basis
+  int f_N = 0, f_P = 0;
+  DevBuf f_U, f_sigma, f_isigma, f_m;
 };
 
 namespace {
@@ -154,6 +158,10 @@ int pib_ctx_destroy(pib_ctx* c) {
   release(c->s_chol);
   release(c->s_shift);
   release(c->partials);
+  release(c->f_U);
+  release(c->f_sigma);
+  release(c->f_isigma);
+  release(c->f_m);
   delete c;
   return PIB_OK;
 }
@@ -414,6 +422,151 @@ int pib_block_sums(pib_ctx* c, const double* out, int64_t out_stride, int64_t B,
   PIB_HIP(c, hipSetDevice(c->device));
   hipError_t e = launch_block_sums(a, (double*)c->partials.p, nparts, sums, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "block_sums launch");
+  return PIB_OK;
+}
+
+int pib_gemm_f32(pib_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, const float* A,
+                 int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
+                 const float* bias, int relu, const float* mask, int64_t ldmask, int accumulate,
+                 void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  GemmArgs g{A, B, C, bias, mask, M, N, K, lda, ldb, ldc, ldmask, relu ? 1 : 0, accumulate ? 1 : 0};
+  if (!A || !B || !C) return fail(c, PIB_ERR_ARG, "gemm_f32: null operand");
+  if (!gemm_shape_ok(g)) return fail(c, PIB_ERR_ARG, "gemm_f32: M and N must be multiples of 64, K of 16");
+  const int64_t rowsA = ta ? K : M, colsA = ta ? M : K, rowsB = tb ? N : K, colsB = tb ? K : N;
+  (void)rowsA; (void)rowsB;
+  if (lda < colsA || ldb < colsB || ldc < N || (mask && ldmask < N))
+    return fail(c, PIB_ERR_ARG, "gemm_f32: leading dimension smaller than the row length");
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_gemm_f32(ta != 0, tb != 0, g, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "gemm_f32 launch");
+  return PIB_OK;
+}
+
+int pib_flow_set_basis(pib_ctx* c, int N, int P, const double* U, const double* sigma,
+                       const double* m) {
+  if (!c) return PIB_ERR_ARG;
+  if (N < 1 || P < 64 || (P % 64) || !U || !sigma || !m)
+    return fail(c, PIB_ERR_ARG, "flow_set_basis: need N >= 1, P a multiple of 64 and non-null arrays");
+  if (!finite_all(U, (size_t)P * P) || !finite_all(sigma, (size_t)N * P) || !finite_all(m, N))
+    return fail(c, PIB_ERR_ARG, "flow_set_basis: non-finite value");
+  std::vector<float> Uf((size_t)P * P), sf((size_t)N * P), isf((size_t)N * P);
+  for (size_t i = 0; i < Uf.size(); ++i) Uf[i] = (float)U[i];
+  for (size_t i = 0; i < sf.size(); ++i) {
+    if (!(sigma[i] > 0.0)) return fail(c, PIB_ERR_ARG, "flow_set_basis: sigma must be > 0");
+    sf[i] = (float)sigma[i];
+    isf[i] = (float)(1.0 / sigma[i]);
+  }
+  int r;
+  if ((r = upload(c, c->f_U, Uf.data(), Uf.size() * sizeof(float)))) return r;
+  if ((r = upload(c, c->f_sigma, sf.data(), sf.size() * sizeof(float)))) return r;
+  if ((r = upload(c, c->f_isigma, isf.data(), isf.size() * sizeof(float)))) return r;
+  if ((r = upload(c, c->f_m, m, (size_t)N * sizeof(double)))) return r;
+  c->f_N = N;
+  c->f_P = P;
+  return PIB_OK;
+}
+
+int pib_flow_whiten(pib_ctx* c, int mode, int64_t B, const void* in, float* work, void* out,
+                    void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (!c->f_P) return fail(c, PIB_ERR_STATE, "flow_whiten: basis not set");
+  if (mode < PIB_FLOW_WHITEN || mode > PIB_FLOW_GRAD) return fail(c, PIB_ERR_ARG, "flow_whiten: bad mode");
+  if (B < 1 || !in || !work || !out) return fail(c, PIB_ERR_ARG, "flow_whiten: B < 1 or null pointer");
+  const int N = c->f_N, P = c->f_P;
+  if ((B * N) % 64 != 0 || P % 16 != 0)
+    return fail(c, PIB_ERR_ARG, "flow_whiten: B*N must be a multiple of 64 and P of 16");
+  BasisArgs ba{B, N, P, (const float*)c->f_sigma.p, (const float*)c->f_isigma.p, (const double*)c->f_m.p};
+  float* X = work;
+  float* Y = work + B * (int64_t)N * P;
+  hipStream_t s = (hipStream_t)stream;
+  PIB_HIP(c, hipSetDevice(c->device));
+  const float* U = (const float*)c->f_U.p;
+  hipError_t e;
+  if (mode == PIB_FLOW_UNWHITEN) {
+    if ((e = launch_flow_pre(FLOW_UNWHITEN, ba, in, Y, s)) != hipSuccess) return hip_fail(c, e, "flow pre");
+    // X = Y U^T : B operand U transposed (B(k, p) = U[p][k])
+    GemmArgs g{Y, U, X, nullptr, nullptr, B * N, P, P, P, P, P, 0, 0, 0};
+    if ((e = launch_gemm_f32(false, true, g, s)) != hipSuccess) return hip_fail(c, e, "flow gemm");
+    if ((e = launch_flow_post(FLOW_UNWHITEN, ba, X, out, s)) != hipSuccess) return hip_fail(c, e, "flow post");
+  } else {
+    const int pre = mode == PIB_FLOW_WHITEN ? FLOW_WHITEN : FLOW_GRAD;
+    if ((e = launch_flow_pre(pre, ba, in, X, s)) != hipSuccess) return hip_fail(c, e, "flow pre");
+    GemmArgs g{X, U, Y, nullptr, nullptr, B * N, P, P, P, P, P, 0, 0, 0};
+    if ((e = launch_gemm_f32(false, false, g, s)) != hipSuccess) return hip_fail(c, e, "flow gemm");
+    if ((e = launch_flow_post(pre, ba, Y, out, s)) != hipSuccess) return hip_fail(c, e, "flow post");
+  }
+  return PIB_OK;
+}
+
+int pib_flow_coupling(pib_ctx* c, int inverse, int64_t B, int D, int a_off, int b_off,
+                      float smax, float* z, const float* o, double* logdet, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (B < 0 || D < 2 || (D & 1) || !z || !o || !logdet || !(smax > 0.0f))
+    return fail(c, PIB_ERR_ARG, "flow_coupling: bad size, smax or null pointer");
+  if (!((a_off == 0 && b_off == D / 2) || (a_off == D / 2 && b_off == 0)))
+    return fail(c, PIB_ERR_ARG, "flow_coupling: halves must be (0, D/2) or (D/2, 0)");
+  CouplingArgs a{};
+  a.B = B; a.D = D; a.a_off = a_off; a.b_off = b_off; a.smax = smax;
+  a.z = z; a.o = o; a.logdet = logdet;
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_coupling(inverse ? 1 : 0, a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "flow_coupling launch");
+  return PIB_OK;
+}
+
+int pib_flow_coupling_inverse_backward(pib_ctx* c, int64_t B, int D, int a_off, int b_off,
+                                       float smax, const float* y, const float* o, float* dz,
+                                       float* dout, const double* cb, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (B < 0 || D < 2 || (D & 1) || !y || !o || !dz || !dout || !cb || !(smax > 0.0f))
+    return fail(c, PIB_ERR_ARG, "flow_coupling_inverse_backward: bad size, smax or null pointer");
+  if (!((a_off == 0 && b_off == D / 2) || (a_off == D / 2 && b_off == 0)))
+    return fail(c, PIB_ERR_ARG, "flow_coupling_inverse_backward: halves must be (0, D/2) or (D/2, 0)");
+  CouplingArgs a{};
+  a.B = B; a.D = D; a.a_off = a_off; a.b_off = b_off; a.smax = smax;
+  a.o = o; a.y = y; a.dz = dz; a.dout = dout; a.c = cb;
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_coupling_inverse_backward(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "flow_coupling_inverse_backward launch");
+  return PIB_OK;
+}
+
+int pib_flow_logrho(pib_ctx* c, int64_t B, const double* R, const double* cb, float* dR,
+                    double* logrho, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (!c->set_A[SET_RHO] || c->P < 3) return fail(c, PIB_ERR_STATE, "flow_logrho: sampling constants or beads not set");
+  if (B < 0 || !R || (!dR && !logrho)) return fail(c, PIB_ERR_ARG, "flow_logrho: B < 0 or nothing to compute");
+  LogRhoArgs a{};
+  a.B = B; a.N = c->set_N[SET_RHO]; a.P = c->P; a.A = c->set_A[SET_RHO];
+  a.coth = (const double*)c->coth[SET_RHO].p;
+  a.csch = (const double*)c->csch[SET_RHO].p;
+  a.pref = (const double*)c->pref[SET_RHO].p;
+  a.shift = (const double*)c->shift[SET_RHO].p;
+  a.R = R; a.c = cb; a.dR = dR; a.logrho = logrho;
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_logrho(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "flow_logrho launch (more than 16 surfaces?)");
+  return PIB_OK;
+}
+
+int pib_colsum_f32(pib_ctx* c, int64_t M, int64_t N, const float* X, int64_t ldx, float* out,
+                   void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (M < 1 || N < 1 || ldx < N || !X || !out) return fail(c, PIB_ERR_ARG, "colsum_f32: bad size or null pointer");
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_colsum_f32(M, N, X, ldx, out, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "colsum_f32 launch");
+  return PIB_OK;
+}
+
+int pib_adam_f32(pib_ctx* c, int64_t n, float* p, const float* g, float* m, float* v, float lr,
+                 float b1, float b2, float eps, int step, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (n < 1 || !p || !g || !m || !v || step < 1) return fail(c, PIB_ERR_ARG, "adam_f32: bad size, step or null pointer");
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_adam_f32(n, p, g, m, v, lr, b1, b2, eps, step, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "adam_f32 launch");
   return PIB_OK;
 }
 

@@ -1,0 +1,258 @@
+// Learned proposal: elementwise and reduction kernels of the affine-coupling flow.
+//
+// The flow acts on whitened normal-mode coordinates of the bead ring. With U the circulant
+// eigenvectors (the reference's normal-mode basis, pibronic/pimc/pimc.py:684-689) and
+// sigma_jk the harmonic standard deviations (:401-404), Z = (R - m) U / sigma; a single
+// harmonic surface gives Z ~ N(0, I). Coupling layers (RealNVP, Dinh et al. 2017) alternate
+// between the even-k and odd-k normal modes; the state is kept permuted so that each half is
+// one contiguous row slice: zp[b][h*D/2 + j*P/2 + kk] = Z[b][j][2 kk + h].
+#include <math.h>
+
+#include "pib_common.hpp"
+#include "pib_flow.hpp"
+
+namespace pib {
+
+// ------------------------------------------------------------------ whitening pre/post
+__global__ void k_flow_pre(int mode, BasisArgs b, const void* in, float* out) {
+  const int64_t n = b.B * (int64_t)b.N * b.P;
+  const int NP = b.N * b.P, Ph = b.P / 2, H = NP / 2;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = e / NP;
+    const int r = (int)(e - s * NP);
+    const int j = r / b.P, k = r - j * b.P;
+    if (mode == FLOW_WHITEN) {
+      out[e] = (float)(((const double*)in)[e] - b.m[j]);
+    } else if (mode == FLOW_GRAD) {
+      out[e] = ((const float*)in)[e];
+    } else {  // FLOW_UNWHITEN: out (row-major B*N x P) = unperm(zp) * sigma
+      const int h = k & 1, kk = k >> 1;
+      out[e] = ((const float*)in)[s * NP + h * H + j * Ph + kk] * b.sigma[r];
+    }
+  }
+}
+
+__global__ void k_flow_post(int mode, BasisArgs b, const float* in, void* out) {
+  const int64_t n = b.B * (int64_t)b.N * b.P;
+  const int NP = b.N * b.P, Ph = b.P / 2, H = NP / 2;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = e / NP;
+    const int r = (int)(e - s * NP);
+    const int j = r / b.P, k = r - j * b.P;
+    if (mode == FLOW_UNWHITEN) {
+      ((double*)out)[e] = (double)in[e] + b.m[j];
+    } else {
+      const float f = (mode == FLOW_WHITEN) ? b.inv_sigma[r] : b.sigma[r];
+      const int h = k & 1, kk = k >> 1;
+      ((float*)out)[s * NP + h * H + j * Ph + kk] = in[e] * f;
+    }
+  }
+}
+
+static dim3 grid_for(int64_t n, int threads) {
+  int64_t g = (n + threads - 1) / threads;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return dim3((unsigned)g);
+}
+
+hipError_t launch_flow_pre(int mode, const BasisArgs& b, const void* in, float* out, hipStream_t s) {
+  const int64_t n = b.B * (int64_t)b.N * b.P;
+  hipLaunchKernelGGL(k_flow_pre, grid_for(n, 256), dim3(256), 0, s, mode, b, in, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_flow_post(int mode, const BasisArgs& b, const float* in, void* out, hipStream_t s) {
+  const int64_t n = b.B * (int64_t)b.N * b.P;
+  hipLaunchKernelGGL(k_flow_post, grid_for(n, 256), dim3(256), 0, s, mode, b, in, out);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ block reduction helper
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (t < w) sh[t] += sh[t + w];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------------ coupling layers
+// forward:  y_B = z_B * exp(s) + t,   logdet += sum s
+// inverse:  z_B = (y_B - t) * exp(-s), logdet -= sum s
+// with s = smax * tanh(raw / smax) (bounded scales), one work-group per sample row
+__global__ void __launch_bounds__(256) k_coupling(int inverse, CouplingArgs a) {
+  __shared__ double sh[256];
+  const int64_t b = blockIdx.x;
+  const int H = a.D / 2;
+  const float* o = a.o + b * a.D;
+  float* z = a.z + b * a.D + a.b_off;
+  double part = 0.0;
+  for (int i = threadIdx.x; i < H; i += blockDim.x) {
+    const float s = a.smax * tanhf(o[i] / a.smax);
+    const float t = o[H + i];
+    z[i] = inverse ? (z[i] - t) * expf(-s) : fmaf(z[i], expf(s), t);
+    part += (double)s;
+  }
+  const double tot = block_sum(part, sh);
+  if (threadIdx.x == 0) a.logdet[b] += inverse ? -tot : tot;
+}
+
+hipError_t launch_coupling(int inverse, const CouplingArgs& a, hipStream_t s) {
+  if (a.B < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_coupling, dim3((unsigned)a.B), dim3(256), 0, s, inverse, a);
+  return hipGetLastError();
+}
+
+// Backward of one inverse coupling layer (input y, output x; x_A = y_A,
+// x_B = (y_B - t) e^{-s}, logdet_inv -= sum s) for the loss
+// L = -sum_b c_b (log rho(R0_b) + logdet_inv_b). Given dz = dL/dx:
+//   dL/dy_B = dz_B e^{-s};  dL/dt = -dz_B e^{-s};  dL/ds = -dz_B x_B + c_b;
+//   dL/draw = dL/ds (1 - (s/smax)^2).
+// dz's B-half is overwritten with dL/dy_B; dz's A-half (dL/dx_A, which passes through) is left
+// for the MLP backward to accumulate into.
+__global__ void __launch_bounds__(256) k_coupling_inv_bwd(CouplingArgs a) {
+  const int64_t b = blockIdx.x;
+  const int H = a.D / 2;
+  const float* o = a.o + b * a.D;
+  const float* yb = a.y + b * a.D + a.b_off;
+  float* dzb = a.dz + b * a.D + a.b_off;
+  float* dout = a.dout + b * a.D;
+  const float cb = (float)a.c[b];
+  for (int i = threadIdx.x; i < H; i += blockDim.x) {
+    const float th = tanhf(o[i] / a.smax);
+    const float s = a.smax * th;
+    const float t = o[H + i];
+    const float e = expf(-s);
+    const float xb = (yb[i] - t) * e;
+    const float g = dzb[i];
+    const float ds = fmaf(-g, xb, cb);
+    dout[i] = ds * (1.0f - th * th);
+    dout[H + i] = -g * e;
+    dzb[i] = g * e;
+  }
+}
+
+hipError_t launch_coupling_inverse_backward(const CouplingArgs& a, hipStream_t s) {
+  if (a.B < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_coupling_inv_bwd, dim3((unsigned)a.B), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ log rho and its gradient
+// rho(R) = sum_a prod_p O^rho_a(p) of the sampling model (pibronic/pimc/pimc.py:1087-1136,
+// unscaled), log_a = P log F_a - 1/2 sum_jp [coth_j (x_p^2 + x_{p+1}^2) - 2 csch_j x_p x_{p+1}],
+// x = R - d_a. d log_a / dx_p = -(2 coth_j x_p - csch_j (x_{p-1} + x_{p+1})). The gradient of
+// log rho is the responsibility-weighted sum over surfaces. One work-group per sample.
+constexpr int kMaxRhoSurfaces = 16;
+
+__global__ void __launch_bounds__(256) k_logrho(LogRhoArgs a) {
+  __shared__ double sh[256];
+  __shared__ double resp[kMaxRhoSurfaces];
+  const int64_t b = blockIdx.x;
+  const int N = a.N, P = a.P, NP = N * P;
+  const double* R = a.R + b * (int64_t)NP;
+  double part[kMaxRhoSurfaces];
+#pragma unroll
+  for (int s = 0; s < kMaxRhoSurfaces; ++s) part[s] = 0.0;
+  for (int e = threadIdx.x; e < NP; e += blockDim.x) {
+    const int j = e / P, p = e - j * P;
+    const int pn = (p + 1 == P) ? 0 : p + 1;
+    const double r0 = R[e], r1 = R[j * P + pn];
+#pragma unroll
+    for (int s = 0; s < kMaxRhoSurfaces; ++s) {
+      if (s < a.A) {
+        const double d = a.shift[s * N + j];
+        const double x1 = r0 - d, x2 = r1 - d;
+        part[s] += a.coth[j] * (x1 * x1 + x2 * x2) - (2.0 * a.csch[j] * x1) * x2;
+      }
+    }
+  }
+  double lmax = -INFINITY;
+  double la[kMaxRhoSurfaces];
+#pragma unroll
+  for (int s = 0; s < kMaxRhoSurfaces; ++s) {
+    if (s < a.A) {
+      la[s] = P * log(a.pref[s]) - 0.5 * block_sum(part[s], sh);
+      lmax = fmax(lmax, la[s]);
+    }
+  }
+  double tot = 0.0;
+#pragma unroll
+  for (int s = 0; s < kMaxRhoSurfaces; ++s)
+    if (s < a.A) tot += exp(la[s] - lmax);
+  if (threadIdx.x == 0) {
+    if (a.logrho) a.logrho[b] = lmax + log(tot);
+#pragma unroll
+    for (int s = 0; s < kMaxRhoSurfaces; ++s)
+      if (s < a.A) resp[s] = exp(la[s] - lmax) / tot;
+  }
+  __syncthreads();
+  if (!a.dR) return;
+  const double cb = a.c ? a.c[b] : 1.0;
+  for (int e = threadIdx.x; e < NP; e += blockDim.x) {
+    const int j = e / P, p = e - j * P;
+    const int pn = (p + 1 == P) ? 0 : p + 1, pp = (p == 0) ? P - 1 : p - 1;
+    const double r0 = R[e], rn = R[j * P + pn], rp = R[j * P + pp];
+    double g = 0.0;
+    for (int s = 0; s < a.A; ++s) {
+      const double d = a.shift[s * N + j];
+      g -= resp[s] * (2.0 * a.coth[j] * (r0 - d) - a.csch[j] * ((rn - d) + (rp - d)));
+    }
+    a.dR[b * (int64_t)NP + e] = (float)(-cb * g);
+  }
+}
+
+hipError_t launch_logrho(const LogRhoArgs& a, hipStream_t s) {
+  if (a.A > kMaxRhoSurfaces) return hipErrorInvalidValue;
+  if (a.B < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_logrho, dim3((unsigned)a.B), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ column sums, Adam
+__global__ void k_colsum(int64_t M, int64_t N, const float* X, int64_t ldx, float* out) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < N;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int64_t i = 0; i < M; ++i) acc += X[i * ldx + j];
+    out[j] = (float)acc;
+  }
+}
+
+hipError_t launch_colsum_f32(int64_t M, int64_t N, const float* X, int64_t ldx, float* out,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_colsum, grid_for(N, 64), dim3(64), 0, s, M, N, X, ldx, out);
+  return hipGetLastError();
+}
+
+// Adam (Kingma & Ba 2015) with bias correction; step counts from 1.
+__global__ void k_adam(int64_t n, float* p, const float* g, float* m, float* v, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = fmaf(b1, m[i], (1.0f - b1) * gi);
+    const float vi = fmaf(b2, v[i], (1.0f - b2) * gi * gi);
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+hipError_t launch_adam_f32(int64_t n, float* p, const float* g, float* m, float* v, float lr,
+                           float b1, float b2, float eps, int step, hipStream_t s) {
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, grid_for(n, 256), dim3(256), 0, s, n, p, g, m, v, lr, b1, b2, eps,
+                     c1, c2);
+  return hipGetLastError();
+}
+
+}  // namespace pib

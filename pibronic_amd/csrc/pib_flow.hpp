@@ -1,0 +1,79 @@
+// Learned-proposal (affine-coupling flow) kernels: argument blocks and launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pib {
+
+struct GemmArgs {
+  const float* A;
+  const float* B;
+  float* C;
+  const float* bias;   // N or nullptr
+  const float* mask;   // M x N (ldmask) or nullptr: C = (A B) * [mask > 0]
+  int64_t M, N, K, lda, ldb, ldc, ldmask;
+  int relu;            // C = max(C, 0) after the bias
+  int accumulate;      // C += result
+};
+
+hipError_t launch_gemm_f32(bool ta, bool tb, const GemmArgs& g, hipStream_t s);
+bool gemm_shape_ok(const GemmArgs& g);
+
+// Whitening between bead coordinates R (B, N, P) and the flow's permuted normal-mode
+// coordinates zp (B, D): zp[b][h*D/2 + j*P/2 + kk] = Z[b][j][2 kk + h], Z = (R - m) U / sigma.
+struct BasisArgs {
+  int64_t B;
+  int N, P;
+  const float* sigma;      // N * P
+  const float* inv_sigma;  // N * P
+  const double* m;         // N (mixture mean of the sampling model's displacements)
+};
+
+// mode of launch_flow_pre / launch_flow_post
+enum : int {
+  FLOW_WHITEN = 0,   // R (double) -> X = R - m (float) ; Y -> zp = perm(Y / sigma)
+  FLOW_UNWHITEN = 1, // zp -> Y = unperm(zp) * sigma ; X -> R (double) = X + m
+  FLOW_GRAD = 2      // dR (float) -> X = dR ; Y -> dzp = perm(Y * sigma)
+};
+
+hipError_t launch_flow_pre(int mode, const BasisArgs& b, const void* in, float* out, hipStream_t s);
+hipError_t launch_flow_post(int mode, const BasisArgs& b, const float* in, void* out, hipStream_t s);
+
+struct CouplingArgs {
+  int64_t B;
+  int D;             // flow dimension (N * P); halves of D/2
+  int a_off, b_off;  // offsets of the conditioning and transformed halves (0 or D/2)
+  float smax;        // s = smax * tanh(raw / smax)
+  float* z;          // (B, D) state, B-half updated in place
+  const float* o;    // (B, D) MLP output: [raw s | t]
+  double* logdet;    // (B,) accumulated log|det|
+  // backward
+  const float* y;    // (B, D) saved layer input
+  float* dz;         // (B, D) in: dL/d(layer output), out (B-half): dL/d(layer input)
+  float* dout;       // (B, D) out: dL/d(o) = [dL/d raw s | dL/dt]
+  const double* c;   // (B,) dL/d(-logdet) per sample
+};
+
+hipError_t launch_coupling(int inverse, const CouplingArgs& a, hipStream_t s);
+hipError_t launch_coupling_inverse_backward(const CouplingArgs& a, hipStream_t s);
+
+struct LogRhoArgs {
+  int64_t B;
+  int N, P, A;            // A = sampling-model surfaces
+  const double* coth;     // N
+  const double* csch;     // N
+  const double* pref;     // A
+  const double* shift;    // A * N
+  const double* R;        // (B, N, P)
+  const double* c;        // (B,) loss coefficient or nullptr
+  float* dR;              // (B, N, P): -c * grad log rho, or nullptr
+  double* logrho;         // (B,) or nullptr
+};
+
+hipError_t launch_logrho(const LogRhoArgs& a, hipStream_t s);
+hipError_t launch_colsum_f32(int64_t M, int64_t N, const float* X, int64_t ldx, float* out,
+                             hipStream_t s);
+hipError_t launch_adam_f32(int64_t n, float* p, const float* g, float* m, float* v, float lr,
+                           float b1, float b2, float eps, int step, hipStream_t s);
+
+}  // namespace pib

@@ -1,0 +1,243 @@
+"""Learned proposal: an affine-coupling flow that replaces the harmonic rho as the sampling
+distribution of the PIMC estimator.
+
+Construction. Let R0 ~ rho (the reference's harmonic sampling distribution, drawn by the
+device sampler). Whiten it into the ring's normal-mode coordinates,
+Z = (R0 - m) U / sigma (U the circulant eigenvectors, sigma the harmonic standard deviations,
+m the weighted mean of the surfaces' displacements), apply L affine coupling layers
+(RealNVP, Dinh, Sohl-Dickstein & Bengio 2017) y = F(Z), and map back, R = m + (y sigma) U^T.
+The linear maps cancel in the Jacobian, so the proposal density in the reference's
+normalisation is
+
+    log q(R) = log rho(R0) - log|det dF/dZ|,
+
+and w = g(R) / q(R) has the same expectation Z/Z_rho as the harmonic estimator g/rho. The
+last layer of every coupling network starts at zero, so an untrained flow is exactly rho.
+
+Layers alternate the even-k and odd-k normal modes as the transformed half; each coupling
+network is D/2 -> hidden -> hidden -> D (ReLU), giving raw scales and shifts for the other
+half, with scales bounded by smax * tanh(raw / smax).
+
+Training minimises KL(p || q) for p proportional to |g|: the loss of a batch with fixed
+weights c_b (summing to one) is -sum_b c_b log q(R_b), differentiated through the inverse
+flow and the analytic gradient of log rho. Samples and weights come from the method itself:
+first from rho (weights g/rho), then from the flow (weights g/q).
+
+Every operation runs in hand-written HIP kernels behind the C-ABI (MFMA GEMMs, coupling,
+whitening, log rho, Adam); torch only holds the device buffers.
+"""
+import math
+
+import numpy as np
+
+from . import _native
+
+
+class Flow:
+    def __init__(self, data, layers=8, hidden=256, smax=2.0, seed=0):
+        import torch
+        self.data = data
+        self.ctx = data.context()
+        self.N, self.P = data.modes, data.beads
+        self.D = self.N * self.P
+        self.H = self.D // 2
+        self.layers, self.hidden, self.smax = layers, hidden, float(smax)
+        self.dev = torch.device("cuda", data.device)
+        rho = data.rho
+        sigma = rho.standard_deviation[0]                 # (N, P), identical for all surfaces
+        m = rho.state_weight @ rho.state_shift            # (N,)
+        self.ctx.flow_set_basis(data.circulant_eigvects, sigma, m)
+        # parameter layout per layer: W1 (H x h), b1, W2 (h x h), b2, W3 (h x D), b3 (D)
+        H, h, D = self.H, hidden, self.D
+        self.shapes = [("W1", (H, h)), ("b1", (h,)), ("W2", (h, h)), ("b2", (h,)),
+                       ("W3", (h, D)), ("b3", (D,))]
+        self.per_layer = sum(int(np.prod(s)) for _, s in self.shapes)
+        self.n_params = self.per_layer * layers
+        rng = np.random.default_rng(seed)
+        flat = np.zeros(self.n_params, dtype=np.float32)
+        for l in range(layers):
+            for name, shp in self.shapes:
+                off, n = self._offset(l, name), int(np.prod(shp))
+                if name in ("W1", "W2"):  # He initialisation for the ReLU layers
+                    flat[off:off + n] = rng.normal(0.0, math.sqrt(2.0 / shp[0]), n)
+        self.params = torch.from_numpy(flat).to(self.dev)
+        self.grad = torch.zeros_like(self.params)
+        self.adam_m = torch.zeros_like(self.params)
+        self.adam_v = torch.zeros_like(self.params)
+        self.step_count = 0
+        self._bufs = {}
+
+    # ------------------------------------------------------------------ layout helpers
+    def _offset(self, layer, name):
+        off = layer * self.per_layer
+        for n, shp in self.shapes:
+            if n == name:
+                return off
+            off += int(np.prod(shp))
+        raise KeyError(name)
+
+    def p(self, layer, name, src=None):
+        t = self.params if src is None else src
+        shp = dict(self.shapes)[name]
+        off = self._offset(layer, name)
+        return t[off:off + int(np.prod(shp))].view(*shp)
+
+    def halves(self, layer):
+        """(a_off, b_off): even layers transform the odd-k half, odd layers the even-k half."""
+        return (0, self.H) if layer % 2 == 0 else (self.H, 0)
+
+    def set_params(self, flat):
+        import torch
+        self.params.copy_(torch.as_tensor(np.asarray(flat, dtype=np.float32)))
+
+    def _buf(self, name, shape, dtype=None):
+        import torch
+        dtype = torch.float32 if dtype is None else dtype
+        key = (name, tuple(shape), dtype)
+        if key not in self._bufs:
+            self._bufs[key] = torch.empty(shape, dtype=dtype, device=self.dev)
+        return self._bufs[key]
+
+    def _check_batch(self, B):
+        if B % 64:
+            raise ValueError("flow batches must be a multiple of 64 samples")
+
+    # ------------------------------------------------------------------ MLP of one layer
+    def _mlp(self, layer, zp, B, h1, h2, o, stream):
+        a_off, _ = self.halves(layer)
+        H, h, D = self.H, self.hidden, self.D
+        ctx = self.ctx
+        ctx.gemm_f32(zp[:, a_off:], self.p(layer, "W1"), h1, B, h, H, D, h, h,
+                     bias=self.p(layer, "b1"), relu=True, stream=stream)
+        ctx.gemm_f32(h1, self.p(layer, "W2"), h2, B, h, h, h, h, h,
+                     bias=self.p(layer, "b2"), relu=True, stream=stream)
+        ctx.gemm_f32(h2, self.p(layer, "W3"), o, B, D, h, h, D, D,
+                     bias=self.p(layer, "b3"), stream=stream)
+
+    # ------------------------------------------------------------------ directions
+    def forward(self, R0, stream=None):
+        """Sampling direction: R0 (B, N, P) double -> R (B, N, P) double, log|det dF/dZ| (B,)."""
+        import torch
+        B = R0.shape[0]
+        self._check_batch(B)
+        D, h = self.D, self.hidden
+        zp = self._buf("zp", (B, D))
+        work = self._buf("work", (2 * B * D,))
+        h1, h2, o = self._buf("h1", (B, h)), self._buf("h2", (B, h)), self._buf("o", (B, D))
+        logdet = torch.zeros(B, dtype=torch.float64, device=self.dev)
+        R = torch.empty_like(R0)
+        self.ctx.flow_whiten(_native.FLOW_WHITEN, B, R0, work, zp, stream)
+        for l in range(self.layers):
+            a_off, b_off = self.halves(l)
+            self._mlp(l, zp, B, h1, h2, o, stream)
+            self.ctx.flow_coupling(False, B, D, a_off, b_off, self.smax, zp, o, logdet, stream)
+        self.ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, R, stream)
+        return R, logdet
+
+    def log_q(self, R, stream=None):
+        """Density direction: log q(R) = log rho(R0) + log|det dF^-1/dy| with R0 = T^-1(R)."""
+        import torch
+        B = R.shape[0]
+        self._check_batch(B)
+        D, h = self.D, self.hidden
+        zp = self._buf("zp", (B, D))
+        work = self._buf("work", (2 * B * D,))
+        h1, h2, o = self._buf("h1", (B, h)), self._buf("h2", (B, h)), self._buf("o", (B, D))
+        logdet = torch.zeros(B, dtype=torch.float64, device=self.dev)
+        R0 = torch.empty_like(R)
+        self.ctx.flow_whiten(_native.FLOW_WHITEN, B, R, work, zp, stream)
+        for l in reversed(range(self.layers)):
+            a_off, b_off = self.halves(l)
+            self._mlp(l, zp, B, h1, h2, o, stream)
+            self.ctx.flow_coupling(True, B, D, a_off, b_off, self.smax, zp, o, logdet, stream)
+        self.ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, R0, stream)
+        logrho = torch.empty(B, dtype=torch.float64, device=self.dev)
+        self.ctx.flow_logrho(B, R0, logrho=logrho, stream=stream)
+        return logrho + logdet, R0
+
+    def loss_and_grad(self, R, c, stream=None):
+        """Loss -sum_b c_b log q(R_b) and its gradient (into self.grad) for fixed R, c."""
+        import torch
+        B = R.shape[0]
+        self._check_batch(B)
+        D, H, h, L = self.D, self.H, self.hidden, self.layers
+        ctx = self.ctx
+        c = c.to(dtype=torch.float64)
+        ys = self._buf("ys", (L, B, D))
+        h1s, h2s, os_ = self._buf("h1s", (L, B, h)), self._buf("h2s", (L, B, h)), self._buf("os", (L, B, D))
+        zp = self._buf("zp", (B, D))
+        work = self._buf("work", (2 * B * D,))
+        logdet = torch.zeros(B, dtype=torch.float64, device=self.dev)
+        R0 = self._buf("R0", (B, self.N, self.P), torch.float64)
+        ctx.flow_whiten(_native.FLOW_WHITEN, B, R, work, zp, stream)
+        for l in reversed(range(L)):
+            a_off, b_off = self.halves(l)
+            ys[l].copy_(zp)
+            self._mlp(l, zp, B, h1s[l], h2s[l], os_[l], stream)
+            ctx.flow_coupling(True, B, D, a_off, b_off, self.smax, zp, os_[l], logdet, stream)
+        ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, R0, stream)
+        logrho = torch.empty(B, dtype=torch.float64, device=self.dev)
+        dR0 = self._buf("dR0", (B, self.N, self.P))
+        ctx.flow_logrho(B, R0, c=c, dR=dR0, logrho=logrho, stream=stream)
+        loss = -(c * (logrho + logdet)).sum()
+        dz = self._buf("dz", (B, D))
+        ctx.flow_whiten(_native.FLOW_GRAD, B, dR0, work, dz, stream)
+        dout = self._buf("dout", (B, D))
+        dh2, dh1 = self._buf("dh2", (B, h)), self._buf("dh1", (B, h))
+        g = self.grad
+        for l in range(L):
+            a_off, b_off = self.halves(l)
+            ctx.flow_coupling_inverse_backward(B, D, a_off, b_off, self.smax, ys[l], os_[l], dz,
+                                               dout, c, stream)
+            # o = h2 W3 + b3
+            ctx.gemm_f32(h2s[l], dout, self.p(l, "W3", g), h, D, B, h, D, D, trans_a=True, stream=stream)
+            ctx.colsum_f32(B, D, dout, D, self.p(l, "b3", g), stream)
+            ctx.gemm_f32(dout, self.p(l, "W3"), dh2, B, h, D, D, D, h, trans_b=True,
+                         mask=h2s[l], ldmask=h, stream=stream)
+            # h2 = relu(h1 W2 + b2)
+            ctx.gemm_f32(h1s[l], dh2, self.p(l, "W2", g), h, h, B, h, h, h, trans_a=True, stream=stream)
+            ctx.colsum_f32(B, h, dh2, h, self.p(l, "b2", g), stream)
+            ctx.gemm_f32(dh2, self.p(l, "W2"), dh1, B, h, h, h, h, h, trans_b=True,
+                         mask=h1s[l], ldmask=h, stream=stream)
+            # h1 = relu(y_A W1 + b1)
+            ctx.gemm_f32(ys[l][:, a_off:], dh1, self.p(l, "W1", g), H, h, B, D, h, h, trans_a=True,
+                         stream=stream)
+            ctx.colsum_f32(B, h, dh1, h, self.p(l, "b1", g), stream)
+            ctx.gemm_f32(dh1, self.p(l, "W1"), dz[:, a_off:], B, H, h, h, h, D, trans_b=True,
+                         accumulate=True, stream=stream)
+        return loss
+
+    def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
+        self.step_count += 1
+        self.ctx.adam_f32(self.n_params, self.params, self.grad, self.adam_m, self.adam_v, lr,
+                          beta1, beta2, eps, self.step_count, stream)
+
+    # ------------------------------------------------------------------ proposal draws
+    def draw(self, B, seed, offset, stream=None):
+        """B proposal samples: R (B, N, P) double and log q(R) (B,)."""
+        import torch
+        R0 = torch.empty((B, self.N, self.P), dtype=torch.float64, device=self.dev)
+        self.ctx.sample_block(seed, offset, B, R0, None, stream)
+        logrho = torch.empty(B, dtype=torch.float64, device=self.dev)
+        self.ctx.flow_logrho(B, R0, logrho=logrho, stream=stream)
+        R, logdet = self.forward(R0, stream)
+        return R, logrho - logdet
+
+
+def log_weights(data, R, logq, stream=None):
+    """log|g(R)| - log q(R) and sign(g) for proposal samples, g from the FP64 kernel."""
+    import torch
+    B = R.shape[0]
+    out = torch.empty((_native.NUM_OUT, B), dtype=torch.float64, device=R.device)
+    data.evaluate_device(R, B, False, out, B, noscale=False, stream=stream)
+    g = out[_native.OUT_G]
+    return torch.log(torch.abs(g)) + out[_native.OUT_LOGS] - logq, torch.sign(g)
+
+
+def harmonic_log_weights(data, R, stream=None):
+    """log|g/rho| and sign for rho samples (the harmonic estimator's weights)."""
+    import torch
+    B = R.shape[0]
+    out = torch.empty((_native.NUM_OUT, B), dtype=torch.float64, device=R.device)
+    data.evaluate_device(R, B, False, out, B, noscale=False, stream=stream)
+    return torch.log(torch.abs(out[_native.OUT_G] / out[_native.OUT_RHO])), torch.sign(out[_native.OUT_G])

@@ -1,0 +1,185 @@
+"""Learned-proposal kernels on the GPU against the float64 torch reference (FP32 bounds).
+
+GEMM (all transposes, bias, ReLU, mask, accumulate), whitening, log rho, the flow in both
+directions, and the training gradient; then a short training run that must lower the
+weight variance relative to the harmonic rho on the same model.
+"""
+import numpy as np
+import pytest
+import torch
+
+from flow_helpers import random_params, ref_for
+from helpers import make_data, stage_dict_models
+from pibronic_amd import _native, flow, synthetic
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_gemm_f32(tmp_path, ta, tb):
+    _announce(f"gemm ta={ta} tb={tb}")
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, 2, quadratic=False, seed=0))
+    ctx = make_data(FS, 300.0, 64, 10, 10).context()
+    M, N, K = 128, 192, 48
+    g = torch.Generator().manual_seed(ta * 2 + tb)
+    A = torch.randn((K, M) if ta else (M, K), generator=g, dtype=torch.float32)
+    B = torch.randn((N, K) if tb else (K, N), generator=g, dtype=torch.float32)
+    bias = torch.randn(N, generator=g)
+    mask = (torch.rand((M, N), generator=g) > 0.5).float()
+    C0 = torch.randn((M, N), generator=g)
+    ref = (A.double().T if ta else A.double()) @ (B.double().T if tb else B.double())
+    Ad, Bd, bd, md = A.to(DEV), B.to(DEV), bias.to(DEV), mask.to(DEV)
+    lda, ldb = A.shape[1], B.shape[1]
+    C = torch.empty((M, N), device=DEV)
+    ctx.gemm_f32(Ad, Bd, C, M, N, K, lda, ldb, N, trans_a=ta, trans_b=tb)
+    torch.cuda.synchronize()
+    assert torch.allclose(C.cpu().double(), ref, rtol=1e-5, atol=1e-4)
+    ctx.gemm_f32(Ad, Bd, C, M, N, K, lda, ldb, N, trans_a=ta, trans_b=tb, bias=bd, relu=True)
+    assert torch.allclose(C.cpu().double(), torch.relu(ref + bias.double()), rtol=1e-5, atol=1e-4)
+    ctx.gemm_f32(Ad, Bd, C, M, N, K, lda, ldb, N, trans_a=ta, trans_b=tb, mask=md, ldmask=N)
+    assert torch.allclose(C.cpu().double(), ref * mask.double(), rtol=1e-5, atol=1e-4)
+    Cacc = C0.to(DEV).clone()
+    ctx.gemm_f32(Ad, Bd, Cacc, M, N, K, lda, ldb, N, trans_a=ta, trans_b=tb, accumulate=True)
+    assert torch.allclose(Cacc.cpu().double(), ref + C0.double(), rtol=1e-5, atol=1e-4)
+    with pytest.raises(_native.NativeError, match="multiples"):
+        ctx.gemm_f32(Ad, Bd, C, 100, N, K, lda, ldb, N, trans_a=ta, trans_b=tb)
+
+
+@pytest.fixture(scope="module")
+def model(tmp_path_factory):
+    FS = stage_dict_models(tmp_path_factory.mktemp("flow"), synthetic.generate(3, 2, quadratic=True, seed=1))
+    data = make_data(FS, 300.0, 64, 1024, 1024, seed=3)
+    return data
+
+
+def _flow_and_ref(data, layers=4, hidden=64, seed=7, scale=0.05):
+    fl = flow.Flow(data, layers=layers, hidden=hidden, smax=2.0)
+    ref = ref_for(data, layers, hidden, 2.0)
+    flat = random_params(ref, seed, scale)
+    fl.set_params(flat)
+    return fl, ref, flat
+
+
+def _host_q(data, B, seed):
+    q, _ = data.draw_host_sample(np.random.default_rng(seed), B)
+    return q
+
+
+def test_whitening_and_logrho(model):
+    _announce("flow whitening + log rho")
+    data = model
+    fl, ref, _ = _flow_and_ref(data)
+    B = 128
+    q = _host_q(data, B, 1)
+    R = torch.as_tensor(q, device=DEV)
+    zp = torch.empty((B, fl.D), device=DEV)
+    work = torch.empty(2 * B * fl.D, device=DEV)
+    fl.ctx.flow_whiten(_native.FLOW_WHITEN, B, R, work, zp)
+    zref = ref.whiten(torch.as_tensor(q))
+    assert torch.allclose(zp.cpu().double(), zref, rtol=1e-4, atol=1e-4)
+    back = torch.empty_like(R)
+    fl.ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, back)
+    assert torch.allclose(back.cpu(), torch.as_tensor(q), atol=1e-4)
+    lr = torch.empty(B, dtype=torch.float64, device=DEV)
+    fl.ctx.flow_logrho(B, R, logrho=lr)
+    torch.cuda.synchronize()
+    assert torch.allclose(lr.cpu(), ref.log_rho(torch.as_tensor(q)), rtol=0, atol=1e-10)
+    # the same rho as the FP64 evaluation kernel (scaled rho times the removed scale)
+    r = __import__("pibronic_amd.pimc", fromlist=["x"]).evaluate_samples(data, q, pm=False)
+    assert np.allclose(lr.cpu().numpy(), np.log(r["s_rho"]) + r["log_scale"], atol=1e-9)
+
+
+def test_flow_forward_and_density(model):
+    _announce("flow forward + log q")
+    data = model
+    fl, ref, flat = _flow_and_ref(data)
+    B = 256
+    q = _host_q(data, B, 2)
+    R0 = torch.as_tensor(q, device=DEV)
+    R, ld = fl.forward(R0)
+    Rr, ldr = ref.forward(torch.as_tensor(q), flat)
+    torch.cuda.synchronize()
+    assert torch.allclose(R.cpu(), Rr, rtol=1e-4, atol=2e-4)
+    assert torch.allclose(ld.cpu(), ldr, rtol=1e-4, atol=2e-3)
+    lq, R0b = fl.log_q(R)
+    lqr, _ = ref.log_q(R.cpu(), flat)
+    assert torch.allclose(R0b.cpu(), torch.as_tensor(q), atol=5e-4)
+    assert torch.allclose(lq.cpu(), lqr, rtol=1e-5, atol=5e-3)
+
+
+def test_training_gradient(model):
+    _announce("flow gradient")
+    data = model
+    fl, ref, flat = _flow_and_ref(data)
+    B = 256
+    q = _host_q(data, B, 3)
+    c = np.random.default_rng(4).uniform(0.5, 1.5, B)
+    c /= c.sum()
+    loss = fl.loss_and_grad(torch.as_tensor(q, device=DEV), torch.as_tensor(c, device=DEV))
+    torch.cuda.synchronize()
+    lref, gref = ref.loss_and_grad(q, c, flat)
+    g = fl.grad.cpu().numpy().astype(np.float64)
+    assert float(loss) == pytest.approx(lref, rel=1e-5, abs=1e-3)
+    rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    print(f"gradient relative error {rel:.2e}")
+    assert rel < 2e-3
+    # every parameter block gets a gradient
+    for l in range(fl.layers):
+        for name, _ in fl.shapes:
+            assert torch.count_nonzero(fl.p(l, name, fl.grad)) > 0, (l, name)
+
+
+def test_adam_step_matches_formula(model):
+    _announce("adam")
+    data = model
+    fl, ref, flat = _flow_and_ref(data, layers=2, hidden=64)
+    rng = np.random.default_rng(0)
+    g = rng.normal(size=fl.n_params).astype(np.float32)
+    fl.grad.copy_(torch.as_tensor(g))
+    p0 = fl.params.cpu().numpy().copy()
+    fl.adam_step(1e-2)
+    fl.grad.copy_(torch.as_tensor(2 * g))
+    fl.adam_step(1e-2)
+    torch.cuda.synchronize()
+    m = 0.1 * g
+    v = 0.001 * g * g
+    p1 = p0 - 1e-2 * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    m = 0.9 * m + 0.1 * 2 * g
+    v = 0.999 * v + 0.001 * 4 * g * g
+    p2 = p1 - 1e-2 * (m / (1 - 0.81)) / (np.sqrt(v / (1 - 0.999 ** 2)) + 1e-8)
+    assert np.allclose(fl.params.cpu().numpy(), p2, rtol=1e-5, atol=1e-6)
+
+
+def test_identity_flow_reproduces_harmonic_weights(model):
+    _announce("zero-initialised flow = rho")
+    data = model
+    fl = flow.Flow(data, layers=2, hidden=64)
+    B = 512
+    R, lq = fl.draw(B, seed=5, offset=0)
+    lw, sg = flow.log_weights(data, R, lq)
+    lh, sh = flow.harmonic_log_weights(data, R)
+    torch.cuda.synchronize()
+    assert torch.allclose(lw, lh, atol=1e-4)
+
+
+def test_training_lowers_weight_variance(tmp_path):
+    """A short on-policy training run (samples and weights drawn by the method itself): the
+    trained proposal's Kish ESS fraction on fresh samples beats the harmonic rho's, and both
+    give the same unbiased Z/Z_rho."""
+    _announce("flow training gain A2 N2 P64")
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, 2, quadratic=True, seed=5))
+    data = make_data(FS, 300.0, 64, 1024, 1024, seed=11)
+    from pibronic_amd import learn
+    res = learn.train_and_compare(data, layers=4, hidden=64, steps=300, batch=1024,
+                                  eval_samples=65536, lr=1e-3, seed=11)
+    print({k: v for k, v in res.items() if k not in ("flow", "train")})
+    for st in res["train"]:
+        print(st)
+    assert res["ess_frac_flow"] > 1.5 * res["ess_frac_rho"]
+    assert abs(res["z_flow"] - res["z_rho"]) < 4 * np.hypot(res["z_flow_err"], res["z_rho_err"])
