@@ -112,6 +112,61 @@ def cpu_baseline(vib_path, rho_path, T, P, seconds, cores, B=500):
 
 
 # --------------------------------------------------------------------------------------
+def weight_summary(s, elapsed):
+    """Effective-sample statistics from summed block sums over a timed region."""
+    from pibronic_amd import _native
+    n = s[_native.SUM_N]
+    ess = s[_native.SUM_ABSR] ** 2 / s[_native.SUM_R2]
+    mean_w = s[_native.SUM_ABSR] / n
+    cv2 = max(s[_native.SUM_R2] / n - mean_w ** 2, 0.0) / mean_w ** 2
+    n_needed = cv2 / 1e-6
+    z = s[_native.SUM_R] / n
+    z_err = math.sqrt(max(s[_native.SUM_R2] / n - z * z, 0.0) / (n - 1))
+    return {"samples": n, "seconds": elapsed, "samples_per_s": n / elapsed,
+            "ess_fraction": ess / n, "ess_per_s": ess / elapsed,
+            "negative_fraction": s[_native.SUM_NEG] / n, "z_ratio": z, "z_ratio_err": z_err,
+            "samples_for_1e-3_z_error": n_needed, "time_to_1e-3_z_error_s": n_needed * elapsed / n}
+
+
+def learned_leg(data, args, log):
+    """Train the flow on-policy, then time K steps of: flow draw (device sampler + log rho +
+    flow forward), FP64 evaluation of g, block sums with q as the denominator."""
+    import torch
+    from pibronic_amd import _native, flow, learn
+    t0 = time.perf_counter()
+    fl = flow.Flow(data, layers=args.flow_layers, hidden=args.flow_hidden, seed=args.seed)
+    hist = learn.train(fl, args.train_steps, args.train_batch, args.lr, args.seed + 1,
+                       log=lambda st: log(f"[bench] train step {st['step']} ess {st['ess_frac']:.3e}"),
+                       log_every=max(1, args.train_steps // 5))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    B = args.flow_block
+    K, W = args.steps, args.warmup
+    out = torch.empty((_native.NUM_OUT, B), dtype=torch.float64, device="cuda")
+    sums = torch.zeros((K + W, _native.NUM_SUMS), dtype=torch.float64, device="cuda")
+    ctx = data.context()
+    stream = torch.cuda.current_stream()
+
+    def step(i):
+        R, logq = fl.draw(B, args.seed + 2, (1 << 40) + i * B, stream)
+        data.evaluate_device(R, B, False, out, B, stream=stream)
+        ctx.block_sums_q(out, B, B, False, 1.0, logq, sums[i], stream)
+
+    for i in range(W):
+        step(i)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(W, W + K):
+        step(i)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    res = weight_summary(sums[W:].sum(dim=0).cpu().numpy(), el)
+    res.update(train_seconds=train_s, train_steps=args.train_steps, train_batch=args.train_batch,
+               samples_per_step=B, layers=args.flow_layers, hidden=args.flow_hidden,
+               train_final_batch_ess=hist[-1]["ess_frac"] if hist else None)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +178,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-cores", type=int, default=0, help="0 = min(16, os.cpu_count())")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--workload", default="A4_N8_P64_T300")
+    ap.add_argument("--train-steps", type=int, default=1000)
+    ap.add_argument("--train-batch", type=int, default=2048)
+    ap.add_argument("--flow-block", type=int, default=65536)
+    ap.add_argument("--flow-layers", type=int, default=8)
+    ap.add_argument("--flow-hidden", type=int, default=256)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this many MiB 3 times "
@@ -135,9 +198,12 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
     import numpy as np
     from pibronic_amd import pimc, synthetic, _native
-    name = "A4_N8_P64_T300"
+    name = args.workload
     cfg = synthetic.WORKLOADS[name]
     A, N, P, T = cfg["A"], cfg["N"], cfg["P"], cfg["T"]
     B = args.block
@@ -150,6 +216,7 @@ def main():
         cores = args.cpu_cores or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(FS.path_vib_model, FS.path_rho_model, T, P, args.cpu_seconds, cores)
         cpu["cores"] = cores
+        log(f"[bench] cpu baseline {cpu['samples_beads_per_s']:.3e} samples*beads/s on {cores} cores")
 
     import torch
     data = (pimc.BoxDataPM if pm else pimc.BoxData).from_FileStructure(FS)
@@ -186,7 +253,7 @@ def main():
         step(i, True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    print(f"[bench] {K} steps of {B} samples in {elapsed:.4f} s", file=sys.stderr, flush=True)
+    log(f"[bench] harmonic: {K} steps of {B} samples in {elapsed:.4f} s")
 
     if args.calib_copy_mib:
         nel = args.calib_copy_mib * (1 << 20) // 8
@@ -197,14 +264,8 @@ def main():
         torch.cuda.synchronize()
         del src_t, dst_t
 
-    s = sums[W:].sum(dim=0).cpu().numpy()
-    n = s[_native.SUM_N]
-    ess = s[_native.SUM_ABSR] ** 2 / s[_native.SUM_R2]
-    mean_w = s[_native.SUM_ABSR] / n
-    var_w = s[_native.SUM_R2] / n - mean_w ** 2
-    cv2 = var_w / mean_w ** 2
-    n_needed = cv2 / 1e-6
-    samples_per_s = n / elapsed
+    harm = weight_summary(sums[W:].sum(dim=0).cpu().numpy(), elapsed)
+    n = harm["samples"]
     eval_ms = sum(a.elapsed_time(b) for a, b in zip(ev0, ev1)) / K
     nq = int(np.count_nonzero(np.abs(data.vib.quadratic).sum(axis=(2, 3))))
     fl = algorithmic_flops(A, N, P, nq, pm)
@@ -214,24 +275,32 @@ def main():
         traffic, tdetail = profiled_traffic(
             os.path.join(ROOT, "profiles", f"k_eval_fp64_A{A}_N{N}_P{P}_B{B}.md"),
             f"void pib::k_eval_fp64<{A}, {A}, false, false>")
+
+    learned = None
+    if not args.no_learned and not pm:
+        learned = learned_leg(data, args, log)
+        log(f"[bench] learned: ess fraction {learned['ess_fraction']:.3e}, "
+            f"{learned['ess_per_s']:.3e} ESS/s (harmonic {harm['ess_per_s']:.3e})")
+
+    value = learned["ess_per_s"] if learned else harm["ess_per_s"]
     line = {
         "metric": "effective samples/sec at fixed Z-ratio std error; learned ρ vs harmonic ρ",
-        "value": ess / elapsed,
-        "unit": "effective samples/s (harmonic rho)",
+        "value": value,
+        "unit": "effective samples/s" + ("" if learned else " (harmonic rho)"),
         "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": elapsed / K * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded model, device Philox samples)",
         "config": {"workload": f"{name}: A={A} N={N} P={P} T={T:g}K block={B}"
                                + (" +-dbeta" if pm else ""),
-                   "samples_per_step": B, "lanes_per_sample": lanes},
+                   "samples_per_step": B, "lanes_per_sample": lanes,
+                   "value_from": "learned proposal" if learned else "harmonic rho"},
         "samples_beads_per_s": n * P / elapsed,
-        "samples_per_s": samples_per_s,
-        "ess_fraction": ess / n,
-        "negative_fraction": s[_native.SUM_NEG] / n,
-        "z_ratio_rel_err_1e-3_samples": n_needed,
-        "time_to_1e-3_s": n_needed / samples_per_s,
+        "samples_per_s": harm["samples_per_s"],
         "eval_kernel_ms": eval_ms,
+        "harmonic": harm,
+        "learned": learned,
+        "learned_over_harmonic_ess_per_s": (learned["ess_per_s"] / harm["ess_per_s"]) if learned else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "k_eval_fp64", "flops_per_sample": fl["eval"],

@@ -142,6 +142,12 @@ int pib_sample_block(pib_ctx* ctx, uint64_t seed, int64_t offset, int64_t B, dou
 int pib_block_sums(pib_ctx* ctx, const double* out_dev, int64_t out_stride, int64_t B, int pm,
                    double delta_beta, double* sums_dev, void* stream);
 
+/* pib_block_sums for samples drawn from another proposal than rho (the learned flow):
+ * logq_dev (B doubles) is its log density in rho's normalisation, and every r = g/q uses
+ * q = exp(logq - PIB_OUT_LOGS) in place of the PIB_OUT_RHO row. */
+int pib_block_sums_q(pib_ctx* ctx, const double* out_dev, int64_t out_stride, int64_t B, int pm,
+                     double delta_beta, const double* logq_dev, double* sums_dev, void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * Learned proposal (affine-coupling flow over bead normal-mode coordinates). No reference
  * counterpart: the reference's nearest hooks are the ML data generators

@@ -44,6 +44,7 @@ SIGNATURES = {
     "pib_set_sampler": (_c_int, [_vp, _c_int, _c_int, _c_int, _dp, _dp, _dp, _dp]),
     "pib_sample_block": (_c_int, [_vp, _c_u64, _c_i64, _c_i64, _vp, _vp, _vp]),
     "pib_block_sums": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_int, _c_dbl, _vp, _vp]),
+    "pib_block_sums_q": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_int, _c_dbl, _vp, _vp, _vp]),
     "pib_gemm_f32": (_c_int, [_vp, _c_int, _c_int, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp,
                               _c_i64, _vp, _c_i64, _vp, _c_int, _vp, _c_i64, _c_int, _vp]),
     "pib_flow_set_basis": (_c_int, [_vp, _c_int, _c_int, _dp, _dp, _dp]),
@@ -190,6 +191,11 @@ class Context:
         self._check(self.lib.pib_block_sums(self.h, _devptr(out), int(out_stride), int(B),
                                             int(bool(pm)), float(delta_beta), _devptr(sums),
                                             _stream(stream)))
+
+    def block_sums_q(self, out, out_stride, B, pm, delta_beta, logq, sums, stream=None):
+        self._check(self.lib.pib_block_sums_q(self.h, _devptr(out), int(out_stride), int(B),
+                                              int(bool(pm)), float(delta_beta), _devptr(logq),
+                                              _devptr(sums), _stream(stream)))
 
     # ------------------------------------------------------------------ learned proposal
     def gemm_f32(self, A, B, C, M, N, K, lda, ldb, ldc, trans_a=False, trans_b=False, bias=None,

@@ -399,8 +399,23 @@ int pib_sample_block(pib_ctx* c, uint64_t seed, int64_t offset, int64_t B, doubl
   return PIB_OK;
 }
 
+static int block_sums_impl(pib_ctx* c, const double* out, int64_t out_stride, int64_t B, int pm,
+                           double delta_beta, const double* logq, double* sums, void* stream);
+
 int pib_block_sums(pib_ctx* c, const double* out, int64_t out_stride, int64_t B, int pm,
                    double delta_beta, double* sums, void* stream) {
+  return block_sums_impl(c, out, out_stride, B, pm, delta_beta, nullptr, sums, stream);
+}
+
+int pib_block_sums_q(pib_ctx* c, const double* out, int64_t out_stride, int64_t B, int pm,
+                     double delta_beta, const double* logq, double* sums, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (!logq) return fail(c, PIB_ERR_ARG, "block_sums_q: null logq");
+  return block_sums_impl(c, out, out_stride, B, pm, delta_beta, logq, sums, stream);
+}
+
+static int block_sums_impl(pib_ctx* c, const double* out, int64_t out_stride, int64_t B, int pm,
+                           double delta_beta, const double* logq, double* sums, void* stream) {
   if (!c) return PIB_ERR_ARG;
   if (!out || !sums || B < 1 || out_stride < B)
     return fail(c, PIB_ERR_ARG, "block_sums: null pointer, B < 1 or out_stride < B");
@@ -419,6 +434,7 @@ int pib_block_sums(pib_ctx* c, const double* out, int64_t out_stride, int64_t B,
   a.B = B;
   a.pm = pm ? 1 : 0;
   a.delta_beta = pm ? delta_beta : 1.0;
+  a.logq = logq;
   PIB_HIP(c, hipSetDevice(c->device));
   hipError_t e = launch_block_sums(a, (double*)c->partials.p, nparts, sums, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "block_sums launch");

@@ -65,6 +65,8 @@ struct SumArgs {
   int64_t B;
   int pm;
   double delta_beta;
+  const double* logq;  // nullptr: denominator rho (row OUT_RHO); else the proposal density
+                       // q = exp(logq), rescaled like g: exp(logq - OUT_LOGS)
 };
 
 // launchers (return hipError_t of the launch)

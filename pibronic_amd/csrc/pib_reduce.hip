@@ -7,7 +7,8 @@
 // (stats.py:84-90), the basic E and Cv (:93-106), the alpha-variant means (:58-81) and the
 // Kish effective sample size. Stage 1: each work-group reduces a fixed strided slice in a
 // fixed order into one partial row; stage 2: one work-group adds the partial rows in index
-// order. No atomics, so the result does not depend on scheduling.
+// order. No atomics, so the result does not depend on scheduling. With a learned proposal
+// the denominator is its density q (log q given per sample) instead of rho.
 #include "pib_common.hpp"
 #include "pib_kernels.hpp"
 
@@ -39,7 +40,7 @@ __global__ void __launch_bounds__(kSumThreads) k_sum_partials(const SumArgs a, d
   const double idb2 = 1.0 / (a.delta_beta * a.delta_beta);
   for (int64_t i = (int64_t)blockIdx.x * kSumThreads + threadIdx.x; i < a.B;
        i += (int64_t)gridDim.x * kSumThreads) {
-    const double rho = a.out[OUT_RHO * st + i];
+    const double rho = a.logq ? exp(a.logq[i] - a.out[OUT_LOGS * st + i]) : a.out[OUT_RHO * st + i];
     const double g = a.out[OUT_G * st + i];
     const double r = g / rho;
     v[SUM_N] += 1.0;

@@ -8,9 +8,13 @@ Draw order with numpy.random.default_rng(seed), all energies in eV:
      (a, b) in row-major upper-triangle order (diagonal included), mirrored
   5. (quadratic=True) Q^jj_ab ~ N(0, 0.005) for a <= b, same order, mirrored; Q^jk = 0
      for j != k
-with off = 0.05, or 0.1 for the near-degenerate large-surface workload. The sampling
-model rho is the surface diagonal of the coupled model (remove_coupling_from_model
-semantics, pibronic/vibronic/vibronic_model_io.py:749-761).
+with off = 0.05, or 0.1 for the near-degenerate large-surface workload. Then, per mode j,
+the quadratic block is made stable: if the smallest eigenvalue of w_j I + Q^jj is below
+w_j / 2, the eigenvalues of Q^jj are raised to -w_j / 2 (same eigenvectors). As drawn, the
+A=8 and A=12 workloads have modes with w_j I + Q^jj indefinite, i.e. a potential unbounded
+below and a divergent partition function; the clip leaves the A=2 and A=4 models unchanged.
+The sampling model rho is the surface diagonal of the coupled model
+(remove_coupling_from_model semantics, pibronic/vibronic/vibronic_model_io.py:749-761).
 
 WORKLOADS names each configuration by its sizes.
 """
@@ -59,6 +63,11 @@ def generate(A, N, quadratic=True, off=0.05, seed=0):
             for a in range(A):
                 for b in range(a, A):
                     Q[j, j, a, b] = Q[j, j, b, a] = rng.normal(0.0, 0.005)
+        for j in range(N):
+            lam, vec = np.linalg.eigh(Q[j, j])
+            if lam.min() < -0.5 * w[j]:
+                Q[j, j] = (vec * np.maximum(lam, -0.5 * w[j])) @ vec.T
+                Q[j, j] = 0.5 * (Q[j, j] + Q[j, j].T)
     return {VMK.N: N, VMK.A: A, VMK.E: E, VMK.w: w, VMK.G1: L, VMK.G2: Q}
 
 

@@ -183,3 +183,32 @@ def test_training_lowers_weight_variance(tmp_path):
         print(st)
     assert res["ess_frac_flow"] > 1.5 * res["ess_frac_rho"]
     assert abs(res["z_flow"] - res["z_rho"]) < 4 * np.hypot(res["z_flow_err"], res["z_rho_err"])
+
+
+def test_block_sums_with_logq_matches_host(model):
+    """pib_block_sums_q: the device reduction with q as the denominator equals the same sums
+    formed on the host from the per-sample g, log scale and log q."""
+    _announce("block sums with log q")
+    data = model
+    fl = flow.Flow(data, layers=2, hidden=64)
+    fl.set_params(random_params(ref_for(data, 2, 64, 2.0), 9, 0.05))
+    B = 1024
+    R, lq = fl.draw(B, seed=21, offset=0)
+    out = torch.empty((_native.NUM_OUT, B), dtype=torch.float64, device=DEV)
+    data.evaluate_device(R, B, False, out, B)
+    sums = torch.zeros(_native.NUM_SUMS, dtype=torch.float64, device=DEV)
+    data.context().block_sums_q(out, B, B, False, 1.0, lq, sums)
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    r = o[_native.OUT_G] / np.exp(lq.cpu().numpy() - o[_native.OUT_LOGS])
+    s = sums.cpu().numpy()
+    assert s[_native.SUM_N] == B
+    assert s[_native.SUM_R] == pytest.approx(r.sum(), rel=1e-12)
+    assert s[_native.SUM_R2] == pytest.approx((r * r).sum(), rel=1e-12)
+    assert s[_native.SUM_ABSR] == pytest.approx(np.abs(r).sum(), rel=1e-12)
+    assert s[_native.SUM_NEG] == (r < 0).sum()
+    # the same as the flow's own log-weight path
+    lw, sg = flow.log_weights(data, R, lq)
+    assert np.allclose(np.log(np.abs(r)), lw.cpu().numpy(), rtol=0, atol=1e-9)
+    with pytest.raises(_native.NativeError):
+        data.context().block_sums_q(out, B, B, False, 1.0, None, sums)
