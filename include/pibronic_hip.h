@@ -45,6 +45,8 @@ extern "C" {
 /* flags of pib_eval_block */
 #define PIB_EVAL_PM 1      /* also g(beta +- dbeta): block_compute_pm                     */
 #define PIB_EVAL_NOSCALE 2 /* skip the S12 scaling (S_p = 1), as the explicit golden test */
+#define PIB_EVAL_WIDE 4    /* force the one-wave-per-sample kernel (used for A > 8; any
+                              A, A_rho <= 16); a tuning/testing switch, same results       */
 
 /* rows of the per-sample output matrix written by pib_eval_block */
 #define PIB_OUT_RHO 0  /* s_rho = rho / prod_p S_p             (BoxResult.scaled_rho)       */
@@ -113,7 +115,9 @@ int pib_get_lanes_per_sample(pib_ctx* ctx, int64_t B);
  * build_o_matrix, build_scaling_factors, scale_o_matrices, build_denominator,
  * diagonalize_coupling_matrix and build_numerator (:1062-1213) fused into one launch.
  * debug_dev, if not NULL, receives per (sample, bead) records of
- * [O_rho (A_rho), O_vib (A), S_p, M (A*A)] (unscaled O), B*P records. */
+ * [O_rho (A_rho), O_vib (A), S_p, M (A*A)] (unscaled O), B*P records.
+ * A <= 8 runs a lanes-per-sample kernel; 8 < A <= 16 (and any A with PIB_EVAL_WIDE) runs one
+ * wave per sample with 16x16 padded matrices. A or A_rho > 16: PIB_ERR_UNSUPPORTED. */
 int pib_eval_block(pib_ctx* ctx, const double* q_dev, int64_t B, int flags, double* out_dev,
                    int64_t out_stride, double* debug_dev, void* stream);
 

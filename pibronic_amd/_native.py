@@ -20,7 +20,7 @@ PIB_ERR_STATE = -3
 PIB_ERR_UNSUPPORTED = -4
 
 SET_RHO, SET_VIB, SET_VIB_PLUS, SET_VIB_MINUS = 0, 1, 2, 3
-EVAL_PM, EVAL_NOSCALE = 1, 2
+EVAL_PM, EVAL_NOSCALE, EVAL_WIDE = 1, 2, 4
 OUT_RHO, OUT_G, OUT_GP, OUT_GM, OUT_LOGS = 0, 1, 2, 3, 4
 NUM_OUT = 5
 (SUM_N, SUM_R, SUM_R2, SUM_ABSR, SUM_NEG, SUM_D1, SUM_D1SQ, SUM_D2, SUM_D2SQ,

@@ -30,6 +30,7 @@ struct pib_ctx {
   // coupling
   int A = 0, N = 0;
   DevBuf energy, linear, qdiag, pair, qpair;
+  DevBuf w_lq, w_e, w_qpair;  // padded 16 x 16 constants of the wide kernel (A <= 16)
   int npair = 0;
   bool has_qdiag = false;
   bool coupling_set = false;
@@ -148,6 +149,9 @@ int pib_ctx_destroy(pib_ctx* c) {
   release(c->qdiag);
   release(c->pair);
   release(c->qpair);
+  release(c->w_lq);
+  release(c->w_e);
+  release(c->w_qpair);
   for (int s = 0; s < NUM_SETS; ++s) {
     release(c->coth[s]);
     release(c->csch[s]);
@@ -213,6 +217,26 @@ int pib_set_coupling(pib_ctx* c, int A, int N, const double* E, const double* L,
     if ((r = upload(c, c->pair, pairs.data(), pairs.size() * sizeof(int)))) return r;
     if ((r = upload(c, c->qpair, qp.data(), qp.size() * sizeof(double)))) return r;
   }
+  if (A <= kWideMaxStates) {
+    // 16 x 16 padded copies for the wide kernel; element (b, c) of every matrix is taken
+    // from the lower triangle (max(b,c), min(b,c)), as eigh(UPLO='L') reads V
+    constexpr int W = kWideMaxStates;
+    auto lo = [A](int b, int cc) { return b >= cc ? b * A + cc : cc * A + b; };
+    std::vector<double> lq((size_t)N * W * W * 2, 0.0), ew((size_t)W * W, 0.0);
+    std::vector<double> qw((size_t)std::max(c->npair, 1) * W * W, 0.0);
+    for (int b = 0; b < A; ++b)
+      for (int cc = 0; cc < A; ++cc) {
+        ew[b * W + cc] = E[lo(b, cc)];
+        for (int j = 0; j < N; ++j) {
+          lq[(((size_t)j * W + b) * W + cc) * 2] = L[(size_t)j * AA + lo(b, cc)];
+          lq[(((size_t)j * W + b) * W + cc) * 2 + 1] = qd[(size_t)j * AA + lo(b, cc)];
+        }
+        for (int t = 0; t < c->npair; ++t) qw[((size_t)t * W + b) * W + cc] = qp[(size_t)t * AA + lo(b, cc)];
+      }
+    if ((r = upload(c, c->w_lq, lq.data(), lq.size() * sizeof(double)))) return r;
+    if ((r = upload(c, c->w_e, ew.data(), ew.size() * sizeof(double)))) return r;
+    if ((r = upload(c, c->w_qpair, qw.data(), qw.size() * sizeof(double)))) return r;
+  }
   c->A = A;
   c->N = N;
   c->coupling_set = true;
@@ -266,7 +290,8 @@ int pib_eval_block(pib_ctx* c, const double* q, int64_t B, int flags, double* ou
                    int64_t out_stride, double* dbg, void* stream) {
   if (!c) return PIB_ERR_ARG;
   const bool pm = (flags & PIB_EVAL_PM) != 0;
-  if (flags & ~(PIB_EVAL_PM | PIB_EVAL_NOSCALE)) return fail(c, PIB_ERR_ARG, "eval_block: unknown flag");
+  if (flags & ~(PIB_EVAL_PM | PIB_EVAL_NOSCALE | PIB_EVAL_WIDE))
+    return fail(c, PIB_ERR_ARG, "eval_block: unknown flag");
   if (!c->coupling_set || c->P < 3) return fail(c, PIB_ERR_STATE, "eval_block: coupling or beads not set");
   const int need = pm ? 4 : 2;
   for (int s = 0; s < need; ++s)
@@ -276,7 +301,9 @@ int pib_eval_block(pib_ctx* c, const double* q, int64_t B, int flags, double* ou
       return fail(c, PIB_ERR_ARG, "eval_block: coupled-model O-matrix set has the wrong surface/mode count");
   if (c->set_N[0] != c->N) return fail(c, PIB_ERR_ARG, "eval_block: sampling model has a different number of modes");
   const int AR = c->set_A[SET_RHO];
-  if (!eval_supported(c->A, AR))
+  const bool fits_wide = c->A <= kWideMaxStates && AR <= kWideMaxStates;
+  const bool wide = fits_wide && ((flags & PIB_EVAL_WIDE) || !eval_supported(c->A, AR));
+  if (!wide && !eval_supported(c->A, AR))
     return fail(c, PIB_ERR_UNSUPPORTED, "eval_block: no FP64 kernel for A=" + std::to_string(c->A) +
                                              ", A_rho=" + std::to_string(AR));
   if (B < 0 || out_stride < B) return fail(c, PIB_ERR_ARG, "eval_block: B < 0 or out_stride < B");
@@ -307,8 +334,14 @@ int pib_eval_block(pib_ctx* c, const double* q, int64_t B, int flags, double* ou
   a.out = out;
   a.out_stride = out_stride;
   a.dbg = dbg;
+  a.A = c->A;
+  a.AR = AR;
+  a.wide_lq = (const double2*)c->w_lq.p;
+  a.wide_e = (const double*)c->w_e.p;
+  a.wide_qpair = (const double*)c->w_qpair.p;
   PIB_HIP(c, hipSetDevice(c->device));
-  hipError_t e = launch_eval_fp64(c->A, AR, pm ? 1 : 0, a, (hipStream_t)stream);
+  hipError_t e = wide ? launch_eval_wide(pm ? 1 : 0, a, (hipStream_t)stream)
+                      : launch_eval_fp64(c->A, AR, pm ? 1 : 0, a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "eval_block launch");
   return PIB_OK;
 }

@@ -1,0 +1,376 @@
+// FP64 per-sample evaluation for large surface counts (A <= 16, padded to 16), one wave per
+// sample.
+//
+// Same estimator as k_eval_fp64 (pibronic/pimc/pimc.py:1087-1213 fused; block loops
+// :1341-1462), mapped for matrices that do not fit one lane's registers:
+//
+//  * Beads are processed four at a time. Lane (g, j) = (lane / 16, lane % 16) owns column j
+//    of the coupling matrix of bead 4c + g: it assembles V[:, j] from the linear and
+//    quadratic couplings, forms the O-factor of surface j for every temperature set, and
+//    takes part in the per-bead scaling maximum over its 16-lane group.
+//  * Eigen-decomposition: one-sided (Hestenes) Jacobi on G = V + sigma I, sigma = 1.5 |V|_F,
+//    which is positive definite with condition number <= 5. Its singular vectors are the
+//    eigenvectors of V and lambda_j = |G u_j| - sigma. A sweep is the 15 perfect matchings
+//    {j, j xor k}, k = 1..15 (every pair once); partners swap columns with cross-lane moves,
+//    so no rotation parameters have to be broadcast. The eigenvectors are the normalised
+//    columns of the rotated G (Demmel & Veselic, "Jacobi's method is more accurate than QR",
+//    SIAM J. Matrix Anal. Appl. 13 (1992), for the one-sided scheme).
+//  * M = U diag(exp(-tau lambda)) U^T and the ring product run on the matrix cores: 16x16x4
+//    FP64 MFMA tiles. The FP64 accumulator layout is: register r of lane l holds
+//    D[l/16 + 4r][l%16] (measured on gfx950). Permuting the sum index of each tile step
+//    (k = lane/16 + 4*step) makes that layout directly the operand layout of the next
+//    product, so the chain never leaves registers.
+//    The chain is carried transposed, R^T <- diag(O_p / S_p) M_p R^T, whose trace equals the
+//    reference's tr(prod_p M_p O_p) because M_p is symmetric and O_p diagonal.
+//
+// Padding: columns j >= A hold sigma e_j, are never rotated (their inner products with real
+// columns are exactly zero) and carry O = 0, so they drop out of every trace.
+#include "pib_common.hpp"
+#include "pib_kernels.hpp"
+
+namespace pib {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kW = 16;                 // padded matrix dimension = lanes per bead
+constexpr int kBeadsPerChunk = kWave / kW;
+constexpr int kWideWaves = 2;          // waves (samples) per work-group
+constexpr int kURow = kW + 1;          // LDS row stride of the eigenvector tiles (bank spread)
+
+struct WideLds {
+  double U[kBeadsPerChunk][kW * kURow];  // U[b][i * kURow + m] = eigenvector m, component i
+  double D[kBeadsPerChunk][kW];          // exp(-tau lambda_m)
+  double O[3][kBeadsPerChunk][kW];       // scaled O-factors (vib, vib+, vib-)
+};
+
+__device__ __forceinline__ double rcp_nr_w(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
+__device__ __forceinline__ double rsq_nr_w(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-(x * y), y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
+__device__ __forceinline__ double oterm_w(double coth, double csch2, double x1, double x2) {
+  return coth * (x1 * x1 + x2 * x2) - (csch2 * x1) * x2;
+}
+
+// Wave-local ordering of LDS traffic: every access of the wave before this point is complete
+// and visible to every lane of the same wave after it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// sum / max over the 16 lanes of a group; the xor butterfly gives every lane the same bits
+// (a + b == b + a exactly)
+__device__ __forceinline__ double group_sum(double x) {
+#pragma unroll
+  for (int m = 1; m < kW; m <<= 1) x += shfl_xor_d(x, m);
+  return x;
+}
+__device__ __forceinline__ double group_max(double x) {
+#pragma unroll
+  for (int m = 1; m < kW; m <<= 1) x = fmax(x, shfl_xor_d(x, m));
+  return x;
+}
+
+template <int K>
+struct HiBit {
+  static constexpr int v = (K >= 8) ? 8 : (K >= 4) ? 4 : (K >= 2) ? 2 : 1;
+};
+
+// One step of the sweep: lane j and lane j ^ K orthogonalise their columns.
+template <int K>
+__device__ __forceinline__ void hestenes_step(double (&v)[kW], double& al, int j, double tol2,
+                                              bool& busy) {
+  double pc[kW];
+#pragma unroll
+  for (int i = 0; i < kW; ++i) pc[i] = shfl_xor_d(v[i], K);
+  const double be = shfl_xor_d(al, K);
+  double gam = 0.0;
+#pragma unroll
+  for (int i = 0; i < kW; ++i) gam = fma(v[i], pc[i], gam);
+  const bool low = (j & HiBit<K>::v) == 0;  // this lane holds the lower column index p
+  const double app = low ? al : be, aqq = low ? be : al;
+  // Gram block [[app, gam], [gam, aqq]]: t = tan of the rotation that zeroes gam (smaller
+  // root, |t| <= 1); gam == 0 gives t = 0, an exact no-op (see jacobi_rotate)
+  const double d = aqq - app, h = 2.0 * gam;
+  const double x = fma(d, d, h * h);
+  const double r = x * rsq_nr_w(x > 0.0 ? x : 1.0);
+  const double den = fabs(d) + r;
+  double t = h * rcp_nr_w(den > 0.0 ? den : 1.0);
+  t = d < 0.0 ? -t : t;
+  const double c = rsq_nr_w(fma(t, t, 1.0));
+  const double s = t * c;
+  busy |= (gam * gam > tol2 * (app * aqq));
+  const double sg = low ? -s : s;  // p: c g_p - s g_q ; q: s g_p + c g_q
+#pragma unroll
+  for (int i = 0; i < kW; ++i) v[i] = fma(c, v[i], sg * pc[i]);
+  double a2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < kW; ++i) a2 = fma(v[i], v[i], a2);
+  al = a2;
+}
+
+template <bool PM, bool DBG>
+__global__ void __launch_bounds__(kWave * kWideWaves) k_eval_wide(const EvalArgs args) {
+  __shared__ WideLds lds_all[kWideWaves];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  WideLds& L = lds_all[wv];
+  const int64_t sample = (int64_t)blockIdx.x * kWideWaves + wv;
+  if (sample >= args.B) return;  // whole wave; no work-group barrier is used below
+  const int g = lane / kW, j = lane % kW;
+  const int A = args.A, AR = args.AR, N = args.N, P = args.P;
+  const bool sep = AR != A;
+  const bool jr = j < AR, jv = j < A;
+  const double* __restrict__ qs = args.q + sample * (int64_t)N * P;
+  const OSet sr = args.set[SET_RHO], sv = args.set[SET_VIB];
+  const OSet sp = args.set[SET_VIB_PLUS], sm = args.set[SET_VIB_MINUS];
+  const double2* __restrict__ LQ = args.wide_lq;
+  const double* __restrict__ Ew = args.wide_e;
+  const int* __restrict__ pairs = args.pair;
+  const double* __restrict__ Qp = args.wide_qpair;
+  const double tol2 = 1e-26;  // |g_p . g_q| <= 1e-13 |g_p| |g_q| for every pair ends the sweeps
+  constexpr int kMaxSweeps = 16;
+
+  // transposed ring products in accumulator layout: reg r of lane l = R^T[l/16 + 4r][l%16]
+  d4 Rg, Rp, Rm;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    Rg[r] = (g + 4 * r == j) ? 1.0 : 0.0;
+    Rp[r] = Rg[r];
+    Rm[r] = Rg[r];
+  }
+  double rp = 1.0;  // rho: product over this group's beads of O_rho,j / S
+  double smant = 1.0;
+  int sexp = 0;
+
+  for (int p0 = 0; p0 < P; p0 += kBeadsPerChunk) {
+    const int p = p0 + g;
+    const bool bead = p < P;  // uniform over the 16-lane group
+    const int pc_ = bead ? p : P - 1;
+    const int pn = (pc_ + 1 == P) ? 0 : pc_ + 1;
+
+    double er = 0.0, ev = 0.0, ep = 0.0, em = 0.0;
+    double v[kW];
+#pragma unroll
+    for (int i = 0; i < kW; ++i) v[i] = 0.0;
+    for (int d = 0; d < N; ++d) {
+      const double x0 = qs[d * P + pc_], xn = qs[d * P + pn];
+      double q1 = x0, q1n = xn;
+      if (jr) {  // sampling model: q -= d_rho; O_rho; q += d_rho (pimc.py:1091-1127)
+        const double dd = sr.shift[j * N + d];
+        const double x1 = x0 - dd, x2 = xn - dd;
+        er += oterm_w(sr.coth[d], 2.0 * sr.csch[d], x1, x2);
+        if (!sep) { q1 = x1 + dd; q1n = x2 + dd; }
+      }
+      if (jv) {  // coupled model at tau (and tau +- dbeta/P)
+        const double dd = sv.shift[j * N + d];
+        const double y1 = q1 - dd, y2 = q1n - dd;
+        ev += oterm_w(sv.coth[d], 2.0 * sv.csch[d], y1, y2);
+        if (PM) {
+          const double z1 = (y1 + dd) - dd, z2 = (y2 + dd) - dd;
+          ep += oterm_w(sp.coth[d], 2.0 * sp.csch[d], z1, z2);
+          const double w1 = (z1 + dd) - dd, w2 = (z2 + dd) - dd;
+          em += oterm_w(sm.coth[d], 2.0 * sm.csch[d], w1, w2);
+        }
+      }
+      // column j of V (lower triangle mirrored): sum_d x_d (L + Qh x_d)  (pimc.py:1147-1160)
+      const double2* lq = LQ + (size_t)d * kW * kW + j;
+#pragma unroll
+      for (int i = 0; i < kW; ++i) {
+        if (i < A) {
+          const double2 c = lq[i * kW];
+          v[i] = fma(x0, fma(c.y, x0, c.x), v[i]);
+        }
+      }
+    }
+    for (int t = 0; t < args.npair; ++t) {  // off-diagonal quadratic mode pairs
+      const int jj = pairs[2 * t], kk = pairs[2 * t + 1];
+      const double xj = qs[jj * P + pc_], xk = qs[kk * P + pc_];
+      const double* qp = Qp + (size_t)t * kW * kW + j;
+#pragma unroll
+      for (int i = 0; i < kW; ++i)
+        if (i < A) v[i] = fma(xk * qp[i * kW], xj, v[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < kW; ++i)
+      if (i < A) v[i] += Ew[i * kW + j];
+
+    // O-factors and the per-bead scaling S_p = max over both O's (pimc.py:1076-1084)
+    const double Or = jr ? exp(-0.5 * er) * sr.pref[j] : 0.0;
+    const double Ov = jv ? exp(-0.5 * ev) * sv.pref[j] : 0.0;
+    double Opl = 0.0, Omi = 0.0;
+    if (PM && jv) {
+      Opl = exp(-0.5 * ep) * sp.pref[j];
+      Omi = exp(-0.5 * em) * sm.pref[j];
+    }
+    double S = group_max(fmax(Or, Ov));
+    if (args.noscale) S = 1.0;
+
+    if (DBG && bead) {
+      const int rec = eval_debug_record(A, AR);
+      double* dr = args.dbg + ((int64_t)sample * P + p) * rec;
+      if (jr) dr[j] = Or;
+      if (jv) dr[AR + j] = Ov;
+      if (j == 0) dr[AR + A] = S;
+    }
+
+    // G = V + sigma I, one-sided Jacobi
+    double f = 0.0;
+#pragma unroll
+    for (int i = 0; i < kW; ++i) f = fma(v[i], v[i], f);
+    f = group_sum(f);
+    const double sig = 1.5 * (f * rsq_nr_w(f > 0.0 ? f : 1.0)) + 1e-100;
+#pragma unroll
+    for (int i = 0; i < kW; ++i) v[i] += (i == j) ? sig : 0.0;
+    double al = 0.0;
+#pragma unroll
+    for (int i = 0; i < kW; ++i) al = fma(v[i], v[i], al);
+    for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
+      bool busy = false;
+      hestenes_step<1>(v, al, j, tol2, busy);
+      hestenes_step<2>(v, al, j, tol2, busy);
+      hestenes_step<3>(v, al, j, tol2, busy);
+      hestenes_step<4>(v, al, j, tol2, busy);
+      hestenes_step<5>(v, al, j, tol2, busy);
+      hestenes_step<6>(v, al, j, tol2, busy);
+      hestenes_step<7>(v, al, j, tol2, busy);
+      hestenes_step<8>(v, al, j, tol2, busy);
+      hestenes_step<9>(v, al, j, tol2, busy);
+      hestenes_step<10>(v, al, j, tol2, busy);
+      hestenes_step<11>(v, al, j, tol2, busy);
+      hestenes_step<12>(v, al, j, tol2, busy);
+      hestenes_step<13>(v, al, j, tol2, busy);
+      hestenes_step<14>(v, al, j, tol2, busy);
+      hestenes_step<15>(v, al, j, tol2, busy);
+      if (!__any(busy)) break;
+    }
+    const double rs = rsq_nr_w(al);
+    const double lam = al * rs - sig;
+    const double inv = 1.0 / S;
+
+    wave_sync();  // the previous chunk's ring steps have read the tiles
+#pragma unroll
+    for (int i = 0; i < kW; ++i) L.U[g][i * kURow + j] = v[i] * rs;
+    L.D[g][j] = exp(-args.tau * lam);
+    L.O[0][g][j] = Ov * inv;
+    if (PM) {
+      L.O[1][g][j] = Opl * inv;
+      L.O[2][g][j] = Omi * inv;
+    }
+    if (bead) {
+      rp *= Or * inv;
+      int e;
+      smant = frexp(smant * S, &e);
+      sexp += e;
+    }
+    wave_sync();
+
+    // ring steps, beads in order (pimc.py:1198-1206)
+    const int nb = min(kBeadsPerChunk, P - p0);
+    for (int b = 0; b < nb; ++b) {
+      d4 M = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) {
+        const int m = g + 4 * kb;
+        const double u = L.U[b][j * kURow + m];
+        M = __builtin_amdgcn_mfma_f64_16x16x4f64(u * L.D[b][m], u, M, 0, 0, 0);
+      }
+      if (DBG) {
+        const int rec = eval_debug_record(A, AR);
+        double* dr = args.dbg + ((int64_t)sample * P + p0 + b) * rec + AR + A + 1;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = g + 4 * r;
+          if (row < A && j < A) dr[row * A + j] = M[r];
+        }
+      }
+      d4 Z = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) Z = __builtin_amdgcn_mfma_f64_16x16x4f64(M[kb], Rg[kb], Z, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Rg[r] = Z[r] * L.O[0][b][g + 4 * r];
+      if (PM) {
+        d4 Zp = {0.0, 0.0, 0.0, 0.0}, Zm = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+          Zp = __builtin_amdgcn_mfma_f64_16x16x4f64(M[kb], Rp[kb], Zp, 0, 0, 0);
+          Zm = __builtin_amdgcn_mfma_f64_16x16x4f64(M[kb], Rm[kb], Zm, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          Rp[r] = Zp[r] * L.O[1][b][g + 4 * r];
+          Rm[r] = Zm[r] * L.O[2][b][g + 4 * r];
+        }
+      }
+    }
+  }
+
+  // traces: lane l holds diagonal element (j, j) iff j % 4 == l / 16, in register j / 4
+  const bool diag = (j & 3) == g;
+  double tg = 0.0, tp = 0.0, tm = 0.0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (diag && (j >> 2) == r) {
+      tg = Rg[r];
+      tp = Rp[r];
+      tm = Rm[r];
+    }
+  }
+  double ls = log(smant) + (double)sexp * 0.69314718055994530942;
+  // rho: product over the four bead groups, then sum over surfaces
+  rp *= shfl_xor_d(rp, 16);
+  rp *= shfl_xor_d(rp, 32);
+  double rho = group_sum(jr ? rp : 0.0);
+  ls += shfl_xor_d(ls, 16);
+  ls += shfl_xor_d(ls, 32);
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) {
+    tg += shfl_xor_d(tg, m);
+    if (PM) {
+      tp += shfl_xor_d(tp, m);
+      tm += shfl_xor_d(tm, m);
+    }
+  }
+  if (lane == 0) {
+    const int64_t st = args.out_stride;
+    args.out[OUT_RHO * st + sample] = rho;
+    args.out[OUT_G * st + sample] = tg;
+    if (PM) {
+      args.out[OUT_GP * st + sample] = tp;
+      args.out[OUT_GM * st + sample] = tm;
+    }
+    args.out[OUT_LOGS * st + sample] = ls;
+  }
+}
+
+hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s) {
+  const dim3 grid((unsigned)((a.B + kWideWaves - 1) / kWideWaves));
+  const dim3 block(kWave * kWideWaves);
+  const bool dbg = a.dbg != nullptr;
+  if (pm && dbg)
+    hipLaunchKernelGGL((k_eval_wide<true, true>), grid, block, 0, s, a);
+  else if (pm)
+    hipLaunchKernelGGL((k_eval_wide<true, false>), grid, block, 0, s, a);
+  else if (dbg)
+    hipLaunchKernelGGL((k_eval_wide<false, true>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_eval_wide<false, false>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace pib

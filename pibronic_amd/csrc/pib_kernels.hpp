@@ -37,6 +37,11 @@ struct EvalArgs {
   double* out;            // NUM_OUT rows of out_stride doubles
   int64_t out_stride;
   double* dbg;            // optional per-(sample, bead) dump, see eval_debug_record()
+  // wide kernel (A <= 16): constants padded to 16 x 16, lower triangle mirrored
+  int A, AR;
+  const double2* wide_lq;     // N * 16 * 16 pairs {L[j,b,c], 0.5 Q[j,j,b,c]}
+  const double* wide_e;       // 16 * 16
+  const double* wide_qpair;   // npair * 16 * 16
 };
 
 __host__ __device__ inline int eval_debug_record(int A, int AR) { return AR + A + 1 + A * A; }
@@ -71,6 +76,8 @@ struct SumArgs {
 
 // launchers (return hipError_t of the launch)
 hipError_t launch_eval_fp64(int A, int AR, int pm, const EvalArgs& a, hipStream_t s);
+hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s);
+constexpr int kWideMaxStates = 16;
 hipError_t launch_sample_fp64(const SampleArgs& a, hipStream_t s);
 hipError_t launch_block_sums(const SumArgs& a, double* partials, int nparts, double* sums,
                              hipStream_t s);

@@ -245,6 +245,9 @@ class BoxData:
         self.hash_vib = self.hash_rho = None
         self.vib = self.rho = None
         self.sampler = "device"
+        # True: run the one-wave-per-sample kernel (16x16 padded matrices) whatever A is;
+        # it is chosen automatically for A > 8
+        self.force_wide = False
         self.seed = None
         self.device = 0
         self._ctx = None
@@ -423,7 +426,8 @@ class BoxData:
     def evaluate_device(self, q, B, pm, out, out_stride, debug=None, noscale=False, stream=None):
         """One pib_eval_block call on device buffers (torch tensors)."""
         import torch
-        flags = (_native.EVAL_PM if pm else 0) | (_native.EVAL_NOSCALE if noscale else 0)
+        flags = ((_native.EVAL_PM if pm else 0) | (_native.EVAL_NOSCALE if noscale else 0)
+                 | (_native.EVAL_WIDE if self.force_wide else 0))
         if stream is None:
             stream = torch.cuda.current_stream(self.device)
         self.context().eval_block(q, B, flags, out, out_stride, debug, stream)
