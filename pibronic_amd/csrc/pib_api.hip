@@ -49,6 +49,8 @@ struct pib_ctx {
   // reduction scratch
   DevBuf partials;
   int nparts = 0;
+  // split-K GEMM and column-sum scratch (training calls)
+  DevBuf work;
   // flow basis
   int f_N = 0, f_P = 0;
   DevBuf f_U, f_sigma, f_isigma, f_m;
@@ -89,6 +91,22 @@ int upload(pib_ctx* c, DevBuf& b, const void* host, size_t bytes) {
     PIB_HIP(c, hipDeviceSynchronize());
     PIB_HIP(c, hipMemcpy(b.p, host, bytes, hipMemcpyHostToDevice));
   }
+  return PIB_OK;
+}
+
+// Make b hold at least `bytes` of device memory (contents undefined). Synchronises the
+// device before replacing a buffer that queued kernels may still use.
+int reserve(pib_ctx* c, DevBuf& b, size_t bytes) {
+  if (b.p && b.bytes >= bytes) return PIB_OK;
+  PIB_HIP(c, hipSetDevice(c->device));
+  if (b.p) {
+    PIB_HIP(c, hipDeviceSynchronize());
+    PIB_HIP(c, hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  PIB_HIP(c, hipMalloc(&b.p, bytes));
+  b.bytes = bytes;
   return PIB_OK;
 }
 
@@ -162,6 +180,7 @@ int pib_ctx_destroy(pib_ctx* c) {
   release(c->s_chol);
   release(c->s_shift);
   release(c->partials);
+  release(c->work);
   release(c->f_U);
   release(c->f_sigma);
   release(c->f_isigma);
@@ -479,7 +498,8 @@ int pib_gemm_f32(pib_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, co
                  const float* bias, int relu, const float* mask, int64_t ldmask, int accumulate,
                  void* stream) {
   if (!c) return PIB_ERR_ARG;
-  GemmArgs g{A, B, C, bias, mask, M, N, K, lda, ldb, ldc, ldmask, relu ? 1 : 0, accumulate ? 1 : 0};
+  GemmArgs g{A, B, C, bias, mask, M, N, K, lda, ldb, ldc, ldmask, relu ? 1 : 0, accumulate ? 1 : 0,
+             0, nullptr};
   if (!A || !B || !C) return fail(c, PIB_ERR_ARG, "gemm_f32: null operand");
   if (!gemm_shape_ok(g)) return fail(c, PIB_ERR_ARG, "gemm_f32: M and N must be multiples of 64, K of 16");
   const int64_t rowsA = ta ? K : M, colsA = ta ? M : K, rowsB = tb ? N : K, colsB = tb ? K : N;
@@ -487,6 +507,13 @@ int pib_gemm_f32(pib_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, co
   if (lda < colsA || ldb < colsB || ldc < N || (mask && ldmask < N))
     return fail(c, PIB_ERR_ARG, "gemm_f32: leading dimension smaller than the row length");
   PIB_HIP(c, hipSetDevice(c->device));
+  const int splits = gemm_splits(g, c->num_cu);
+  if (splits > 1) {
+    int r;
+    if ((r = reserve(c, c->work, (size_t)splits * M * N * sizeof(float)))) return r;
+    g.kchunk = K / splits;
+    g.partial = (float*)c->work.p;
+  }
   hipError_t e = launch_gemm_f32(ta != 0, tb != 0, g, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "gemm_f32 launch");
   return PIB_OK;
@@ -603,8 +630,10 @@ int pib_colsum_f32(pib_ctx* c, int64_t M, int64_t N, const float* X, int64_t ldx
                    void* stream) {
   if (!c) return PIB_ERR_ARG;
   if (M < 1 || N < 1 || ldx < N || !X || !out) return fail(c, PIB_ERR_ARG, "colsum_f32: bad size or null pointer");
+  int r;
+  if ((r = reserve(c, c->work, (size_t)colsum_chunks(M) * N * sizeof(double)))) return r;
   PIB_HIP(c, hipSetDevice(c->device));
-  hipError_t e = launch_colsum_f32(M, N, X, ldx, out, (hipStream_t)stream);
+  hipError_t e = launch_colsum_f32(M, N, X, ldx, out, (double*)c->work.p, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "colsum_f32 launch");
   return PIB_OK;
 }

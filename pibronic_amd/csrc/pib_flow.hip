@@ -71,43 +71,42 @@ hipError_t launch_flow_post(int mode, const BasisArgs& b, const float* in, void*
 }
 
 // ------------------------------------------------------------------ block reduction helper
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (t < w) sh[t] += sh[t + w];
-    __syncthreads();
-  }
-  const double r = sh[0];
-  __syncthreads();
-  return r;
-}
 
 // ------------------------------------------------------------------ coupling layers
 // forward:  y_B = z_B * exp(s) + t,   logdet += sum s
 // inverse:  z_B = (y_B - t) * exp(-s), logdet -= sum s
-// with s = smax * tanh(raw / smax) (bounded scales), one work-group per sample row
-__global__ void __launch_bounds__(256) k_coupling(int inverse, CouplingArgs a) {
-  __shared__ double sh[256];
-  const int64_t b = blockIdx.x;
+// with s = smax * tanh(raw / smax) (bounded scales). One wave per sample row, four rows per
+// work-group; the logdet sum is a fixed-order xor butterfly over the wave.
+constexpr int kRowsPerBlock = 4;
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(64 * kRowsPerBlock) k_coupling(int inverse, CouplingArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / 64;
+  if (b >= a.B) return;  // whole wave
   const int H = a.D / 2;
   const float* o = a.o + b * a.D;
   float* z = a.z + b * a.D + a.b_off;
   double part = 0.0;
-  for (int i = threadIdx.x; i < H; i += blockDim.x) {
+  for (int i = lane; i < H; i += 64) {
     const float s = a.smax * tanhf(o[i] / a.smax);
     const float t = o[H + i];
     z[i] = inverse ? (z[i] - t) * expf(-s) : fmaf(z[i], expf(s), t);
     part += (double)s;
   }
-  const double tot = block_sum(part, sh);
-  if (threadIdx.x == 0) a.logdet[b] += inverse ? -tot : tot;
+  const double tot = wave_sum_d(part);
+  if (lane == 0) a.logdet[b] += inverse ? -tot : tot;
 }
 
 hipError_t launch_coupling(int inverse, const CouplingArgs& a, hipStream_t s) {
   if (a.B < 1) return hipSuccess;
-  hipLaunchKernelGGL(k_coupling, dim3((unsigned)a.B), dim3(256), 0, s, inverse, a);
+  hipLaunchKernelGGL(k_coupling, dim3((unsigned)((a.B + kRowsPerBlock - 1) / kRowsPerBlock)),
+                     dim3(64 * kRowsPerBlock), 0, s, inverse, a);
   return hipGetLastError();
 }
 
@@ -153,34 +152,37 @@ hipError_t launch_coupling_inverse_backward(const CouplingArgs& a, hipStream_t s
 // log rho is the responsibility-weighted sum over surfaces. One work-group per sample.
 constexpr int kMaxRhoSurfaces = 16;
 
-__global__ void __launch_bounds__(256) k_logrho(LogRhoArgs a) {
-  __shared__ double sh[256];
-  __shared__ double resp[kMaxRhoSurfaces];
-  const int64_t b = blockIdx.x;
+__global__ void __launch_bounds__(64 * kRowsPerBlock) k_logrho(LogRhoArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / 64;
+  if (b >= a.B) return;  // whole wave
   const int N = a.N, P = a.P, NP = N * P;
   const double* R = a.R + b * (int64_t)NP;
   double part[kMaxRhoSurfaces];
 #pragma unroll
   for (int s = 0; s < kMaxRhoSurfaces; ++s) part[s] = 0.0;
-  for (int e = threadIdx.x; e < NP; e += blockDim.x) {
+  for (int e = lane; e < NP; e += 64) {
     const int j = e / P, p = e - j * P;
     const int pn = (p + 1 == P) ? 0 : p + 1;
     const double r0 = R[e], r1 = R[j * P + pn];
+    const double cj = a.coth[j], sj2 = 2.0 * a.csch[j];
 #pragma unroll
     for (int s = 0; s < kMaxRhoSurfaces; ++s) {
       if (s < a.A) {
         const double d = a.shift[s * N + j];
         const double x1 = r0 - d, x2 = r1 - d;
-        part[s] += a.coth[j] * (x1 * x1 + x2 * x2) - (2.0 * a.csch[j] * x1) * x2;
+        part[s] += cj * (x1 * x1 + x2 * x2) - (sj2 * x1) * x2;
       }
     }
   }
+  // every lane ends with the same sums (xor butterfly), so every lane holds la, lmax, tot
   double lmax = -INFINITY;
   double la[kMaxRhoSurfaces];
 #pragma unroll
   for (int s = 0; s < kMaxRhoSurfaces; ++s) {
+    la[s] = -INFINITY;
     if (s < a.A) {
-      la[s] = P * log(a.pref[s]) - 0.5 * block_sum(part[s], sh);
+      la[s] = P * log(a.pref[s]) - 0.5 * wave_sum_d(part[s]);
       lmax = fmax(lmax, la[s]);
     }
   }
@@ -188,23 +190,23 @@ __global__ void __launch_bounds__(256) k_logrho(LogRhoArgs a) {
 #pragma unroll
   for (int s = 0; s < kMaxRhoSurfaces; ++s)
     if (s < a.A) tot += exp(la[s] - lmax);
-  if (threadIdx.x == 0) {
-    if (a.logrho) a.logrho[b] = lmax + log(tot);
-#pragma unroll
-    for (int s = 0; s < kMaxRhoSurfaces; ++s)
-      if (s < a.A) resp[s] = exp(la[s] - lmax) / tot;
-  }
-  __syncthreads();
+  if (lane == 0 && a.logrho) a.logrho[b] = lmax + log(tot);
   if (!a.dR) return;
+  double resp[kMaxRhoSurfaces];
+#pragma unroll
+  for (int s = 0; s < kMaxRhoSurfaces; ++s) resp[s] = (s < a.A) ? exp(la[s] - lmax) / tot : 0.0;
   const double cb = a.c ? a.c[b] : 1.0;
-  for (int e = threadIdx.x; e < NP; e += blockDim.x) {
+  for (int e = lane; e < NP; e += 64) {
     const int j = e / P, p = e - j * P;
     const int pn = (p + 1 == P) ? 0 : p + 1, pp = (p == 0) ? P - 1 : p - 1;
     const double r0 = R[e], rn = R[j * P + pn], rp = R[j * P + pp];
     double g = 0.0;
-    for (int s = 0; s < a.A; ++s) {
-      const double d = a.shift[s * N + j];
-      g -= resp[s] * (2.0 * a.coth[j] * (r0 - d) - a.csch[j] * ((rn - d) + (rp - d)));
+#pragma unroll
+    for (int s = 0; s < kMaxRhoSurfaces; ++s) {
+      if (s < a.A) {
+        const double d = a.shift[s * N + j];
+        g -= resp[s] * (2.0 * a.coth[j] * (r0 - d) - a.csch[j] * ((rn - d) + (rp - d)));
+      }
     }
     a.dR[b * (int64_t)NP + e] = (float)(-cb * g);
   }
@@ -213,23 +215,51 @@ __global__ void __launch_bounds__(256) k_logrho(LogRhoArgs a) {
 hipError_t launch_logrho(const LogRhoArgs& a, hipStream_t s) {
   if (a.A > kMaxRhoSurfaces) return hipErrorInvalidValue;
   if (a.B < 1) return hipSuccess;
-  hipLaunchKernelGGL(k_logrho, dim3((unsigned)a.B), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_logrho, dim3((unsigned)((a.B + kRowsPerBlock - 1) / kRowsPerBlock)),
+                     dim3(64 * kRowsPerBlock), 0, s, a);
   return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ column sums, Adam
-__global__ void k_colsum(int64_t M, int64_t N, const float* X, int64_t ldx, float* out) {
+// Pass 1: work-group (column tile of 64, row chunk of kColsumRows) -> partial[chunk][col], the
+// 256 threads as 64 columns x 4 row phases combined through LDS. Pass 2: per column, the
+// chunks in order.
+constexpr int kColsumRows = 64;
+
+int64_t colsum_chunks(int64_t M) { return (M + kColsumRows - 1) / kColsumRows; }
+
+__global__ void __launch_bounds__(256) k_colsum_partial(int64_t M, int64_t N, const float* X,
+                                                        int64_t ldx, double* partial) {
+  __shared__ double sh[4][64];
+  const int c = threadIdx.x & 63, ph = threadIdx.x >> 6;
+  const int64_t col = (int64_t)blockIdx.x * 64 + c;
+  const int64_t r0 = (int64_t)blockIdx.y * kColsumRows;
+  double acc = 0.0;
+  if (col < N)
+    for (int64_t i = r0 + ph; i < min(r0 + kColsumRows, M); i += 4) acc += X[i * ldx + col];
+  sh[ph][c] = acc;
+  __syncthreads();
+  if (ph == 0 && col < N)
+    partial[(int64_t)blockIdx.y * N + col] = ((sh[0][c] + sh[1][c]) + sh[2][c]) + sh[3][c];
+}
+
+__global__ void k_colsum_final(int64_t chunks, int64_t N, const double* partial, float* out) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < N;
        j += (int64_t)gridDim.x * blockDim.x) {
     double acc = 0.0;
-    for (int64_t i = 0; i < M; ++i) acc += X[i * ldx + j];
+    for (int64_t k = 0; k < chunks; ++k) acc += partial[k * N + j];
     out[j] = (float)acc;
   }
 }
 
 hipError_t launch_colsum_f32(int64_t M, int64_t N, const float* X, int64_t ldx, float* out,
-                             hipStream_t s) {
-  hipLaunchKernelGGL(k_colsum, grid_for(N, 64), dim3(64), 0, s, M, N, X, ldx, out);
+                             double* partial, hipStream_t s) {
+  const int64_t chunks = colsum_chunks(M);
+  hipLaunchKernelGGL(k_colsum_partial, dim3((unsigned)((N + 63) / 64), (unsigned)chunks),
+                     dim3(256), 0, s, M, N, X, ldx, partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_colsum_final, grid_for(N, 256), dim3(256), 0, s, chunks, N, partial, out);
   return hipGetLastError();
 }
 

@@ -14,10 +14,17 @@ struct GemmArgs {
   int64_t M, N, K, lda, ldb, ldc, ldmask;
   int relu;            // C = max(C, 0) after the bias
   int accumulate;      // C += result
+  // split-K: grid.z = splits, split z covers K range [z*kchunk, (z+1)*kchunk) and writes its
+  // raw tile to partial + z*M*N; a second kernel adds the splits in order and applies the
+  // epilogue (deterministic)
+  int64_t kchunk;
+  float* partial;
 };
 
 hipError_t launch_gemm_f32(bool ta, bool tb, const GemmArgs& g, hipStream_t s);
 bool gemm_shape_ok(const GemmArgs& g);
+// number of K splits worth using for this shape (1 = none)
+int gemm_splits(const GemmArgs& g, int num_cu);
 
 // Whitening between bead coordinates R (B, N, P) and the flow's permuted normal-mode
 // coordinates zp (B, D): zp[b][h*D/2 + j*P/2 + kk] = Z[b][j][2 kk + h], Z = (R - m) U / sigma.
@@ -54,6 +61,12 @@ struct CouplingArgs {
   const double* c;   // (B,) dL/d(-logdet) per sample
 };
 
+// column sums out[j] = sum_i X[i][j] in two deterministic passes; partial holds
+// colsum_chunks(M) * N doubles
+int64_t colsum_chunks(int64_t M);
+hipError_t launch_colsum_f32(int64_t M, int64_t N, const float* X, int64_t ldx, float* out,
+                             double* partial, hipStream_t s);
+
 hipError_t launch_coupling(int inverse, const CouplingArgs& a, hipStream_t s);
 hipError_t launch_coupling_inverse_backward(const CouplingArgs& a, hipStream_t s);
 
@@ -71,8 +84,6 @@ struct LogRhoArgs {
 };
 
 hipError_t launch_logrho(const LogRhoArgs& a, hipStream_t s);
-hipError_t launch_colsum_f32(int64_t M, int64_t N, const float* X, int64_t ldx, float* out,
-                             hipStream_t s);
 hipError_t launch_adam_f32(int64_t n, float* p, const float* g, float* m, float* v, float lr,
                            float b1, float b2, float eps, int step, hipStream_t s);
 

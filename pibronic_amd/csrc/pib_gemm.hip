@@ -13,6 +13,8 @@
 #include "pib_common.hpp"
 #include "pib_flow.hpp"
 
+#include <algorithm>
+
 namespace pib {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -27,13 +29,15 @@ __global__ void __launch_bounds__(256) k_gemm_f32(GemmArgs g) {
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int64_t m0 = (int64_t)blockIdx.y * kTM, n0 = (int64_t)blockIdx.x * kTN;
+  const int64_t kbeg = g.partial ? (int64_t)blockIdx.z * g.kchunk : 0;
+  const int64_t kend = g.partial ? kbeg + g.kchunk : g.K;
   f32x4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < g.K; k0 += kTK) {
+  for (int64_t k0 = kbeg; k0 < kend; k0 += kTK) {
     // stage A (kTM x kTK) and B (kTK x kTN): 1024 elements each, 4 per thread
 #pragma unroll
     for (int e = tid; e < kTM * kTK; e += 256) {
@@ -65,6 +69,21 @@ __global__ void __launch_bounds__(256) k_gemm_f32(GemmArgs g) {
     }
     __syncthreads();
   }
+  if (g.partial) {  // split-K: raw tile, epilogue in k_gemm_reduce
+    float* P = g.partial + (int64_t)blockIdx.z * g.M * g.N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int64_t col = n0 + wc * 32 + j * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+          P[row * g.N + col] = acc[i][j][r];
+        }
+      }
+    return;
+  }
   // epilogue
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -84,17 +103,48 @@ __global__ void __launch_bounds__(256) k_gemm_f32(GemmArgs g) {
     }
 }
 
+// sum of the K splits in split order, then the same epilogue as k_gemm_f32
+__global__ void k_gemm_reduce(GemmArgs g, int splits) {
+  const int64_t MN = g.M * g.N;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < MN;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = e / g.N, col = e - row * g.N;
+    float v = g.partial[e];
+    for (int z = 1; z < splits; ++z) v += g.partial[z * MN + e];
+    if (g.bias) v += g.bias[col];
+    if (g.relu) v = v > 0.f ? v : 0.f;
+    if (g.mask) v = g.mask[row * g.ldmask + col] > 0.f ? v : 0.f;
+    if (g.accumulate) v += g.C[row * g.ldc + col];
+    g.C[row * g.ldc + col] = v;
+  }
+}
+
+int gemm_splits(const GemmArgs& g, int num_cu) {
+  const int64_t tiles = (g.M / kTM) * (g.N / kTN);
+  int s = 1;
+  // split while the grid is under one work-group per CU and each split keeps >= 256 of K
+  while (tiles * s * 2 <= num_cu && g.K % (kTK * s * 2) == 0 && g.K / (s * 2) >= 256) s *= 2;
+  return s;
+}
+
 bool gemm_shape_ok(const GemmArgs& g) {
   return g.M > 0 && g.N > 0 && g.K > 0 && g.M % kTM == 0 && g.N % kTN == 0 && g.K % kTK == 0;
 }
 
 hipError_t launch_gemm_f32(bool ta, bool tb, const GemmArgs& g, hipStream_t s) {
   if (!gemm_shape_ok(g)) return hipErrorInvalidValue;
-  const dim3 grid((unsigned)(g.N / kTN), (unsigned)(g.M / kTM));
+  const int splits = g.partial ? (int)(g.K / g.kchunk) : 1;
+  if (g.partial && (g.kchunk <= 0 || g.kchunk % kTK || g.K % g.kchunk)) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)(g.N / kTN), (unsigned)(g.M / kTM), (unsigned)splits);
   if (!ta && !tb) hipLaunchKernelGGL((k_gemm_f32<false, false>), grid, dim3(256), 0, s, g);
   else if (!ta && tb) hipLaunchKernelGGL((k_gemm_f32<false, true>), grid, dim3(256), 0, s, g);
   else if (ta && !tb) hipLaunchKernelGGL((k_gemm_f32<true, false>), grid, dim3(256), 0, s, g);
   else hipLaunchKernelGGL((k_gemm_f32<true, true>), grid, dim3(256), 0, s, g);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !g.partial) return e;
+  const int64_t MN = g.M * g.N;
+  const unsigned blocks = (unsigned)std::min<int64_t>((MN + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_gemm_reduce, dim3(blocks), dim3(256), 0, s, g, splits);
   return hipGetLastError();
 }
 

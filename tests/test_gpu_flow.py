@@ -51,6 +51,44 @@ def test_gemm_f32(tmp_path, ta, tb):
         ctx.gemm_f32(Ad, Bd, C, 100, N, K, lda, ldb, N, trans_a=ta, trans_b=tb)
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_gemm_f32_split_k(tmp_path, ta, tb):
+    """Long-K products (the weight gradients, K = batch) run split over K with an ordered
+    second pass; same epilogue options."""
+    _announce(f"gemm split-K ta={ta} tb={tb}")
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, 2, quadratic=False, seed=0))
+    ctx = make_data(FS, 300.0, 64, 10, 10).context()
+    M, N, K = 128, 128, 2048
+    g = torch.Generator().manual_seed(7 + ta * 2 + tb)
+    A = torch.randn((K, M) if ta else (M, K), generator=g, dtype=torch.float32)
+    B = torch.randn((N, K) if tb else (K, N), generator=g, dtype=torch.float32)
+    bias = torch.randn(N, generator=g)
+    mask = (torch.rand((M, N), generator=g) > 0.5).float()
+    C0 = torch.randn((M, N), generator=g)
+    ref = (A.double().T if ta else A.double()) @ (B.double().T if tb else B.double())
+    Ad, Bd = A.to(DEV), B.to(DEV)
+    lda, ldb = A.shape[1], B.shape[1]
+    C = C0.to(DEV).clone()
+    ctx.gemm_f32(Ad, Bd, C, M, N, K, lda, ldb, N, trans_a=ta, trans_b=tb, bias=bias.to(DEV),
+                 relu=True, mask=mask.to(DEV), ldmask=N, accumulate=True)
+    torch.cuda.synchronize()
+    want = torch.relu(ref + bias.double()) * mask.double() + C0.double()
+    assert torch.allclose(C.cpu().double(), want, rtol=1e-5, atol=1e-3)
+
+
+def test_colsum_f32(tmp_path):
+    """Two-pass column sums, ragged shapes, strided rows."""
+    _announce("colsum")
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, 2, quadratic=False, seed=0))
+    ctx = make_data(FS, 300.0, 64, 10, 10).context()
+    M, N, ld = 2085, 200, 211
+    X = torch.randn((M, ld), generator=torch.Generator().manual_seed(3), dtype=torch.float32)
+    out = torch.empty(N, device=DEV)
+    ctx.colsum_f32(M, N, X.to(DEV), ld, out)
+    torch.cuda.synchronize()
+    assert torch.allclose(out.cpu().double(), X[:, :N].double().sum(dim=0), rtol=1e-6, atol=1e-4)
+
+
 @pytest.fixture(scope="module")
 def model(tmp_path_factory):
     FS = stage_dict_models(tmp_path_factory.mktemp("flow"), synthetic.generate(3, 2, quadratic=True, seed=1))
