@@ -167,6 +167,93 @@ def learned_leg(data, args, log):
     return res
 
 
+def batched_leg(FS, cfg, args, cpu, log):
+    """Sixteen temperatures of one model (T_i = 100 * 5^(i/15) K, P_i = ceil(beta_i / tau0))
+    in one persistent launch per block, the block step replayed as a hipGraph."""
+    import numpy as np
+    import torch
+    from pibronic_amd import _native, pimc, synthetic
+    from pibronic_amd.batch import TemperatureBatch
+    Ts, Ps, tau0 = synthetic.c3_temperatures()
+    B, K, W = args.block, args.steps, args.warmup
+    datas = []
+    for T, P in zip(Ts, Ps):
+        d = pimc.BoxData.from_FileStructure(FS)
+        d.samples, d.beads, d.temperature = B * (K + W), int(P), float(T)
+        d.block_size, d.blocks = B, K + W
+        d.preprocess()
+        datas.append(d)
+    tb = TemperatureBatch(datas, B, max_blocks=K + W)
+    tune = tb.tune(seed=args.seed)
+    log(f"[bench] batched tune: {tune}")
+    tb.run(args.seed, 0, W)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tb.run(args.seed, W, K)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    sums = tb.block_sums()[W:W + K]
+    # evaluation launch alone (HIP events on the launch stream), for the roofline
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(3):
+        tb.batch.eval(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    eval_ms = e0.elapsed_time(e1) / 3
+    nq = int(np.count_nonzero(np.abs(datas[0].vib.quadratic).sum(axis=(2, 3))))
+    flops = sum(algorithmic_flops(cfg["A"], cfg["N"], int(P), nq, False)["eval"] for P in Ps) * B
+    achieved = flops / (eval_ms * 1e-3) / 1e12
+    per_t = []
+    ess_tot = 0.0
+    for i in range(len(Ts)):
+        s = sums[:, i].sum(axis=0)
+        w = weight_summary(s, el)
+        ess_tot += w["ess_per_s"] * el
+        per_t.append({"T": float(Ts[i]), "P": int(Ps[i]), "ess_fraction": w["ess_fraction"],
+                      "z_ratio": w["z_ratio"], "z_ratio_err": w["z_ratio_err"]})
+    beads = int(np.sum(Ps))
+    line = {
+        "metric": "effective samples/sec at fixed Z-ratio std error; learned ρ vs harmonic ρ",
+        "value": ess_tot / el, "unit": "effective samples/s (harmonic rho, all temperatures)",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": el / K * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded model, device Philox samples)",
+        "config": {"workload": f"A6_N12_T100-500: A={cfg['A']} N={cfg['N']}, 16 temperatures "
+                               f"100-500 K, P_i = ceil(beta_i/tau0), tau0 = beta(100 K)/256 "
+                               f"(P {int(Ps.min())}-{int(Ps.max())}), block={B} per temperature",
+                   "samples_per_step": B * len(Ts), "schedule": tune["schedule"],
+                   "value_from": "harmonic rho; block step (16 samplers, evaluation, 16 reductions) "
+                                 "replayed as one hipGraph, evaluation schedule chosen by the autotuner"},
+        "samples_beads_per_s": B * beads * K / el,
+        "samples_per_s": B * len(Ts) * K / el,
+        "batched_eval_ms": eval_ms,
+        "tuned_batched_ms": tune["batched_ms"],
+        "sequential_eval_ms": tune["sequential_ms"],
+        "batched_speedup_vs_sequential": tune["sequential_ms"] / tune["batched_ms"],
+        "temperatures": per_t,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": {"lanes": "k_eval_batched (persistent)", "wide": "k_eval_wide_batched (persistent)",
+                                "sequential": "k_eval_fp64, one launch per temperature in the graph step"
+                                }[tune["schedule"]["family"]],
+                     "flops_per_launch": flops, "traffic_detail": "not profiled for this workload"},
+        "cpu_baseline": None,
+    }
+    if cpu is not None:
+        line["cpu_baseline"] = {
+            "value": cpu["samples_beads_per_s"], "unit": "samples*beads/s", "cores": cpu["cores"],
+            "kind": "port",
+            "sample": f"oracle at the middle temperature only (T={cpu['T']:.1f} K, P={cpu['P']}), "
+                      f"{cpu['samples']} samples in blocks of 500 over {cpu['seconds']:.1f} s, "
+                      f"one process per core, numpy {cpu['numpy']}",
+            "one_core_value": cpu["one_core"],
+            "gpu_over_cpu": (B * beads * K / el) / cpu["samples_beads_per_s"],
+        }
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,12 +298,20 @@ def main():
     tmp = tempfile.mkdtemp(prefix="pib_bench_")
     FS = synthetic.write_workload(tmp, name)
 
+    batched = P is None
     cpu = None
     if not args.no_cpu:
         cores = args.cpu_cores or min(16, os.cpu_count() or 1)
+        if batched:
+            Ts, Ps, _ = synthetic.c3_temperatures()
+            T, P = float(Ts[8]), int(Ps[8])
         cpu = cpu_baseline(FS.path_vib_model, FS.path_rho_model, T, P, args.cpu_seconds, cores)
-        cpu["cores"] = cores
+        cpu.update(cores=cores, T=T, P=P)
         log(f"[bench] cpu baseline {cpu['samples_beads_per_s']:.3e} samples*beads/s on {cores} cores")
+
+    if batched:
+        print(json.dumps(batched_leg(FS, cfg, args, cpu, log)))
+        return
 
     import torch
     data = (pimc.BoxDataPM if pm else pimc.BoxData).from_FileStructure(FS)

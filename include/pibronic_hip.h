@@ -214,6 +214,65 @@ int pib_adam_f32(pib_ctx* ctx, int64_t n, float* p_dev, const float* g_dev, floa
                  float* v_dev, float lr, float beta1, float beta2, float eps, int step,
                  void* stream);
 
+
+/* ------------------------------------------------------------------------------------------
+ * Batched evaluation. No counterpart in the reference, which runs one (model, temperature)
+ * per process and farms them out (pibronic/server/job_boss.py:538-611, PimcSubmissionClass).
+ * A batch evaluates one block of B samples for each of n problems (contexts: model, O-matrix
+ * sets, beads, sampler) in ONE persistent launch; problems may differ in P (the temperatures
+ * of a run) and, for the wide family, in A, A_rho and N. The contexts must outlive the batch
+ * and keep their constants while it exists. Not re-entrant (one stream at a time).
+ * ---------------------------------------------------------------------------------------- */
+typedef struct pib_batch pib_batch;
+
+#define PIB_BATCH_LANES 0 /* lanes-per-sample kernel: one compiled (A, A_rho) for all   */
+#define PIB_BATCH_WIDE 1  /* one wave per sample, 16x16 padded: any A, A_rho <= 16      */
+#define PIB_BATCH_SEQUENTIAL 2 /* one launch per problem (tuned lanes), same graph step   */
+
+/* flags: PIB_EVAL_PM and/or PIB_EVAL_NOSCALE; delta_beta is used by the +-dbeta sums;
+ * max_blocks bounds the per-block sums kept by pib_batch_run. Buffers are the caller's
+ * (pib_batch_bind, pib_batch_bind_sums); the batch owns only its schedule and scratch. */
+int pib_batch_create(int n, pib_ctx* const* ctxs, int64_t B, int flags, double delta_beta,
+                     int max_blocks, pib_batch** out);
+int pib_batch_destroy(pib_batch* batch);
+const char* pib_batch_last_error(const pib_batch* batch);
+
+/* Problem i reads q_dev (B*N*P doubles, as pib_eval_block) and writes PIB_NUM_OUT rows of
+ * out_stride (>= B) doubles at out_dev. Rebinding drops a captured graph. */
+int pib_batch_bind(pib_batch* batch, int i, const double* q_dev, double* out_dev,
+                   int64_t out_stride);
+/* Block sums of pib_batch_run: block k of problem i goes to
+ * sums_dev + (k * n + i) * PIB_NUM_SUMS (max_blocks * n * PIB_NUM_SUMS doubles). */
+int pib_batch_bind_sums(pib_batch* batch, double* sums_dev);
+
+/* Schedule: kernel family, lanes per sample of every problem (lanes family; NULL keeps the
+ * current ones), tile size (waves of the lanes family, samples of the wide family) and the
+ * number of persistent waves. */
+int pib_batch_set_schedule(pib_batch* batch, int family, const int* lanes, int tile_waves,
+                           int grid_waves);
+int pib_batch_get_schedule(pib_batch* batch, int* family, int* lanes, int* tile_waves,
+                           int* grid_waves);
+
+/* Autotuner: samples every problem once (seed) into its bound q, then times schedules on the
+ * device (coordinate search over family -- persistent lanes, persistent wide, or one launch
+ * per problem --, beads per lane, tile size and grid size) and keeps the fastest.
+ * best_ms = its time per evaluation step; sequential_ms (optional) = one launch per problem
+ * with the default lanes (as pib_eval_block), for comparison. */
+int pib_batch_tune(pib_batch* batch, uint64_t seed, double* best_ms, double* sequential_ms,
+                   void* stream);
+
+/* Evaluate the q currently in the batch buffers: one persistent launch. */
+int pib_batch_eval(pib_batch* batch, void* stream);
+/* The same problems as one launch each (reference point for the batched launch). */
+int pib_batch_eval_sequential(pib_batch* batch, void* stream);
+
+/* nblocks block steps, blocks first_block .. first_block+nblocks-1: for each, sample every
+ * problem (Philox offsets block*B under `seed`, the counters of pib_sample_block), evaluate
+ * all problems in one persistent launch, and reduce each problem's block into its sums row.
+ * The step is captured once as a hipGraph (kernels only) and replayed. */
+int pib_batch_run(pib_batch* batch, uint64_t seed, int64_t first_block, int nblocks,
+                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif

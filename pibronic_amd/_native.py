@@ -21,6 +21,7 @@ PIB_ERR_UNSUPPORTED = -4
 
 SET_RHO, SET_VIB, SET_VIB_PLUS, SET_VIB_MINUS = 0, 1, 2, 3
 EVAL_PM, EVAL_NOSCALE, EVAL_WIDE = 1, 2, 4
+BATCH_LANES, BATCH_WIDE, BATCH_SEQUENTIAL = 0, 1, 2
 OUT_RHO, OUT_G, OUT_GP, OUT_GM, OUT_LOGS = 0, 1, 2, 3, 4
 NUM_OUT = 5
 (SUM_N, SUM_R, SUM_R2, SUM_ABSR, SUM_NEG, SUM_D1, SUM_D1SQ, SUM_D2, SUM_D2SQ,
@@ -57,6 +58,19 @@ SIGNATURES = {
     "pib_colsum_f32": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _vp, _vp]),
     "pib_adam_f32": (_c_int, [_vp, _c_i64, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float,
                               ctypes.c_float, ctypes.c_float, _c_int, _vp]),
+    "pib_batch_create": (_c_int, [_c_int, ctypes.POINTER(_vp), _c_i64, _c_int, _c_dbl, _c_int,
+                                  ctypes.POINTER(_vp)]),
+    "pib_batch_destroy": (_c_int, [_vp]),
+    "pib_batch_last_error": (ctypes.c_char_p, [_vp]),
+    "pib_batch_bind": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64]),
+    "pib_batch_bind_sums": (_c_int, [_vp, _vp]),
+    "pib_batch_set_schedule": (_c_int, [_vp, _c_int, ctypes.POINTER(_c_int), _c_int, _c_int]),
+    "pib_batch_get_schedule": (_c_int, [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
+                                        ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
+    "pib_batch_tune": (_c_int, [_vp, _c_u64, _dp, _dp, _vp]),
+    "pib_batch_eval": (_c_int, [_vp, _vp]),
+    "pib_batch_eval_sequential": (_c_int, [_vp, _vp]),
+    "pib_batch_run": (_c_int, [_vp, _c_u64, _c_i64, _c_int, _vp]),
 }
 
 FLOW_WHITEN, FLOW_UNWHITEN, FLOW_GRAD = 0, 1, 2
@@ -236,3 +250,78 @@ class Context:
         self._check(self.lib.pib_adam_f32(self.h, int(n), _devptr(p), _devptr(g), _devptr(m), _devptr(v),
                                           float(lr), float(beta1), float(beta2), float(eps), int(step),
                                           _stream(stream)))
+
+
+class Batch:
+    """A pib_batch over Contexts (one block of B samples per problem, one persistent launch).
+
+    The Contexts must stay alive, with unchanged constants, while the batch exists (the batch
+    keeps references to them)."""
+
+    def __init__(self, contexts, B, pm=False, noscale=False, delta_beta=1.0, max_blocks=1):
+        self.lib = load()
+        self.contexts = list(contexts)
+        self.n, self.B, self.pm, self.max_blocks = len(self.contexts), int(B), bool(pm), int(max_blocks)
+        arr = (_vp * self.n)(*[c.h.value for c in self.contexts])
+        h = ctypes.c_void_p()
+        flags = (EVAL_PM if pm else 0) | (EVAL_NOSCALE if noscale else 0)
+        rc = self.lib.pib_batch_create(self.n, arr, self.B, flags, float(delta_beta), self.max_blocks,
+                                       ctypes.byref(h))
+        if rc != PIB_OK:
+            msg = self.lib.pib_batch_last_error(h).decode() if h.value else "bad arguments"
+            if h.value:
+                self.lib.pib_batch_destroy(h)
+            raise NativeError(rc, msg)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pib_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != PIB_OK:
+            raise NativeError(rc, self.lib.pib_batch_last_error(self.h).decode())
+
+    def bind(self, i, q, out, out_stride):
+        """Problem i reads q (B, N, P) and writes out (NUM_OUT rows of out_stride)."""
+        self._check(self.lib.pib_batch_bind(self.h, int(i), _devptr(q), _devptr(out), int(out_stride)))
+
+    def bind_sums(self, sums):
+        """sums: (max_blocks, n, NUM_SUMS) float64 device tensor."""
+        self._check(self.lib.pib_batch_bind_sums(self.h, _devptr(sums)))
+
+    def set_schedule(self, family, lanes=None, tile_waves=4, grid_waves=2048):
+        arr = None if lanes is None else (_c_int * self.n)(*[int(g) for g in lanes])
+        self._check(self.lib.pib_batch_set_schedule(self.h, int(family), arr, int(tile_waves), int(grid_waves)))
+
+    def schedule(self):
+        fam, tile, grid = _c_int(), _c_int(), _c_int()
+        lanes = (_c_int * self.n)()
+        self._check(self.lib.pib_batch_get_schedule(self.h, ctypes.byref(fam), lanes, ctypes.byref(tile),
+                                                    ctypes.byref(grid)))
+        return {"family": {BATCH_LANES: "lanes", BATCH_WIDE: "wide", BATCH_SEQUENTIAL: "sequential"}[fam.value],
+                "lanes": list(lanes),
+                "tile": tile.value, "grid_waves": grid.value}
+
+    def tune(self, seed=1, stream=None):
+        best, seq = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.pib_batch_tune(self.h, ctypes.c_uint64(seed), ctypes.byref(best),
+                                            ctypes.byref(seq), _stream(stream)))
+        return best.value, seq.value
+
+    def eval(self, stream=None):
+        self._check(self.lib.pib_batch_eval(self.h, _stream(stream)))
+
+    def eval_sequential(self, stream=None):
+        self._check(self.lib.pib_batch_eval_sequential(self.h, _stream(stream)))
+
+    def run(self, seed, first_block, nblocks, stream=None):
+        self._check(self.lib.pib_batch_run(self.h, ctypes.c_uint64(seed), int(first_block), int(nblocks),
+                                           _stream(stream)))

@@ -9,132 +9,65 @@
 
 #include "../../include/pibronic_hip.h"
 #include "pib_common.hpp"
-#include "pib_kernels.hpp"
+#include "pib_ctx.hpp"
 #include "pib_flow.hpp"
+#include "pib_kernels.hpp"
 
 using namespace pib;
 
-namespace {
+namespace pib {
 
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-}  // namespace
-
-struct pib_ctx {
-  int device = 0;
-  int num_cu = 256;
-  std::string err;
-  // coupling
-  int A = 0, N = 0;
-  DevBuf energy, linear, qdiag, pair, qpair;
-  DevBuf w_lq, w_e, w_qpair;  // padded 16 x 16 constants of the wide kernel (A <= 16)
-  int npair = 0;
-  bool has_qdiag = false;
-  bool coupling_set = false;
-  // O sets
-  int set_A[NUM_SETS] = {0, 0, 0, 0};
-  int set_N[NUM_SETS] = {0, 0, 0, 0};
-  DevBuf coth[NUM_SETS], csch[NUM_SETS], pref[NUM_SETS], shift[NUM_SETS];
-  // beads
-  int P = 0;
-  double tau = 0.0;
-  int lanes = 0;
-  // sampler
-  int s_A = 0, s_N = 0, s_P = 0;
-  DevBuf s_cdf, s_chol, s_shift;
-  bool sampler_set = false;
-  // reduction scratch
-  DevBuf partials;
-  int nparts = 0;
-  // split-K GEMM and column-sum scratch (training calls)
-  DevBuf work;
-  // flow basis
-  int f_N = 0, f_P = 0;
-  DevBuf f_U, f_sigma, f_isigma, f_m;
-};
-
-namespace {
-
-int fail(pib_ctx* c, int code, const std::string& msg) {
-  if (c) c->err = msg;
-  return code;
-}
-
-int hip_fail(pib_ctx* c, hipError_t e, const char* what) {
-  return fail(c, PIB_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define PIB_HIP(ctx, call)                                   \
-  do {                                                       \
-    hipError_t e_ = (call);                                  \
-    if (e_ != hipSuccess) return hip_fail((ctx), e_, #call); \
-  } while (0)
-
-// (Re)allocate a device buffer and copy `bytes` from host. Synchronises the device first so
-// that no queued kernel can still read the previous contents.
-int upload(pib_ctx* c, DevBuf& b, const void* host, size_t bytes) {
-  PIB_HIP(c, hipSetDevice(c->device));
-  if (b.p && b.bytes < bytes) {
-    PIB_HIP(c, hipDeviceSynchronize());
-    PIB_HIP(c, hipFree(b.p));
-    b.p = nullptr;
-    b.bytes = 0;
+int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* wide_out) {
+  const bool pm = (flags & PIB_EVAL_PM) != 0;
+  if (flags & ~(PIB_EVAL_PM | PIB_EVAL_NOSCALE | PIB_EVAL_WIDE))
+    return fail(c, PIB_ERR_ARG, "eval_block: unknown flag");
+  if (!c->coupling_set || c->P < 3) return fail(c, PIB_ERR_STATE, "eval_block: coupling or beads not set");
+  const int need = pm ? 4 : 2;
+  for (int s = 0; s < need; ++s)
+    if (!c->set_A[s]) return fail(c, PIB_ERR_STATE, "eval_block: O-matrix set " + std::to_string(s) + " not set");
+  for (int s = 1; s < need; ++s)
+    if (c->set_A[s] != c->A || c->set_N[s] != c->N)
+      return fail(c, PIB_ERR_ARG, "eval_block: coupled-model O-matrix set has the wrong surface/mode count");
+  if (c->set_N[0] != c->N) return fail(c, PIB_ERR_ARG, "eval_block: sampling model has a different number of modes");
+  const int AR = c->set_A[SET_RHO];
+  const bool fits_wide = c->A <= kWideMaxStates && AR <= kWideMaxStates;
+  const bool wide = fits_wide && ((flags & PIB_EVAL_WIDE) || !eval_supported(c->A, AR));
+  if (!wide && !eval_supported(c->A, AR))
+    return fail(c, PIB_ERR_UNSUPPORTED, "eval_block: no FP64 kernel for A=" + std::to_string(c->A) +
+                                             ", A_rho=" + std::to_string(AR));
+  if (B < 0) return fail(c, PIB_ERR_ARG, "eval_block: B < 0 or out_stride < B");
+  EvalArgs a{};
+  a.B = B;
+  a.N = c->N;
+  a.P = c->P;
+  a.G = c->lanes ? c->lanes : auto_lanes(c, B);
+  a.K = (c->P + a.G - 1) / a.G;
+  a.noscale = (flags & PIB_EVAL_NOSCALE) ? 1 : 0;
+  a.tau = c->tau;
+  for (int s = 0; s < NUM_SETS; ++s) {
+    a.set[s].coth = (const double*)c->coth[s].p;
+    a.set[s].csch = (const double*)c->csch[s].p;
+    a.set[s].pref = (const double*)c->pref[s].p;
+    a.set[s].shift = (const double*)c->shift[s].p;
+    if (s >= need) a.set[s] = a.set[SET_VIB];
   }
-  if (!b.p && bytes) {
-    PIB_HIP(c, hipMalloc(&b.p, bytes));
-    b.bytes = bytes;
-  }
-  if (bytes) {
-    PIB_HIP(c, hipDeviceSynchronize());
-    PIB_HIP(c, hipMemcpy(b.p, host, bytes, hipMemcpyHostToDevice));
-  }
+  a.energy = (const double*)c->energy.p;
+  a.linear = (const double*)c->linear.p;
+  a.qdiag = c->has_qdiag ? (const double*)c->qdiag.p : nullptr;
+  a.npair = c->npair;
+  a.pair = (const int*)c->pair.p;
+  a.qpair = (const double*)c->qpair.p;
+  a.A = c->A;
+  a.AR = AR;
+  a.wide_lq = (const double2*)c->w_lq.p;
+  a.wide_e = (const double*)c->w_e.p;
+  a.wide_qpair = (const double*)c->w_qpair.p;
+  *out_args = a;
+  *wide_out = wide;
   return PIB_OK;
 }
 
-// Make b hold at least `bytes` of device memory (contents undefined). Synchronises the
-// device before replacing a buffer that queued kernels may still use.
-int reserve(pib_ctx* c, DevBuf& b, size_t bytes) {
-  if (b.p && b.bytes >= bytes) return PIB_OK;
-  PIB_HIP(c, hipSetDevice(c->device));
-  if (b.p) {
-    PIB_HIP(c, hipDeviceSynchronize());
-    PIB_HIP(c, hipFree(b.p));
-    b.p = nullptr;
-    b.bytes = 0;
-  }
-  PIB_HIP(c, hipMalloc(&b.p, bytes));
-  b.bytes = bytes;
-  return PIB_OK;
-}
-
-void release(DevBuf& b) {
-  if (b.p) (void)hipFree(b.p);
-  b.p = nullptr;
-  b.bytes = 0;
-}
-
-bool finite_all(const double* v, size_t n) {
-  for (size_t i = 0; i < n; ++i)
-    if (!std::isfinite(v[i])) return false;
-  return true;
-}
-
-int auto_lanes(const pib_ctx* c, int64_t B) {
-  // enough lanes in flight for ~24 waves per SIMD over the launch (measured on MI355X at
-  // A=4, N=8, P=64: 16 lanes per sample for blocks of 1e5 samples is the fastest split);
-  // never more lanes per sample than beads, so that each lane owns at least one bead
-  const int64_t target = (int64_t)c->num_cu * 4 * kWave * 24;
-  int maxg = 1;
-  while (maxg < 64 && maxg * 2 <= c->P) maxg *= 2;
-  int g = 1;
-  while (g < maxg && B * g < target) g *= 2;
-  return g;
-}
-
-}  // namespace
+}  // namespace pib
 
 extern "C" {
 
@@ -308,59 +241,21 @@ int pib_get_lanes_per_sample(pib_ctx* c, int64_t B) {
 int pib_eval_block(pib_ctx* c, const double* q, int64_t B, int flags, double* out,
                    int64_t out_stride, double* dbg, void* stream) {
   if (!c) return PIB_ERR_ARG;
-  const bool pm = (flags & PIB_EVAL_PM) != 0;
-  if (flags & ~(PIB_EVAL_PM | PIB_EVAL_NOSCALE | PIB_EVAL_WIDE))
-    return fail(c, PIB_ERR_ARG, "eval_block: unknown flag");
-  if (!c->coupling_set || c->P < 3) return fail(c, PIB_ERR_STATE, "eval_block: coupling or beads not set");
-  const int need = pm ? 4 : 2;
-  for (int s = 0; s < need; ++s)
-    if (!c->set_A[s]) return fail(c, PIB_ERR_STATE, "eval_block: O-matrix set " + std::to_string(s) + " not set");
-  for (int s = 1; s < need; ++s)
-    if (c->set_A[s] != c->A || c->set_N[s] != c->N)
-      return fail(c, PIB_ERR_ARG, "eval_block: coupled-model O-matrix set has the wrong surface/mode count");
-  if (c->set_N[0] != c->N) return fail(c, PIB_ERR_ARG, "eval_block: sampling model has a different number of modes");
-  const int AR = c->set_A[SET_RHO];
-  const bool fits_wide = c->A <= kWideMaxStates && AR <= kWideMaxStates;
-  const bool wide = fits_wide && ((flags & PIB_EVAL_WIDE) || !eval_supported(c->A, AR));
-  if (!wide && !eval_supported(c->A, AR))
-    return fail(c, PIB_ERR_UNSUPPORTED, "eval_block: no FP64 kernel for A=" + std::to_string(c->A) +
-                                             ", A_rho=" + std::to_string(AR));
-  if (B < 0 || out_stride < B) return fail(c, PIB_ERR_ARG, "eval_block: B < 0 or out_stride < B");
+  EvalArgs a{};
+  bool wide = false;
+  int r = make_eval_args(c, flags, B, &a, &wide);
+  if (r) return r;
+  if (out_stride < B) return fail(c, PIB_ERR_ARG, "eval_block: B < 0 or out_stride < B");
   if (B == 0) return PIB_OK;
   if (!q || !out) return fail(c, PIB_ERR_ARG, "eval_block: null q or out");
-  EvalArgs a{};
   a.q = q;
-  a.B = B;
-  a.N = c->N;
-  a.P = c->P;
-  a.G = c->lanes ? c->lanes : auto_lanes(c, B);
-  a.K = (c->P + a.G - 1) / a.G;
-  a.noscale = (flags & PIB_EVAL_NOSCALE) ? 1 : 0;
-  a.tau = c->tau;
-  for (int s = 0; s < NUM_SETS; ++s) {
-    a.set[s].coth = (const double*)c->coth[s].p;
-    a.set[s].csch = (const double*)c->csch[s].p;
-    a.set[s].pref = (const double*)c->pref[s].p;
-    a.set[s].shift = (const double*)c->shift[s].p;
-    if (s >= need) a.set[s] = a.set[SET_VIB];
-  }
-  a.energy = (const double*)c->energy.p;
-  a.linear = (const double*)c->linear.p;
-  a.qdiag = c->has_qdiag ? (const double*)c->qdiag.p : nullptr;
-  a.npair = c->npair;
-  a.pair = (const int*)c->pair.p;
-  a.qpair = (const double*)c->qpair.p;
   a.out = out;
   a.out_stride = out_stride;
   a.dbg = dbg;
-  a.A = c->A;
-  a.AR = AR;
-  a.wide_lq = (const double2*)c->w_lq.p;
-  a.wide_e = (const double*)c->w_e.p;
-  a.wide_qpair = (const double*)c->w_qpair.p;
   PIB_HIP(c, hipSetDevice(c->device));
-  hipError_t e = wide ? launch_eval_wide(pm ? 1 : 0, a, (hipStream_t)stream)
-                      : launch_eval_fp64(c->A, AR, pm ? 1 : 0, a, (hipStream_t)stream);
+  const int pm = (flags & PIB_EVAL_PM) ? 1 : 0;
+  hipError_t e = wide ? launch_eval_wide(pm, a, (hipStream_t)stream)
+                      : launch_eval_fp64(c->A, a.AR, pm, a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "eval_block launch");
   return PIB_OK;
 }

@@ -1,0 +1,138 @@
+// Internal: the context structure shared by the C-ABI translation units, and host helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/pibronic_hip.h"
+#include "pib_common.hpp"
+#include "pib_kernels.hpp"
+
+namespace pib {
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+}  // namespace pib
+
+struct pib_ctx {
+  int device = 0;
+  int num_cu = 256;
+  std::string err;
+  // coupling
+  int A = 0, N = 0;
+  pib::DevBuf energy, linear, qdiag, pair, qpair;
+  pib::DevBuf w_lq, w_e, w_qpair;  // padded 16 x 16 constants of the wide kernel (A <= 16)
+  int npair = 0;
+  bool has_qdiag = false;
+  bool coupling_set = false;
+  // O sets
+  int set_A[pib::NUM_SETS] = {0, 0, 0, 0};
+  int set_N[pib::NUM_SETS] = {0, 0, 0, 0};
+  pib::DevBuf coth[pib::NUM_SETS], csch[pib::NUM_SETS], pref[pib::NUM_SETS], shift[pib::NUM_SETS];
+  // beads
+  int P = 0;
+  double tau = 0.0;
+  int lanes = 0;
+  // sampler
+  int s_A = 0, s_N = 0, s_P = 0;
+  pib::DevBuf s_cdf, s_chol, s_shift;
+  bool sampler_set = false;
+  // reduction scratch
+  pib::DevBuf partials;
+  int nparts = 0;
+  // split-K GEMM and column-sum scratch (training calls)
+  pib::DevBuf work;
+  // flow basis
+  int f_N = 0, f_P = 0;
+  pib::DevBuf f_U, f_sigma, f_isigma, f_m;
+};
+
+namespace pib {
+
+inline int fail(pib_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+inline int hip_fail(pib_ctx* c, hipError_t e, const char* what) {
+  return fail(c, PIB_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define PIB_HIP(ctx, call)                                   \
+  do {                                                       \
+    hipError_t e_ = (call);                                  \
+    if (e_ != hipSuccess) return hip_fail((ctx), e_, #call); \
+  } while (0)
+
+// (Re)allocate a device buffer and copy `bytes` from host. Synchronises the device first so
+// that no queued kernel can still read the previous contents.
+inline int upload(pib_ctx* c, DevBuf& b, const void* host, size_t bytes) {
+  PIB_HIP(c, hipSetDevice(c->device));
+  if (b.p && b.bytes < bytes) {
+    PIB_HIP(c, hipDeviceSynchronize());
+    PIB_HIP(c, hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (!b.p && bytes) {
+    PIB_HIP(c, hipMalloc(&b.p, bytes));
+    b.bytes = bytes;
+  }
+  if (bytes) {
+    PIB_HIP(c, hipDeviceSynchronize());
+    PIB_HIP(c, hipMemcpy(b.p, host, bytes, hipMemcpyHostToDevice));
+  }
+  return PIB_OK;
+}
+
+// Make b hold at least `bytes` of device memory (contents undefined). Synchronises the
+// device before replacing a buffer that queued kernels may still use.
+inline int reserve(pib_ctx* c, DevBuf& b, size_t bytes) {
+  if (b.p && b.bytes >= bytes) return PIB_OK;
+  PIB_HIP(c, hipSetDevice(c->device));
+  if (b.p) {
+    PIB_HIP(c, hipDeviceSynchronize());
+    PIB_HIP(c, hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  PIB_HIP(c, hipMalloc(&b.p, bytes));
+  b.bytes = bytes;
+  return PIB_OK;
+}
+
+inline void release(DevBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+inline bool finite_all(const double* v, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return false;
+  return true;
+}
+
+inline int auto_lanes(const pib_ctx* c, int64_t B) {
+  // enough lanes in flight for ~24 waves per SIMD over the launch (measured on MI355X at
+  // A=4, N=8, P=64: 16 lanes per sample for blocks of 1e5 samples is the fastest split);
+  // never more lanes per sample than beads, so that each lane owns at least one bead
+  const int64_t target = (int64_t)c->num_cu * 4 * kWave * 24;
+  int maxg = 1;
+  while (maxg < 64 && maxg * 2 <= c->P) maxg *= 2;
+  int g = 1;
+  while (g < maxg && B * g < target) g *= 2;
+  return g;
+}
+
+}  // namespace pib
+
+namespace pib {
+// Validate the context for pib_eval_block-style evaluation of B samples with `flags` and fill
+// every EvalArgs field except q, out, out_stride and dbg. Returns PIB_OK or an error code (with
+// the message in c->err); *wide tells which kernel family applies.
+int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* a, bool* wide);
+}  // namespace pib

@@ -145,12 +145,11 @@ struct EvalOccupancy {
   static constexpr int waves = (A <= 4) ? 3 : 1;
 };
 
+// The samples of one wave: wave w evaluates samples [w * 64/G, (w + 1) * 64/G) of `args`.
 template <int A, int AR, bool PM, bool DBG>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A>::waves)))
-k_eval_fp64(const EvalArgs args) {
+__device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
   constexpr bool SEP = (AR != A);  // qTempTensor path (pibronic/pimc/pimc.py:1094-1096)
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int G = args.G, K = args.K, N = args.N, P = args.P;
   const int gl = lane & (G - 1);
   const int spw = kWave / G;
@@ -460,6 +459,33 @@ k_eval_fp64(const EvalArgs args) {
   }
 }
 
+template <int A, int AR, bool PM, bool DBG>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A>::waves)))
+k_eval_fp64(const EvalArgs args) {
+  eval_wave<A, AR, PM, DBG>(args, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+}
+
+// Persistent batched launch: many blocks of samples (temperatures with different P, models
+// with the same surface counts) in one grid. Each wave repeatedly takes the next tile
+// (problem, first wave, wave count) from a device counter until the tile list is exhausted;
+// the host orders tiles by decreasing cost, so the long ones start first. Every wave reaches
+// the exit once the counter passes the tile count.
+template <int A, int AR, bool PM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A>::waves)))
+k_eval_batched(const EvalArgs* __restrict__ probs, const BatchTile* __restrict__ tiles, int ntiles,
+               int* counter) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(counter, 1);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, kWave));
+    if (t >= ntiles) break;
+    const BatchTile tile = tiles[t];
+    const EvalArgs& a = probs[tile.prob];
+    for (int64_t w = tile.wave0; w < tile.wave0 + tile.nwaves; ++w) eval_wave<A, AR, PM, false>(a, w);
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 template <int A, int AR>
 static hipError_t launch_a(int pm, const EvalArgs& a, hipStream_t s) {
@@ -485,6 +511,17 @@ static hipError_t launch_a(int pm, const EvalArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int A, int AR>
+static hipError_t launch_batched_a(int pm, const EvalArgs* probs, const BatchTile* tiles, int ntiles,
+                                   int* counter, int waves, hipStream_t s) {
+  const dim3 grid((unsigned)((waves + 3) / 4));
+  if (pm)
+    hipLaunchKernelGGL((k_eval_batched<A, AR, true>), grid, dim3(256), 0, s, probs, tiles, ntiles, counter);
+  else
+    hipLaunchKernelGGL((k_eval_batched<A, AR, false>), grid, dim3(256), 0, s, probs, tiles, ntiles, counter);
+  return hipGetLastError();
+}
+
 #define PIB_EVAL_CASES(X) \
   X(1, 1) X(2, 2) X(3, 3) X(4, 4) X(5, 5) X(6, 6) X(7, 7) X(8, 8) \
   X(2, 1) X(2, 4) X(2, 8) X(4, 2) X(4, 8)
@@ -498,6 +535,15 @@ bool eval_supported(int A, int AR) {
 
 hipError_t launch_eval_fp64(int A, int AR, int pm, const EvalArgs& a, hipStream_t s) {
 #define PIB_CASE(a_, ar_) if (A == a_ && AR == ar_) return launch_a<a_, ar_>(pm, a, s);
+  PIB_EVAL_CASES(PIB_CASE)
+#undef PIB_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_eval_batched(int A, int AR, int pm, const EvalArgs* probs, const BatchTile* tiles,
+                               int ntiles, int* counter, int waves, hipStream_t s) {
+#define PIB_CASE(a_, ar_) \
+  if (A == a_ && AR == ar_) return launch_batched_a<a_, ar_>(pm, probs, tiles, ntiles, counter, waves, s);
   PIB_EVAL_CASES(PIB_CASE)
 #undef PIB_CASE
   return hipErrorInvalidValue;

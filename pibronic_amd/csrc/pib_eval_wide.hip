@@ -124,14 +124,10 @@ __device__ __forceinline__ void hestenes_step(double (&v)[kW], double& al, int j
   al = a2;
 }
 
+// One sample per wave; L is the wave's private LDS tile set.
 template <bool PM, bool DBG>
-__global__ void __launch_bounds__(kWave * kWideWaves) k_eval_wide(const EvalArgs args) {
-  __shared__ WideLds lds_all[kWideWaves];
+__device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t sample, WideLds& L) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = threadIdx.x / kWave;
-  WideLds& L = lds_all[wv];
-  const int64_t sample = (int64_t)blockIdx.x * kWideWaves + wv;
-  if (sample >= args.B) return;  // whole wave; no work-group barrier is used below
   const int g = lane / kW, j = lane % kW;
   const int A = args.A, AR = args.AR, N = args.N, P = args.P;
   const bool sep = AR != A;
@@ -356,6 +352,50 @@ __global__ void __launch_bounds__(kWave * kWideWaves) k_eval_wide(const EvalArgs
     }
     args.out[OUT_LOGS * st + sample] = ls;
   }
+}
+
+template <bool PM, bool DBG>
+__global__ void __launch_bounds__(kWave * kWideWaves) k_eval_wide(const EvalArgs args) {
+  __shared__ WideLds lds_all[kWideWaves];
+  const int wv = threadIdx.x / kWave;
+  const int64_t sample = (int64_t)blockIdx.x * kWideWaves + wv;
+  if (sample >= args.B) return;  // whole wave; no work-group barrier is used
+  eval_wide_sample<PM, DBG>(args, sample, lds_all[wv]);
+}
+
+// Persistent batched launch (see k_eval_batched): a tile is a range of samples ("waves") of
+// one problem; problems may differ in A, A_rho, N and P (all runtime here).
+template <bool PM>
+__global__ void __launch_bounds__(kWave * kWideWaves)
+k_eval_wide_batched(const EvalArgs* __restrict__ probs, const BatchTile* __restrict__ tiles, int ntiles,
+                    int* counter) {
+  __shared__ WideLds lds_all[kWideWaves];
+  const int lane = threadIdx.x & (kWave - 1);
+  WideLds& L = lds_all[threadIdx.x / kWave];
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(counter, 1);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, kWave));
+    if (t >= ntiles) break;
+    const BatchTile tile = tiles[t];
+    const EvalArgs& a = probs[tile.prob];
+    for (int64_t smp = tile.wave0; smp < tile.wave0 + tile.nwaves; ++smp) {
+      if (smp < a.B) eval_wide_sample<PM, false>(a, smp, L);
+      wave_sync();
+    }
+  }
+}
+
+hipError_t launch_eval_wide_batched(int pm, const EvalArgs* probs, const BatchTile* tiles, int ntiles,
+                                    int* counter, int waves, hipStream_t s) {
+  const dim3 grid((unsigned)((waves + kWideWaves - 1) / kWideWaves));
+  if (pm)
+    hipLaunchKernelGGL((k_eval_wide_batched<true>), grid, dim3(kWave * kWideWaves), 0, s, probs, tiles,
+                       ntiles, counter);
+  else
+    hipLaunchKernelGGL((k_eval_wide_batched<false>), grid, dim3(kWave * kWideWaves), 0, s, probs, tiles,
+                       ntiles, counter);
+  return hipGetLastError();
 }
 
 hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s) {

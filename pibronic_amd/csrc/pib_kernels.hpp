@@ -57,6 +57,7 @@ struct SampleArgs {
   const double* shift;    // A_rho * N sampling-model state shift
   double* q;              // (B, N, P) output
   int32_t* src;           // (B,) surface sources, optional
+  const int64_t* step;    // optional device block counter: the offset becomes offset + step * B
 };
 
 enum : int {
@@ -72,11 +73,25 @@ struct SumArgs {
   double delta_beta;
   const double* logq;  // nullptr: denominator rho (row OUT_RHO); else the proposal density
                        // q = exp(logq), rescaled like g: exp(logq - OUT_LOGS)
+  const int64_t* step;  // optional device block counter: the sums go to sums + step * step_stride
+  int64_t step_stride;  // and are skipped when step >= max_steps
+  int64_t max_steps;
+};
+
+// One tile of a batched launch: waves [wave0, wave0 + nwaves) of problem `prob`.
+struct BatchTile {
+  int prob;
+  int nwaves;
+  int64_t wave0;
 };
 
 // launchers (return hipError_t of the launch)
 hipError_t launch_eval_fp64(int A, int AR, int pm, const EvalArgs& a, hipStream_t s);
 hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s);
+hipError_t launch_eval_batched(int A, int AR, int pm, const EvalArgs* probs, const BatchTile* tiles,
+                               int ntiles, int* counter, int waves, hipStream_t s);
+hipError_t launch_eval_wide_batched(int pm, const EvalArgs* probs, const BatchTile* tiles, int ntiles,
+                                    int* counter, int waves, hipStream_t s);
 constexpr int kWideMaxStates = 16;
 hipError_t launch_sample_fp64(const SampleArgs& a, hipStream_t s);
 hipError_t launch_block_sums(const SumArgs& a, double* partials, int nparts, double* sums,

@@ -69,7 +69,13 @@ __global__ void __launch_bounds__(kSumThreads) k_sum_partials(const SumArgs a, d
 }
 
 __global__ void __launch_bounds__(kSumThreads) k_sum_final(const double* partials, int nparts,
-                                                           double* sums) {
+                                                           double* sums, const int64_t* step,
+                                                           int64_t step_stride, int64_t max_steps) {
+  if (step) {
+    const int64_t k = *step;
+    if (k < 0 || k >= max_steps) return;  // uniform over the work-group
+    sums += k * step_stride;
+  }
   __shared__ double sh[NUM_SUMS * kSumThreads];
   double v[NUM_SUMS];
 #pragma unroll
@@ -87,7 +93,8 @@ hipError_t launch_block_sums(const SumArgs& a, double* partials, int nparts, dou
   hipLaunchKernelGGL(k_sum_partials, dim3(nparts), dim3(kSumThreads), 0, s, a, partials);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kSumThreads), 0, s, partials, nparts, sums);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kSumThreads), 0, s, partials, nparts, sums, a.step,
+                     a.step_stride, a.max_steps);
   return hipGetLastError();
 }
 

@@ -60,13 +60,14 @@ __global__ void __launch_bounds__(kWave * kSampleWaves) k_sample_fp64(const Samp
   const int w = threadIdx.x / kWave;
   double* stage = stage_all[w];
   const int64_t step = (int64_t)gridDim.x * kSampleWaves * kWave;
+  const int64_t offset = a.step ? a.offset + *a.step * a.B : a.offset;
   for (int64_t r0 = ((int64_t)blockIdx.x * kSampleWaves + w) * kWave; r0 < rows; r0 += step) {
     // wave-uniform loop: every lane of the wave runs the same chunks
     const int64_t row = r0 + lane;
     const bool live = row < rows;
     const int64_t s = live ? row / N : 0;
     const int j = live ? (int)(row - s * N) : 0;
-    const int64_t gs = a.offset + s;
+    const int64_t gs = offset + s;
     const int src = live ? draw_source(a, gs) : 0;
     const uint64_t grow = (uint64_t)gs * (uint64_t)N + (uint64_t)j;
     const double* __restrict__ dinv = a.chol + (int64_t)j * 3 * P;
