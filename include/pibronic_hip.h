@@ -47,6 +47,11 @@ extern "C" {
 #define PIB_EVAL_NOSCALE 2 /* skip the S12 scaling (S_p = 1), as the explicit golden test */
 #define PIB_EVAL_WIDE 4    /* force the one-wave-per-sample kernel (used for A > 8; any
                               A, A_rho <= 16); a tuning/testing switch, same results       */
+#define PIB_EVAL_FP32 8    /* reduced precision (A <= 8): O-factors, V, Jacobi, M and ring
+                              products in FP32 (power-of-two renormalised), FP64 rotation
+                              parameters for near-degenerate pairs (gap < 1e-6 |V|_F), FP64
+                              log-scale and rho log-sums; error vs FP64 is the caller's to
+                              report (the outputs have the same meaning)                  */
 
 /* rows of the per-sample output matrix written by pib_eval_block */
 #define PIB_OUT_RHO 0  /* s_rho = rho / prod_p S_p             (BoxResult.scaled_rho)       */
@@ -229,7 +234,7 @@ typedef struct pib_batch pib_batch;
 #define PIB_BATCH_WIDE 1  /* one wave per sample, 16x16 padded: any A, A_rho <= 16      */
 #define PIB_BATCH_SEQUENTIAL 2 /* one launch per problem (tuned lanes), same graph step   */
 
-/* flags: PIB_EVAL_PM and/or PIB_EVAL_NOSCALE; delta_beta is used by the +-dbeta sums;
+/* flags: PIB_EVAL_PM, PIB_EVAL_NOSCALE, PIB_EVAL_FP32 (then one launch per problem); delta_beta is used by the +-dbeta sums;
  * max_blocks bounds the per-block sums kept by pib_batch_run. Buffers are the caller's
  * (pib_batch_bind, pib_batch_bind_sums); the batch owns only its schedule and scratch. */
 int pib_batch_create(int n, pib_ctx* const* ctxs, int64_t B, int flags, double delta_beta,

@@ -20,7 +20,7 @@ PIB_ERR_STATE = -3
 PIB_ERR_UNSUPPORTED = -4
 
 SET_RHO, SET_VIB, SET_VIB_PLUS, SET_VIB_MINUS = 0, 1, 2, 3
-EVAL_PM, EVAL_NOSCALE, EVAL_WIDE = 1, 2, 4
+EVAL_PM, EVAL_NOSCALE, EVAL_WIDE, EVAL_FP32 = 1, 2, 4, 8
 BATCH_LANES, BATCH_WIDE, BATCH_SEQUENTIAL = 0, 1, 2
 OUT_RHO, OUT_G, OUT_GP, OUT_GM, OUT_LOGS = 0, 1, 2, 3, 4
 NUM_OUT = 5
@@ -258,13 +258,13 @@ class Batch:
     The Contexts must stay alive, with unchanged constants, while the batch exists (the batch
     keeps references to them)."""
 
-    def __init__(self, contexts, B, pm=False, noscale=False, delta_beta=1.0, max_blocks=1):
+    def __init__(self, contexts, B, pm=False, noscale=False, delta_beta=1.0, max_blocks=1, fp32=False):
         self.lib = load()
         self.contexts = list(contexts)
         self.n, self.B, self.pm, self.max_blocks = len(self.contexts), int(B), bool(pm), int(max_blocks)
         arr = (_vp * self.n)(*[c.h.value for c in self.contexts])
         h = ctypes.c_void_p()
-        flags = (EVAL_PM if pm else 0) | (EVAL_NOSCALE if noscale else 0)
+        flags = (EVAL_PM if pm else 0) | (EVAL_NOSCALE if noscale else 0) | (EVAL_FP32 if fp32 else 0)
         rc = self.lib.pib_batch_create(self.n, arr, self.B, flags, float(delta_beta), self.max_blocks,
                                        ctypes.byref(h))
         if rc != PIB_OK:

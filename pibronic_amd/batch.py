@@ -20,9 +20,9 @@ class TemperatureBatch:
 
     datas: preprocessed BoxData (or BoxDataPM, all the same kind) on one device; they may
     differ in temperature, beads and model. max_blocks: how many per-block sum rows pib run()
-    may write."""
+    may write. fp32: the reduced-precision kernel (PIB_EVAL_FP32)."""
 
-    def __init__(self, datas, block_size=None, max_blocks=1, noscale=False):
+    def __init__(self, datas, block_size=None, max_blocks=1, noscale=False, fp32=False):
         import torch
         self.datas = list(datas)
         if not self.datas:
@@ -36,7 +36,9 @@ class TemperatureBatch:
         dev = torch.device("cuda", self.device)
         delta_beta = self.datas[0].delta_beta if self.pm else 1.0
         self.batch = _native.Batch([d.context() for d in self.datas], self.B, pm=self.pm,
-                                   noscale=noscale, delta_beta=delta_beta, max_blocks=self.max_blocks)
+                                   noscale=noscale, delta_beta=delta_beta, max_blocks=self.max_blocks,
+                                   fp32=fp32)
+        self.fp32 = bool(fp32)
         self.q = [torch.zeros((self.B, d.modes, d.beads), dtype=torch.float64, device=dev)
                   for d in self.datas]
         self.out = [torch.full((_native.NUM_OUT, self.B), float("nan"), dtype=torch.float64, device=dev)

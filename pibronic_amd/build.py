@@ -18,7 +18,7 @@ ROOT = os.path.dirname(HERE)
 BUILD_DIR = os.path.join(ROOT, "_build")
 LIB_NAME = "libpibronic_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
-SOURCES = ["pib_api.hip", "pib_batch.hip", "pib_eval.hip", "pib_eval_wide.hip", "pib_sample.hip", "pib_reduce.hip", "pib_gemm.hip",
+SOURCES = ["pib_api.hip", "pib_batch.hip", "pib_eval.hip", "pib_eval_wide.hip", "pib_eval_fp32.hip", "pib_sample.hip", "pib_reduce.hip", "pib_gemm.hip",
            "pib_flow.hip"]
 ARCH = "gfx950"
 

@@ -19,8 +19,10 @@ namespace pib {
 
 int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* wide_out) {
   const bool pm = (flags & PIB_EVAL_PM) != 0;
-  if (flags & ~(PIB_EVAL_PM | PIB_EVAL_NOSCALE | PIB_EVAL_WIDE))
+  if (flags & ~(PIB_EVAL_PM | PIB_EVAL_NOSCALE | PIB_EVAL_WIDE | PIB_EVAL_FP32))
     return fail(c, PIB_ERR_ARG, "eval_block: unknown flag");
+  if ((flags & PIB_EVAL_FP32) && (flags & PIB_EVAL_WIDE))
+    return fail(c, PIB_ERR_UNSUPPORTED, "eval_block: the FP32 kernel is the lanes-per-sample kernel (no PIB_EVAL_WIDE)");
   if (!c->coupling_set || c->P < 3) return fail(c, PIB_ERR_STATE, "eval_block: coupling or beads not set");
   const int need = pm ? 4 : 2;
   for (int s = 0; s < need; ++s)
@@ -31,10 +33,16 @@ int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* w
   if (c->set_N[0] != c->N) return fail(c, PIB_ERR_ARG, "eval_block: sampling model has a different number of modes");
   const int AR = c->set_A[SET_RHO];
   const bool fits_wide = c->A <= kWideMaxStates && AR <= kWideMaxStates;
-  const bool wide = fits_wide && ((flags & PIB_EVAL_WIDE) || !eval_supported(c->A, AR));
+  const bool fp32 = (flags & PIB_EVAL_FP32) != 0;
+  const bool wide = !fp32 && fits_wide && ((flags & PIB_EVAL_WIDE) || !eval_supported(c->A, AR));
+  if (fp32)
+    for (int s = 0; s < need; ++s)
+      if (!c->pref_positive[s])
+        return fail(c, PIB_ERR_UNSUPPORTED, "eval_block: the FP32 kernel works with log F_a and needs every O-matrix prefactor > 0");
   if (!wide && !eval_supported(c->A, AR))
-    return fail(c, PIB_ERR_UNSUPPORTED, "eval_block: no FP64 kernel for A=" + std::to_string(c->A) +
-                                             ", A_rho=" + std::to_string(AR));
+    return fail(c, PIB_ERR_UNSUPPORTED, std::string("eval_block: no ") + (fp32 ? "FP32" : "FP64") +
+                                            " kernel for A=" + std::to_string(c->A) +
+                                            ", A_rho=" + std::to_string(AR));
   if (B < 0) return fail(c, PIB_ERR_ARG, "eval_block: B < 0 or out_stride < B");
   EvalArgs a{};
   a.B = B;
@@ -62,6 +70,16 @@ int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* w
   a.wide_lq = (const double2*)c->w_lq.p;
   a.wide_e = (const double*)c->w_e.p;
   a.wide_qpair = (const double*)c->w_qpair.p;
+  for (int s = 0; s < NUM_SETS; ++s) {
+    const int src = (s >= need) ? SET_VIB : s;
+    a.setf[s].tc = (const float*)c->tc_f[src].p;
+    a.setf[s].cs = (const float*)c->cs_f[src].p;
+    a.setf[s].lpref = (const float*)c->lpref_f[src].p;
+  }
+  a.energy_f = (const float*)c->energy_f.p;
+  a.linear_f = (const float*)c->linear_f.p;
+  a.qdiag_f = c->has_qdiag ? (const float*)c->qdiag_f.p : nullptr;
+  a.qpair_f = (const float*)c->qpair_f.p;
   *out_args = a;
   *wide_out = wide;
   return PIB_OK;
@@ -100,6 +118,10 @@ int pib_ctx_destroy(pib_ctx* c) {
   release(c->qdiag);
   release(c->pair);
   release(c->qpair);
+  release(c->energy_f);
+  release(c->linear_f);
+  release(c->qdiag_f);
+  release(c->qpair_f);
   release(c->w_lq);
   release(c->w_e);
   release(c->w_qpair);
@@ -108,6 +130,9 @@ int pib_ctx_destroy(pib_ctx* c) {
     release(c->csch[s]);
     release(c->pref[s]);
     release(c->shift[s]);
+    release(c->tc_f[s]);
+    release(c->cs_f[s]);
+    release(c->lpref_f[s]);
   }
   release(c->s_cdf);
   release(c->s_chol);
@@ -169,6 +194,16 @@ int pib_set_coupling(pib_ctx* c, int A, int N, const double* E, const double* L,
     if ((r = upload(c, c->pair, pairs.data(), pairs.size() * sizeof(int)))) return r;
     if ((r = upload(c, c->qpair, qp.data(), qp.size() * sizeof(double)))) return r;
   }
+  {  // FP32 copies
+    auto tof = [](const double* x, size_t n) { return std::vector<float>(x, x + n); };
+    std::vector<float> ef = tof(E, AA), lf = tof(L, (size_t)N * AA), qdf = tof(qd.data(), qd.size());
+    std::vector<float> qpf = tof(qp.data(), qp.size());
+    if (qpf.empty()) qpf.push_back(0.f);
+    if ((r = upload(c, c->energy_f, ef.data(), ef.size() * sizeof(float)))) return r;
+    if ((r = upload(c, c->linear_f, lf.data(), lf.size() * sizeof(float)))) return r;
+    if ((r = upload(c, c->qdiag_f, qdf.data(), qdf.size() * sizeof(float)))) return r;
+    if ((r = upload(c, c->qpair_f, qpf.data(), qpf.size() * sizeof(float)))) return r;
+  }
   if (A <= kWideMaxStates) {
     // 16 x 16 padded copies for the wide kernel; element (b, c) of every matrix is taken
     // from the lower triangle (max(b,c), min(b,c)), as eigh(UPLO='L') reads V
@@ -209,6 +244,22 @@ int pib_set_omatrix_set(pib_ctx* c, int slot, int A_set, int N, const double* co
   if ((r = upload(c, c->csch[slot], csch, N * sizeof(double)))) return r;
   if ((r = upload(c, c->pref[slot], pref, A_set * sizeof(double)))) return r;
   if ((r = upload(c, c->shift[slot], shift, (size_t)A_set * N * sizeof(double)))) return r;
+  {  // FP32 kernel constants
+    std::vector<float> tc(N), cs(N), lp(A_set);
+    for (int j = 0; j < N; ++j) {
+      tc[j] = (float)(coth[j] - csch[j]);
+      cs[j] = (float)csch[j];
+    }
+    bool pos = true;
+    for (int a = 0; a < A_set; ++a) {
+      pos = pos && pref[a] > 0.0;
+      lp[a] = pref[a] > 0.0 ? (float)std::log(pref[a]) : 0.f;
+    }
+    c->pref_positive[slot] = pos;
+    if ((r = upload(c, c->tc_f[slot], tc.data(), N * sizeof(float)))) return r;
+    if ((r = upload(c, c->cs_f[slot], cs.data(), N * sizeof(float)))) return r;
+    if ((r = upload(c, c->lpref_f[slot], lp.data(), A_set * sizeof(float)))) return r;
+  }
   c->set_A[slot] = A_set;
   c->set_N[slot] = N;
   return PIB_OK;
@@ -254,8 +305,9 @@ int pib_eval_block(pib_ctx* c, const double* q, int64_t B, int flags, double* ou
   a.dbg = dbg;
   PIB_HIP(c, hipSetDevice(c->device));
   const int pm = (flags & PIB_EVAL_PM) ? 1 : 0;
-  hipError_t e = wide ? launch_eval_wide(pm, a, (hipStream_t)stream)
-                      : launch_eval_fp64(c->A, a.AR, pm, a, (hipStream_t)stream);
+  hipError_t e = (flags & PIB_EVAL_FP32) ? launch_eval_fp32(c->A, a.AR, pm, a, (hipStream_t)stream)
+                 : wide                  ? launch_eval_wide(pm, a, (hipStream_t)stream)
+                                         : launch_eval_fp64(c->A, a.AR, pm, a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "eval_block launch");
   return PIB_OK;
 }

@@ -142,7 +142,9 @@ hipError_t launch_batch_eval(pib_batch* b, hipStream_t s) {
   if (b->family == PIB_BATCH_SEQUENTIAL) {  // one launch per problem, tuned lanes
     for (int i = 0; i < b->n; ++i) {
       const EvalArgs& a = b->args[i];
-      hipError_t e = b->lane_ok ? launch_eval_fp64(a.A, a.AR, b->pm, a, s) : launch_eval_wide(b->pm, a, s);
+      hipError_t e = (b->flags & PIB_EVAL_FP32) ? launch_eval_fp32(a.A, a.AR, b->pm, a, s)
+                     : b->lane_ok               ? launch_eval_fp64(a.A, a.AR, b->pm, a, s)
+                                                : launch_eval_wide(b->pm, a, s);
       if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -245,7 +247,7 @@ int pib_batch_create(int n, pib_ctx* const* ctxs, int64_t B, int flags, double d
   *out = nullptr;
   if (n < 1 || !ctxs || B < 1 || max_blocks < 1) return PIB_ERR_ARG;
   if ((flags & PIB_EVAL_PM) && !(delta_beta > 0.0)) return PIB_ERR_ARG;
-  if (flags & ~(PIB_EVAL_PM | PIB_EVAL_NOSCALE)) return PIB_ERR_ARG;
+  if (flags & ~(PIB_EVAL_PM | PIB_EVAL_NOSCALE | PIB_EVAL_FP32)) return PIB_ERR_ARG;
   pib_batch* b = new pib_batch();
   *out = b;
   b->n = n;
@@ -274,6 +276,7 @@ int pib_batch_create(int n, pib_ctx* const* ctxs, int64_t B, int flags, double d
     if (A > kWideMaxStates || AR > kWideMaxStates) b->wide_ok = false;
     b->lanes[i] = auto_lanes(c, B);
   }
+  if (flags & PIB_EVAL_FP32) b->wide_ok = false;  // FP32 runs as one launch per problem
   if (!b->lane_ok && !b->wide_ok) return bfail(b, PIB_ERR_UNSUPPORTED, "batch_create: no kernel family fits every problem");
   if (hipSetDevice(b->device) != hipSuccess) return bfail(b, PIB_ERR_HIP, "batch_create: set device");
   b->nparts = (int)std::max<int64_t>(1, std::min<int64_t>((B + 255) / 256, 1024));
@@ -289,7 +292,7 @@ int pib_batch_create(int n, pib_ctx* const* ctxs, int64_t B, int flags, double d
     b->args[i].out_stride = B;
     b->args[i].dbg = nullptr;
   }
-  b->family = b->lane_ok ? PIB_BATCH_LANES : PIB_BATCH_WIDE;
+  b->family = (flags & PIB_EVAL_FP32) ? PIB_BATCH_SEQUENTIAL : b->lane_ok ? PIB_BATCH_LANES : PIB_BATCH_WIDE;
   b->tile_waves = 4;
   b->grid_waves = b->num_cu * 8;
   PIB_BHIP(b, hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
@@ -334,6 +337,8 @@ int pib_batch_bind_sums(pib_batch* b, double* sums) {
 
 int pib_batch_set_schedule(pib_batch* b, int family, const int* lanes, int tile_waves, int grid_waves) {
   if (!b) return PIB_ERR_ARG;
+  if ((b->flags & PIB_EVAL_FP32) && family != PIB_BATCH_SEQUENTIAL)
+    return bfail(b, PIB_ERR_UNSUPPORTED, "batch_set_schedule: FP32 batches run one launch per problem");
   if (family == PIB_BATCH_LANES && !b->lane_ok)
     return bfail(b, PIB_ERR_UNSUPPORTED, "batch_set_schedule: the lanes family needs one compiled (A, A_rho)");
   if (family == PIB_BATCH_WIDE && !b->wide_ok)
@@ -385,8 +390,9 @@ int pib_batch_eval_sequential(pib_batch* b, void* stream) {
     s.out = a.out;
     s.out_stride = a.out_stride;
     s.dbg = nullptr;
-    hipError_t e = wide ? launch_eval_wide(b->pm, s, (hipStream_t)stream)
-                        : launch_eval_fp64(s.A, s.AR, b->pm, s, (hipStream_t)stream);
+    hipError_t e = (b->flags & PIB_EVAL_FP32) ? launch_eval_fp32(s.A, s.AR, b->pm, s, (hipStream_t)stream)
+                   : wide ? launch_eval_wide(b->pm, s, (hipStream_t)stream)
+                          : launch_eval_fp64(s.A, s.AR, b->pm, s, (hipStream_t)stream);
     if (e != hipSuccess) return bfail(b, PIB_ERR_HIP, std::string("batch_eval_sequential: ") + hipGetErrorString(e));
   }
   return PIB_OK;
@@ -447,7 +453,7 @@ int pib_batch_tune(pib_batch* b, uint64_t seed, double* best_ms, double* sequent
   Cand best{-1, 0, 0, 0, {}, 1e300};
   const int grids[3] = {b->num_cu * 4, b->num_cu * 8, b->num_cu * 16};
   for (int fam = 0; fam < 3; ++fam) {
-    if (fam == PIB_BATCH_LANES && !b->lane_ok) continue;
+    if (fam == PIB_BATCH_LANES && (!b->lane_ok || (b->flags & PIB_EVAL_FP32))) continue;
     if (fam == PIB_BATCH_WIDE && !b->wide_ok) continue;
     Cand cur{fam, 4, 4, b->num_cu * 8, lanes_for(4), 1e300};
     if (fam == PIB_BATCH_SEQUENTIAL) {  // per-problem launches: only the lanes matter

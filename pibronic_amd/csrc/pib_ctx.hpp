@@ -25,6 +25,7 @@ struct pib_ctx {
   int A = 0, N = 0;
   pib::DevBuf energy, linear, qdiag, pair, qpair;
   pib::DevBuf w_lq, w_e, w_qpair;  // padded 16 x 16 constants of the wide kernel (A <= 16)
+  pib::DevBuf energy_f, linear_f, qdiag_f, qpair_f;  // FP32 copies
   int npair = 0;
   bool has_qdiag = false;
   bool coupling_set = false;
@@ -32,6 +33,8 @@ struct pib_ctx {
   int set_A[pib::NUM_SETS] = {0, 0, 0, 0};
   int set_N[pib::NUM_SETS] = {0, 0, 0, 0};
   pib::DevBuf coth[pib::NUM_SETS], csch[pib::NUM_SETS], pref[pib::NUM_SETS], shift[pib::NUM_SETS];
+  pib::DevBuf tc_f[pib::NUM_SETS], cs_f[pib::NUM_SETS], lpref_f[pib::NUM_SETS];
+  bool pref_positive[pib::NUM_SETS] = {false, false, false, false};
   // beads
   int P = 0;
   double tau = 0.0;

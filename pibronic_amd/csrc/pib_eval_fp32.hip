@@ -1,0 +1,400 @@
+// Reduced-precision variant of k_eval_fp64 (same mapping, same outputs, A <= 8).
+//
+// What runs in FP32: the O-matrix exponents, the coupling matrix V, the Jacobi sweeps, M and
+// the ring products. What stays FP64: the coordinate differences x - d (formed once per
+// mode from the FP64 samples), the rotation parameters of near-degenerate Jacobi pairs
+// (|a_qq - a_pp| < 1e-6 |V|_F), the per-sample log-scale sum_p log S_p and the per-surface
+// log-products of rho, and the final assembly of s_g, s_rho.
+//
+// Range: the O-factors are formed in the log domain, log O_a = log F_a - e_a / 2, so that
+// S_p = max_a O_a and O_a / S_p = exp(log O_a - log S_p) never under/overflow before scaling.
+// The ring products are renormalised by a power of two after every bead (exact in binary
+// floating point) and the exponents are carried as integers; s_g = trace * 2^exp in FP64.
+//
+// The O exponent coth (x1^2 + x2^2) - 2 csch x1 x2 cancels when tau w is small; here it is
+// evaluated as tanh(tau w / 2) (x1^2 + x2^2) + csch (x1 - x2)^2, a sum of non-negative terms
+// (the identity coth - csch = tanh(x/2)).
+#include "pib_common.hpp"
+#include "pib_kernels.hpp"
+
+namespace pib {
+
+namespace {
+
+template <int A>
+__device__ __forceinline__ float& UTf(float (&a)[A][A], int i, int j) {
+  return i <= j ? a[i][j] : a[j][i];
+}
+
+// tan of the Jacobi angle (see jacobi_rotate in the FP64 kernel): FP32, or FP64 when the
+// diagonal gap is below `gap` (near-degenerate pair).
+__device__ __forceinline__ float jacobi_t(float app, float aqq, float apq, float gap) {
+  const float d = aqq - app;
+  if (fabsf(d) < gap) {
+    const double dd = (double)aqq - (double)app, h = 2.0 * (double)apq;
+    const double r = sqrt(fma(dd, dd, h * h));
+    const double den = fabs(dd) + r;
+    double t = den > 0.0 ? h / den : 0.0;
+    return (float)(dd < 0.0 ? -t : t);
+  }
+  const float h = 2.0f * apq;
+  const float r = sqrtf(fmaf(d, d, h * h));
+  const float den = fabsf(d) + r;
+  float t = den > 0.0f ? h / den : 0.0f;
+  return d < 0.0f ? -t : t;
+}
+
+template <int A>
+__device__ __forceinline__ void jacobi_rotate_f(float (&a)[A][A], float (&u)[A][A], int p, int q,
+                                                float gap) {
+  const float apq = a[p][q];
+  const float t = jacobi_t(a[p][p], a[q][q], apq, gap);
+  const float c = 1.0f / sqrtf(fmaf(t, t, 1.0f));
+  const float s = t * c;
+  a[p][p] = fmaf(-t, apq, a[p][p]);
+  a[q][q] = fmaf(t, apq, a[q][q]);
+  a[p][q] = 0.0f;
+#pragma unroll
+  for (int r = 0; r < A; ++r) {
+    if (r == p || r == q) continue;
+    const float g = UTf<A>(a, r, p), h = UTf<A>(a, r, q);
+    UTf<A>(a, r, p) = fmaf(c, g, -s * h);
+    UTf<A>(a, r, q) = fmaf(s, g, c * h);
+  }
+#pragma unroll
+  for (int r = 0; r < A; ++r) {
+    const float g = u[r][p], h = u[r][q];
+    u[r][p] = fmaf(c, g, -s * h);
+    u[r][q] = fmaf(s, g, c * h);
+  }
+}
+
+template <int A>
+__device__ __forceinline__ void jacobi_eigh_f(float (&a)[A][A], float (&u)[A][A]) {
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = 0; k < A; ++k) u[i][k] = (i == k) ? 1.0f : 0.0f;
+  if (A == 1) return;
+  float fro2 = 0.0f;
+#pragma unroll
+  for (int i = 0; i < A; ++i) {
+    fro2 = fmaf(a[i][i], a[i][i], fro2);
+#pragma unroll
+    for (int j = i + 1; j < A; ++j) fro2 = fmaf(2.0f * a[i][j], a[i][j], fro2);
+  }
+  const float gap = 1e-6f * sqrtf(fro2);
+  const float tol2 = 1e-12f * fro2;  // relative off-diagonal norm 1e-6
+  constexpr int kMaxSweeps = (A <= 4) ? 12 : 20;
+  for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
+    float off = 0.0f;
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int j = i + 1; j < A; ++j) off = fmaf(a[i][j], a[i][j], off);
+    if (__all(!(off > tol2))) break;
+#pragma unroll
+    for (int p = 0; p < A - 1; ++p)
+#pragma unroll
+      for (int q = p + 1; q < A; ++q) jacobi_rotate_f<A>(a, u, p, q, gap);
+  }
+}
+
+// A x A matrix scaled by a power of two so that its largest magnitude lies in [0.5, 1);
+// the exponent goes to e (exact; zero matrices are left alone)
+template <int A>
+__device__ __forceinline__ void renorm(float (&m)[A][A], int& e) {
+  float mx = 0.0f;
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = 0; k < A; ++k) mx = fmaxf(mx, fabsf(m[i][k]));
+  int ex = 0;
+  (void)frexpf(mx, &ex);
+  if (mx > 0.0f && ex != 0) {
+    const float f = ldexpf(1.0f, -ex);
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int k = 0; k < A; ++k) m[i][k] *= f;
+    e += ex;
+  }
+}
+
+template <int A>
+__device__ __forceinline__ void matmul_left(float (&X)[A][A], const float (&Y)[A][A]) {
+#pragma unroll
+  for (int i = 0; i < A; ++i) {
+    float r[A];
+#pragma unroll
+    for (int c = 0; c < A; ++c) r[c] = X[i][c];
+#pragma unroll
+    for (int k = 0; k < A; ++k) {
+      float acc = r[0] * Y[0][k];
+#pragma unroll
+      for (int c = 1; c < A; ++c) acc = fmaf(r[c], Y[c][k], acc);
+      X[i][k] = acc;
+    }
+  }
+}
+
+__device__ __forceinline__ float shfl_down_f(float v, int d) { return __shfl_down(v, (unsigned)d, kWave); }
+
+}  // namespace
+
+template <int A, int AR, bool PM>
+__global__ void __launch_bounds__(256) k_eval_fp32(const EvalArgs args) {
+  constexpr bool SEP = (AR != A);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int G = args.G, K = args.K, N = args.N, P = args.P;
+  const int gl = lane & (G - 1);
+  const int64_t sample = wave * (kWave / G) + lane / G;
+  const bool live = sample < args.B;
+  const double* __restrict__ qs = args.q + (live ? sample : 0) * (int64_t)N * P;
+  const double* __restrict__ dr = args.set[SET_RHO].shift;
+  const double* __restrict__ dv = args.set[SET_VIB].shift;
+  const OSetF fr = args.setf[SET_RHO], fv = args.setf[SET_VIB];
+  const OSetF fp = args.setf[SET_VIB_PLUS], fm = args.setf[SET_VIB_MINUS];
+  const float* __restrict__ Ev = args.energy_f;
+  const float* __restrict__ Lv = args.linear_f;
+  const float* __restrict__ Qd = args.qdiag_f;
+  const float tau = (float)args.tau;
+
+  float Ng[A][A], Np[A][A], Nm[A][A];
+  int eg = 0, ep_ = 0, em_ = 0;
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = 0; k < A; ++k) {
+      Ng[i][k] = (i == k) ? 1.0f : 0.0f;
+      if (PM) { Np[i][k] = Ng[i][k]; Nm[i][k] = Ng[i][k]; }
+    }
+  double lr[AR];  // sum_p log(O_rho,a / S_p)
+#pragma unroll
+  for (int a = 0; a < AR; ++a) lr[a] = 0.0;
+  double ls = 0.0;  // sum_p log S_p
+
+  const int pbeg = gl * K;
+  const int pend = live ? min(pbeg + K, P) : pbeg;
+  for (int p = pbeg; p < pend; ++p) {
+    const int pn = (p + 1 == P) ? 0 : p + 1;
+    float er[AR], ev[A], epl[A], emi[A], v[A][A];
+#pragma unroll
+    for (int a = 0; a < AR; ++a) er[a] = 0.0f;
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      ev[a] = 0.0f;
+      epl[a] = 0.0f;
+      emi[a] = 0.0f;
+#pragma unroll
+      for (int c = 0; c <= a; ++c) v[a][c] = 0.0f;
+    }
+    for (int j = 0; j < N; ++j) {
+      const double x0 = qs[j * P + p], xn = qs[j * P + pn];
+      {
+        const float tc = fr.tc[j], cs = fr.cs[j];
+#pragma unroll
+        for (int a = 0; a < AR; ++a) {
+          const float x1 = (float)(x0 - dr[a * N + j]), x2 = (float)(xn - dr[a * N + j]);
+          const float dx = x1 - x2;
+          er[a] = fmaf(tc, fmaf(x1, x1, x2 * x2), fmaf(cs, dx * dx, er[a]));
+        }
+      }
+      {
+        const float tc = fv.tc[j], cs = fv.cs[j];
+        float tP = 0.f, cP = 0.f, tM = 0.f, cM = 0.f;
+        if (PM) { tP = fp.tc[j]; cP = fp.cs[j]; tM = fm.tc[j]; cM = fm.cs[j]; }
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+          const float x1 = (float)(x0 - dv[a * N + j]), x2 = (float)(xn - dv[a * N + j]);
+          const float dx = x1 - x2, s2 = fmaf(x1, x1, x2 * x2), d2 = dx * dx;
+          ev[a] = fmaf(tc, s2, fmaf(cs, d2, ev[a]));
+          if (PM) {
+            epl[a] = fmaf(tP, s2, fmaf(cP, d2, epl[a]));
+            emi[a] = fmaf(tM, s2, fmaf(cM, d2, emi[a]));
+          }
+        }
+      }
+      // residual coupling, lower triangle (eigh UPLO='L'), at the sample coordinate
+      const float x = (float)x0;
+#pragma unroll
+      for (int b = 0; b < A; ++b)
+#pragma unroll
+        for (int c = 0; c <= b; ++c) {
+          const float lq = Qd ? fmaf(Qd[(j * A + b) * A + c], x, Lv[(j * A + b) * A + c])
+                              : Lv[(j * A + b) * A + c];
+          v[b][c] = fmaf(x, lq, v[b][c]);
+        }
+    }
+    for (int t = 0; t < args.npair; ++t) {
+      const int jj = args.pair[2 * t], kk = args.pair[2 * t + 1];
+      const float xj = (float)qs[jj * P + p], xk = (float)qs[kk * P + p];
+      const float* Qh = args.qpair_f + (int64_t)t * A * A;
+#pragma unroll
+      for (int b = 0; b < A; ++b)
+#pragma unroll
+        for (int c = 0; c <= b; ++c) v[b][c] = fmaf(xk * Qh[b * A + c], xj, v[b][c]);
+    }
+
+    // log O-factors and the scaling S_p = max over both O's
+    float lOr[AR], lOv[A], lOp[A], lOm[A];
+    float lS = -INFINITY;
+#pragma unroll
+    for (int a = 0; a < AR; ++a) {
+      lOr[a] = fmaf(-0.5f, er[a], fr.lpref[a]);
+      lS = fmaxf(lS, lOr[a]);
+    }
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      lOv[a] = fmaf(-0.5f, ev[a], fv.lpref[a]);
+      lS = fmaxf(lS, lOv[a]);
+      if (PM) {
+        lOp[a] = fmaf(-0.5f, epl[a], fp.lpref[a]);
+        lOm[a] = fmaf(-0.5f, emi[a], fm.lpref[a]);
+      }
+    }
+    if (args.noscale) lS = 0.0f;
+    ls += (double)lS;
+#pragma unroll
+    for (int a = 0; a < AR; ++a) lr[a] += (double)(lOr[a] - lS);
+
+    float a_[A][A], u[A][A];
+#pragma unroll
+    for (int b = 0; b < A; ++b)
+#pragma unroll
+      for (int c = 0; c <= b; ++c) a_[c][b] = v[b][c] + Ev[b * A + c];
+    jacobi_eigh_f<A>(a_, u);
+    float ex[A];
+#pragma unroll
+    for (int d = 0; d < A; ++d) ex[d] = expf(-tau * a_[d][d]);
+    float M[A][A];
+#pragma unroll
+    for (int c = 0; c < A; ++c)
+#pragma unroll
+      for (int e = 0; e <= c; ++e) {
+        float s = (u[c][0] * ex[0]) * u[e][0];
+#pragma unroll
+        for (int d = 1; d < A; ++d) s = fmaf(u[c][d] * ex[d], u[e][d], s);
+        M[c][e] = s;
+        M[e][c] = s;
+      }
+    float Ov[A], Opl[A], Omi[A];
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      Ov[a] = expf(lOv[a] - lS);
+      if (PM) {
+        Opl[a] = expf(lOp[a] - lS);
+        Omi[a] = expf(lOm[a] - lS);
+      }
+    }
+    // ring: N <- N M diag(O / S), renormalised
+#pragma unroll
+    for (int i = 0; i < A; ++i) {
+      float rg[A], rp_[A], rm_[A];
+#pragma unroll
+      for (int c = 0; c < A; ++c) {
+        rg[c] = Ng[i][c];
+        if (PM) { rp_[c] = Np[i][c]; rm_[c] = Nm[i][c]; }
+      }
+#pragma unroll
+      for (int k = 0; k < A; ++k) {
+        float s = rg[0] * M[0][k];
+#pragma unroll
+        for (int c = 1; c < A; ++c) s = fmaf(rg[c], M[c][k], s);
+        Ng[i][k] = s * Ov[k];
+        if (PM) {
+          float sp = rp_[0] * M[0][k], sm = rm_[0] * M[0][k];
+#pragma unroll
+          for (int c = 1; c < A; ++c) {
+            sp = fmaf(rp_[c], M[c][k], sp);
+            sm = fmaf(rm_[c], M[c][k], sm);
+          }
+          Np[i][k] = sp * Opl[k];
+          Nm[i][k] = sm * Omi[k];
+        }
+      }
+    }
+    renorm<A>(Ng, eg);
+    if (PM) {
+      renorm<A>(Np, ep_);
+      renorm<A>(Nm, em_);
+    }
+  }
+
+  // ordered tree over the G lanes of a sample (chunk 0 * chunk 1 * ...)
+  for (int s = 1; s < G; s <<= 1) {
+    const bool leader = (gl & (2 * s - 1)) == 0;
+#pragma unroll
+    for (int w = 0; w < (PM ? 3 : 1); ++w) {
+      float (&X)[A][A] = (w == 0) ? Ng : (w == 1) ? Np : Nm;
+      int& xe = (w == 0) ? eg : (w == 1) ? ep_ : em_;
+      float o[A][A];
+#pragma unroll
+      for (int i = 0; i < A; ++i)
+#pragma unroll
+        for (int k = 0; k < A; ++k) o[i][k] = shfl_down_f(X[i][k], s);
+      const int oe = __shfl_down(xe, (unsigned)s, kWave);
+      if (leader) {
+        matmul_left<A>(X, o);
+        xe += oe;
+        renorm<A>(X, xe);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < AR; ++a) {
+      const double t = shfl_down_d(lr[a], s);
+      if (leader) lr[a] += t;
+    }
+    const double tl = shfl_down_d(ls, s);
+    if (leader) ls += tl;
+  }
+
+  if (live && gl == 0) {
+    const int64_t st = args.out_stride;
+    double lmax = lr[0];
+#pragma unroll
+    for (int a = 1; a < AR; ++a) lmax = fmax(lmax, lr[a]);
+    double rho = 0.0;
+#pragma unroll
+    for (int a = 0; a < AR; ++a) rho += exp(lr[a] - lmax);
+    rho *= exp(lmax);
+    float tg = Ng[0][0], tp = PM ? Np[0][0] : 0.f, tm = PM ? Nm[0][0] : 0.f;
+#pragma unroll
+    for (int i = 1; i < A; ++i) {
+      tg += Ng[i][i];
+      if (PM) { tp += Np[i][i]; tm += Nm[i][i]; }
+    }
+    args.out[OUT_RHO * st + sample] = rho;
+    args.out[OUT_G * st + sample] = ldexp((double)tg, eg);
+    if (PM) {
+      args.out[OUT_GP * st + sample] = ldexp((double)tp, ep_);
+      args.out[OUT_GM * st + sample] = ldexp((double)tm, em_);
+    }
+    args.out[OUT_LOGS * st + sample] = ls;
+  }
+}
+
+template <int A, int AR>
+static hipError_t launch_f(int pm, const EvalArgs& a, hipStream_t s) {
+  const int spw = kWave / a.G;
+  const int64_t waves = (a.B + spw - 1) / spw;
+  const dim3 grid((unsigned)((waves * kWave + 255) / 256));
+  if (pm)
+    hipLaunchKernelGGL((k_eval_fp32<A, AR, true>), grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_eval_fp32<A, AR, false>), grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_eval_fp32(int A, int AR, int pm, const EvalArgs& a, hipStream_t s) {
+#define PIB_F32_CASES(X) \
+  X(1, 1) X(2, 2) X(3, 3) X(4, 4) X(5, 5) X(6, 6) X(7, 7) X(8, 8) X(2, 1) X(2, 4) X(2, 8) X(4, 2) X(4, 8)
+#define PIB_CASE(a_, ar_) if (A == a_ && AR == ar_) return launch_f<a_, ar_>(pm, a, s);
+  PIB_F32_CASES(PIB_CASE)
+#undef PIB_CASE
+#undef PIB_F32_CASES
+  return hipErrorInvalidValue;
+}
+
+}  // namespace pib

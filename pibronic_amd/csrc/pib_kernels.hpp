@@ -16,6 +16,15 @@ struct OSet {
 
 enum : int { SET_RHO = 0, SET_VIB = 1, SET_VIB_PLUS = 2, SET_VIB_MINUS = 3, NUM_SETS = 4 };
 
+// FP32 copies for the reduced-precision kernel: the O-matrix exponent in the
+// cancellation-free form tc (x1^2 + x2^2) + cs (x1 - x2)^2 with tc = coth - csch =
+// tanh(tau w / 2) and cs = csch, and log F_a.
+struct OSetF {
+  const float* tc;     // N
+  const float* cs;     // N
+  const float* lpref;  // A_set
+};
+
 // Output rows of the per-sample result matrix (each row has out_stride entries).
 enum : int { OUT_RHO = 0, OUT_G = 1, OUT_GP = 2, OUT_GM = 3, OUT_LOGS = 4, NUM_OUT = 5 };
 
@@ -42,6 +51,12 @@ struct EvalArgs {
   const double2* wide_lq;     // N * 16 * 16 pairs {L[j,b,c], 0.5 Q[j,j,b,c]}
   const double* wide_e;       // 16 * 16
   const double* wide_qpair;   // npair * 16 * 16
+  // FP32 kernel: constants in float (shifts stay double: x - d is formed in FP64)
+  OSetF setf[NUM_SETS];
+  const float* energy_f;
+  const float* linear_f;
+  const float* qdiag_f;       // or nullptr
+  const float* qpair_f;
 };
 
 __host__ __device__ inline int eval_debug_record(int A, int AR) { return AR + A + 1 + A * A; }
@@ -88,6 +103,7 @@ struct BatchTile {
 // launchers (return hipError_t of the launch)
 hipError_t launch_eval_fp64(int A, int AR, int pm, const EvalArgs& a, hipStream_t s);
 hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s);
+hipError_t launch_eval_fp32(int A, int AR, int pm, const EvalArgs& a, hipStream_t s);
 hipError_t launch_eval_batched(int A, int AR, int pm, const EvalArgs* probs, const BatchTile* tiles,
                                int ntiles, int* counter, int waves, hipStream_t s);
 hipError_t launch_eval_wide_batched(int pm, const EvalArgs* probs, const BatchTile* tiles, int ntiles,
