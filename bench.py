@@ -183,7 +183,7 @@ def batched_leg(FS, cfg, args, cpu, log):
         d.block_size, d.blocks = B, K + W
         d.preprocess()
         datas.append(d)
-    tb = TemperatureBatch(datas, B, max_blocks=K + W)
+    tb = TemperatureBatch(datas, B, max_blocks=K + W, fp32=args.fp32)
     tune = tb.tune(seed=args.seed)
     log(f"[bench] batched tune: {tune}")
     tb.run(args.seed, 0, W)
@@ -204,6 +204,15 @@ def batched_leg(FS, cfg, args, cpu, log):
     eval_ms = e0.elapsed_time(e1) / 3
     nq = int(np.count_nonzero(np.abs(datas[0].vib.quadratic).sum(axis=(2, 3))))
     flops = sum(algorithmic_flops(cfg["A"], cfg["N"], int(P), nq, False)["eval"] for P in Ps) * B
+    errors = None
+    if args.fp32:  # FP32 vs FP64 on the last block's samples, per temperature
+        from pibronic_amd.precision import precision_report
+        errors = []
+        for i, d in enumerate(datas):
+            rep = precision_report(d, tb.q[i], B, pm=False)
+            errors.append({"T": float(Ts[i]), "P": int(Ps[i]), "r_median_rel": rep["r"]["median_rel"],
+                           "r_max_rel": rep["r"]["max_rel"], "s_rho_max_rel": rep["s_rho"]["max_rel"],
+                           "z_shift_sigma": rep["z_shift_sigma"]})
     achieved = flops / (eval_ms * 1e-3) / 1e12
     per_t = []
     ess_tot = 0.0
@@ -219,7 +228,7 @@ def batched_leg(FS, cfg, args, cpu, log):
         "value": ess_tot / el, "unit": "effective samples/s (harmonic rho, all temperatures)",
         "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": el / K * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (seeded model, device Philox samples)",
+        "dtype": "f32" if args.fp32 else "f64", "data": "synthetic (seeded model, device Philox samples)",
         "config": {"workload": f"A6_N12_T100-500: A={cfg['A']} N={cfg['N']}, 16 temperatures "
                                f"100-500 K, P_i = ceil(beta_i/tau0), tau0 = beta(100 K)/256 "
                                f"(P {int(Ps.min())}-{int(Ps.max())}), block={B} per temperature",
@@ -233,8 +242,11 @@ def batched_leg(FS, cfg, args, cpu, log):
         "sequential_eval_ms": tune["sequential_ms"],
         "batched_speedup_vs_sequential": tune["sequential_ms"] / tune["batched_ms"],
         "temperatures": per_t,
+        "fp32_error_vs_fp64": errors,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "peak_note": "78.6 TFLOP/s is also the non-packed FP32 VALU rate (packed FP32 "
+                                  "is disabled in this build)",
                      "kernel": {"lanes": "k_eval_batched (persistent)", "wide": "k_eval_wide_batched (persistent)",
                                 "sequential": "k_eval_fp64, one launch per temperature in the graph step"
                                 }[tune["schedule"]["family"]],
@@ -266,6 +278,9 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0, help="0 = min(16, os.cpu_count())")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--fp32", action="store_true",
+                    help="reduced-precision kernel (PIB_EVAL_FP32); the error against FP64 on the "
+                         "same samples is reported")
     ap.add_argument("--workload", default="A4_N8_P64_T300")
     ap.add_argument("--train-steps", type=int, default=1000)
     ap.add_argument("--train-batch", type=int, default=2048)
@@ -327,7 +342,7 @@ def main():
     stream = torch.cuda.current_stream()
     K, W = args.steps, args.warmup
     sums = torch.zeros((K + W, _native.NUM_SUMS), dtype=torch.float64, device="cuda")
-    flags = _native.EVAL_PM if pm else 0
+    flags = (_native.EVAL_PM if pm else 0) | (_native.EVAL_FP32 if args.fp32 else 0)
     ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
     ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
 
@@ -366,7 +381,13 @@ def main():
     fl = algorithmic_flops(A, N, P, nq, pm)
     achieved = fl["eval"] * B / (eval_ms * 1e-3) / 1e12
     traffic, tdetail = None, "profile only for the default workload"
-    if not pm and B == 100_000:
+    fp32_err = None
+    if args.fp32:
+        from pibronic_amd.precision import precision_report
+        data.fp32 = True
+        fp32_err = precision_report(data, dev["q"], B, pm=pm)
+        data.fp32 = False
+    if not pm and B == 100_000 and not args.fp32:
         traffic, tdetail = profiled_traffic(
             os.path.join(ROOT, "profiles", f"k_eval_fp64_A{A}_N{N}_P{P}_B{B}.md"),
             f"void pib::k_eval_fp64<{A}, {A}, false, false>")
@@ -385,7 +406,7 @@ def main():
         "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": elapsed / K * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (seeded model, device Philox samples)",
+        "dtype": "f32" if args.fp32 else "f64", "data": "synthetic (seeded model, device Philox samples)",
         "config": {"workload": f"{name}: A={A} N={N} P={P} T={T:g}K block={B}"
                                + (" +-dbeta" if pm else ""),
                    "samples_per_step": B, "lanes_per_sample": lanes,
@@ -396,9 +417,10 @@ def main():
         "harmonic": harm,
         "learned": learned,
         "learned_over_harmonic_ess_per_s": (learned["ess_per_s"] / harm["ess_per_s"]) if learned else None,
+        "fp32_error_vs_fp64": fp32_err,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "k_eval_fp64", "flops_per_sample": fl["eval"],
+                     "kernel": "k_eval_fp32" if args.fp32 else "k_eval_fp64", "flops_per_sample": fl["eval"],
                      "algorithmic_bytes": B * N * P * 8 + B * (5 if pm else 3) * 8,
                      "traffic_detail": tdetail},
         "cpu_baseline": None,
