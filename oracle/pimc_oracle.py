@@ -322,3 +322,33 @@ def z_rho(sampling, beta):
     d2 = np.sum((0.5 * w) ** 2 / np.sinh(beta * w / 2.0) ** 2)
     var = np.sum(terms * dlog ** 2) / Z - (np.sum(terms * dlog) / Z) ** 2
     return Z, E_mean, var + d2
+
+
+# --------------------------------------------------------------------------------------
+# training-data hooks (pibronic/pimc/pimc.py:1273-1338)
+# --------------------------------------------------------------------------------------
+def rho_of_R_hook(setup, R):
+    """block_compute_rhoR_from_input_samples' rho(R), unscaled (pibronic/pimc/pimc.py:1273-1299):
+    qTensor = R - d_vib (:1293-1295), then build_o_matrix subtracts the sampling model's shift
+    again (:1092, quirk Q6) and build_denominator takes the trace (:1132-1136)."""
+    B = R.shape[0]
+    q_all = np.empty((B, setup.A, setup.N, setup.P))
+    q_all[:] = R[:, NEW]
+    q_all -= setup.vib["shift"][NEW, :, :, NEW]
+    assert setup.A_rho == setup.A, "the hook reads qTempTensor when A_rho != A"
+    O = o_matrix(q_all, None, setup.rho["shift"], setup.c_rho)
+    return denominator(O)
+
+
+def g_of_R_hook(setup, R):
+    """block_compute_gR_from_raw_samples' g(R), unscaled (pibronic/pimc/pimc.py:1302-1338):
+    qTensor = R - d_vib (generate_random_R_values, :602-611); build_o_matrix with the coupled
+    model subtracts d_vib again (quirk Q6); V is built from the shifted qTensor; no scaling."""
+    B = R.shape[0]
+    q_all = np.empty((B, setup.A, setup.N, setup.P))
+    q_all[:] = R[:, NEW]
+    q_all -= setup.vib["shift"][NEW, :, :, NEW]
+    O = o_matrix(q_all, None, setup.vib["shift"], setup.c_vib)
+    V, lam, U = coupling_eig(q_all, setup.vib["energy"], setup.vib["linear"], setup.vib["quadratic"])
+    M = m_matrix(lam, U, setup.tau)
+    return ring_trace(M, O)

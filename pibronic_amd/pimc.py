@@ -651,3 +651,133 @@ def evaluate_samples(data, q, pm=None, noscale=False, debug=False):
         res.update(O_rho=d[..., :AR], O_vib=d[..., AR:AR + A], S=d[..., AR + A],
                    M=d[..., AR + A + 1:].reshape(B, P, A, A))
     return res
+
+
+# --------------------------------------------------------------------------------------
+# training-data hooks (pibronic/pimc/pimc.py:1216-1338)
+# --------------------------------------------------------------------------------------
+def _hook_context(data, kind):
+    """A second context holding the constants the reference's training-data hooks use.
+
+    Both hooks copy R into qTensor and subtract the coupled model's state shift d once before
+    build_o_matrix subtracts a state shift again (pimc.py:1295 + 1092 and :610 + 1092: quirk
+    Q6). For rho(R) (kind "rho") the sampling-model O-matrices therefore see R - d - d_rho;
+    for g(R) (kind "g") the coupled-model O-matrices see R - 2d, and V is assembled from the
+    shifted copies, row b at R - d_b and column c at R - d_c (diagonalize_coupling_matrix,
+    :1147-1160). Expanding V in R gives E', L' with the shifts folded in (Q unchanged):
+      L'[k,b,c] = L[k,b,c] - 1/2 sum_d Q[d,k,b,c] d_bd - 1/2 sum_e Q[k,e,b,c] d_ce,
+      E'[b,c]   = E[b,c] - sum_d L[d,b,c] d_bd + 1/2 sum_de Q[d,e,b,c] d_ce d_bd;
+    eigh(UPLO='L') reads the lower triangle, which is mirrored before upload.
+    Returns (ctx, asym) with asym = (E'-E'^T, L'-L'^T), the part that makes V non-symmetric.
+    """
+    cache = data.__dict__.setdefault("_hook_ctx", {})
+    if kind in cache:
+        return cache[kind]
+    vib, rho = data.vib, data.rho
+    d = vib.state_shift  # (A, N)
+    ctx = _native.Context(data.device)
+    Qnz = vib.quadratic if np.count_nonzero(vib.quadratic) else None
+    asym = None
+    if kind == "g":
+        Q, L, E = vib.quadratic, vib.linear, vib.energy
+        Lp = (L - 0.5 * np.einsum("dkbc,bd->kbc", Q, d) - 0.5 * np.einsum("kebc,ce->kbc", Q, d))
+        Ep = (E - np.einsum("dbc,bd->bc", L, d) + 0.5 * np.einsum("debc,ce,bd->bc", Q, d, d))
+        asym = (Ep - Ep.T, Lp - Lp.transpose(0, 2, 1))
+        low = np.tril(np.ones(E.shape, dtype=bool))
+        Ep_m = np.where(low, Ep, Ep.T)
+        Lp_m = np.where(low[NEW], Lp, Lp.transpose(0, 2, 1))
+        ctx.set_coupling(Ep_m, Lp_m, Qnz)
+        ctx.set_omatrix_set(_native.SET_RHO, rho.const.cothAN[0], rho.const.cschAN[0],
+                            rho.const.omatrix_prefactor, rho.state_shift)
+        ctx.set_omatrix_set(_native.SET_VIB, vib.const.cothAN[0], vib.const.cschAN[0],
+                            vib.const.omatrix_prefactor, 2.0 * d)
+    elif kind == "rho":
+        if rho.states != data.states:
+            raise NotImplementedError(
+                "block_compute_rhoR_from_input_samples with a sampling model of another "
+                "surface count reads the reference's qTempTensor, which that hook never fills "
+                "from the input R (pibronic/pimc/pimc.py:1094-1096, 1293-1299)")
+        ctx.set_coupling(vib.energy, vib.linear, Qnz)
+        ctx.set_omatrix_set(_native.SET_RHO, rho.const.cothAN[0], rho.const.cschAN[0],
+                            rho.const.omatrix_prefactor, rho.state_shift + d)
+        ctx.set_omatrix_set(_native.SET_VIB, vib.const.cothAN[0], vib.const.cschAN[0],
+                            vib.const.omatrix_prefactor, d)
+    else:
+        raise ValueError(kind)
+    ctx.set_beads(data.beads, data.tau)
+    cache[kind] = (ctx, asym)
+    return cache[kind]
+
+
+def _check_hook_symmetry(data, R, asym):
+    """The reference asserts np.allclose(V^T, V) before eigh (pimc.py:1169). V^T - V is affine
+    in R here; when it is not identically zero, V is formed on the host for the check."""
+    dE, dL = asym
+    if not (np.any(dE) or np.any(dL)):
+        return
+    vib = data.vib
+    d = vib.state_shift
+    for s in range(R.shape[0]):
+        q = R[s][NEW] - d[:, :, NEW]  # (A, N, P): q_b = R - d_b
+        V = (np.einsum("acef,debc,abdf->fbc", q[NEW], 0.5 * vib.quadratic, q[NEW])
+             + np.einsum("dbc,abdf->fbc", vib.linear, q[NEW]) + vib.energy[NEW])
+        assert np.allclose(V.transpose(0, 2, 1), V), \
+            "V(R) is not symmetric for the shifted training coordinates (pibronic/pimc/pimc.py:1169)"
+
+
+def _hook_eval(data, kind, R):
+    import torch
+    ctx, asym = _hook_context(data, kind)
+    if kind == "g":
+        _check_hook_symmetry(data, R, asym)
+    B = R.shape[0]
+    dev = torch.device("cuda", data.device)
+    q = torch.from_numpy(np.ascontiguousarray(R, dtype=F64)).to(dev)
+    out = torch.empty((_native.NUM_OUT, max(B, 1)), dtype=torch.float64, device=dev)
+    ctx.eval_block(q, B, _native.EVAL_NOSCALE, out, out.shape[1],
+                   None, torch.cuda.current_stream(data.device))
+    torch.cuda.synchronize(dev)
+    return out[_native.OUT_G if kind == "g" else _native.OUT_RHO, :B].cpu().numpy()
+
+
+def block_compute_rhoR_from_input_samples(data, result, input_R_values):
+    """rho(R) (unscaled) at the caller's bead coordinates R (X, N, P) into result.scaled_rho
+    (pibronic/pimc/pimc.py:1273-1299), with the reference's shift of R by the coupled model's
+    state shift (quirk Q6)."""
+    R = np.asarray(input_R_values, dtype=F64)
+    for block_index in range(data.blocks):
+        sl = slice(block_index * data.block_size, (block_index + 1) * data.block_size)
+        result.scaled_rho[sl] = _hook_eval(data, "rho", R[sl])
+    return
+
+
+def block_compute_gR_from_raw_samples(data, result, rng=None):
+    """g(R) (unscaled) at uniform random R in [0, 1) (X, N, P), not drawn from rho, into
+    result.scaled_g, then the training files of save_gR_with_samples
+    (pibronic/pimc/pimc.py:1302-1338; quirk Q6 as in _hook_context). The reference draws R
+    with the global legacy np.random; here `rng` (default: numpy Generator seeded with
+    data.seed) makes a run reproducible."""
+    rng = np.random.default_rng(data.seed) if rng is None else rng
+    X = data.blocks * data.block_size
+    input_R_values = np.empty((X, data.modes, data.beads), dtype=F64)
+    end = 0
+    for block_index in range(data.blocks):
+        sl = slice(block_index * data.block_size, (block_index + 1) * data.block_size)
+        input_R_values[sl] = rng.random((data.block_size, data.modes, data.beads))
+        result.scaled_g[sl] = _hook_eval(data, "g", input_R_values[sl])
+        end = sl.stop
+    if result.path_root is not None:
+        save_gR_with_samples(data, result, input_R_values)
+    return input_R_values
+
+
+def save_gR_with_samples(data, result, input_R_values):
+    """The two npz files of a g(R) training run (pibronic/pimc/pimc.py:1248-1270): g with
+    number_of_samples, and the input R values."""
+    result.partial_name = partial(file_structure.training_data_g_output().format, P=data.beads,
+                                  T=data.temperature)
+    np.savez(result.compute_path_to_file(), number_of_samples=result.samples, g=result.scaled_g)
+    result.partial_name = partial(file_structure.training_data_input().format, P=data.beads,
+                                  T=data.temperature)
+    np.savez(result.compute_path_to_file(), number_of_samples=result.samples,
+             input_R_values=input_R_values)
