@@ -456,6 +456,11 @@ int pib_batch_tune(pib_batch* b, uint64_t seed, double* best_ms, double* sequent
     if (fam == PIB_BATCH_LANES && (!b->lane_ok || (b->flags & PIB_EVAL_FP32))) continue;
     if (fam == PIB_BATCH_WIDE && !b->wide_ok) continue;
     Cand cur{fam, 4, 4, b->num_cu * 8, lanes_for(4), 1e300};
+    if (best.family >= 0) {  // prune a family whose default schedule is far behind
+      Cand probe = cur;
+      if ((rc = run(probe))) return rc;
+      if (probe.ms > 1.5 * best.ms) continue;
+    }
     if (fam == PIB_BATCH_SEQUENTIAL) {  // per-problem launches: only the lanes matter
       if (!b->lane_ok) {
         if ((rc = run(cur))) return rc;
