@@ -134,7 +134,8 @@ def learned_leg(data, args, log):
     import torch
     from pibronic_amd import _native, flow, learn
     t0 = time.perf_counter()
-    fl = flow.Flow(data, layers=args.flow_layers, hidden=args.flow_hidden, seed=args.seed)
+    fl = flow.Flow(data, layers=args.flow_layers, hidden=args.flow_hidden, seed=args.seed,
+                   gemm=args.flow_gemm)
     hist = learn.train(fl, args.train_steps, args.train_batch, args.lr, args.seed + 1,
                        log=lambda st: log(f"[bench] train step {st['step']} ess {st['ess_frac']:.3e}"),
                        log_every=max(1, args.train_steps // 5))
@@ -163,6 +164,7 @@ def learned_leg(data, args, log):
     res = weight_summary(sums[W:].sum(dim=0).cpu().numpy(), el)
     res.update(train_seconds=train_s, train_steps=args.train_steps, train_batch=args.train_batch,
                samples_per_step=B, layers=args.flow_layers, hidden=args.flow_hidden,
+               gemm=args.flow_gemm,
                train_final_batch_ess=hist[-1]["ess_frac"] if hist else None)
     return res
 
@@ -282,12 +284,14 @@ def main():
                     help="reduced-precision kernel (PIB_EVAL_FP32); the error against FP64 on the "
                          "same samples is reported")
     ap.add_argument("--workload", default="A4_N8_P64_T300")
-    ap.add_argument("--train-steps", type=int, default=1000)
+    ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--train-batch", type=int, default=2048)
     ap.add_argument("--flow-block", type=int, default=65536)
     ap.add_argument("--flow-layers", type=int, default=8)
     ap.add_argument("--flow-hidden", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--flow-gemm", default="bf16x3", choices=["f32", "bf16x3"],
+                    help="precision of the flow's dense layers (FP32 MFMA or BF16x3 split)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this many MiB 3 times "

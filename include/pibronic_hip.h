@@ -177,6 +177,14 @@ int pib_gemm_f32(pib_ctx* ctx, int trans_a, int trans_b, int64_t M, int64_t N, i
                  int64_t ldc, const float* bias, int relu, const float* mask, int64_t ldmask,
                  int accumulate, void* stream);
 
+/* The same product with each FP32 operand split into bf16 hi + lo parts and
+ * C = A_hi B_hi + A_hi B_lo + A_lo B_hi on the bf16 matrix cores, FP32 accumulation
+ * (~2^-16 relative error per product term instead of FP32's 2^-24; K a multiple of 32). */
+int pib_gemm_bf16x3(pib_ctx* ctx, int trans_a, int trans_b, int64_t M, int64_t N, int64_t K,
+                 const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                 int64_t ldc, const float* bias, int relu, const float* mask, int64_t ldmask,
+                 int accumulate, void* stream);
+
 /* Normal-mode basis of the flow: U_host (P*P, the circulant eigenvectors, row p = bead),
  * sigma_host (N*P, harmonic standard deviations per mode and normal mode) and m_host (N, the
  * centre subtracted before rotating). P must be a multiple of 64 (MFMA tiles of the

@@ -48,6 +48,8 @@ SIGNATURES = {
     "pib_block_sums_q": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_int, _c_dbl, _vp, _vp, _vp]),
     "pib_gemm_f32": (_c_int, [_vp, _c_int, _c_int, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp,
                               _c_i64, _vp, _c_i64, _vp, _c_int, _vp, _c_i64, _c_int, _vp]),
+    "pib_gemm_bf16x3": (_c_int, [_vp, _c_int, _c_int, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp,
+                                 _c_i64, _vp, _c_i64, _vp, _c_int, _vp, _c_i64, _c_int, _vp]),
     "pib_flow_set_basis": (_c_int, [_vp, _c_int, _c_int, _dp, _dp, _dp]),
     "pib_flow_whiten": (_c_int, [_vp, _c_int, _c_i64, _vp, _vp, _vp, _vp]),
     "pib_flow_coupling": (_c_int, [_vp, _c_int, _c_i64, _c_int, _c_int, _c_int, ctypes.c_float,
@@ -218,6 +220,14 @@ class Context:
                                           _devptr(A), int(lda), _devptr(B), int(ldb), _devptr(C),
                                           int(ldc), _devptr(bias), int(bool(relu)), _devptr(mask),
                                           int(ldmask), int(bool(accumulate)), _stream(stream)))
+
+    def gemm_bf16x3(self, A, B, C, M, N, K, lda, ldb, ldc, trans_a=False, trans_b=False, bias=None,
+                    relu=False, mask=None, ldmask=0, accumulate=False, stream=None):
+        """gemm_f32 on the bf16 matrix cores with a BF16x3 operand split (K % 32 == 0)."""
+        self._check(self.lib.pib_gemm_bf16x3(self.h, int(trans_a), int(trans_b), int(M), int(N), int(K),
+                                             _devptr(A), int(lda), _devptr(B), int(ldb), _devptr(C),
+                                             int(ldc), _devptr(bias), int(bool(relu)), _devptr(mask),
+                                             int(ldmask), int(bool(accumulate)), _stream(stream)))
 
     def flow_set_basis(self, U, sigma, m):
         Uh, sh, mh = _host(U), _host(sigma), _host(m)

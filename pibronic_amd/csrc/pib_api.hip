@@ -440,15 +440,17 @@ static int block_sums_impl(pib_ctx* c, const double* out, int64_t out_stride, in
   return PIB_OK;
 }
 
-int pib_gemm_f32(pib_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, const float* A,
+static int gemm_impl(pib_ctx* c, int bf16x3, int ta, int tb, int64_t M, int64_t N, int64_t K, const float* A,
                  int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
                  const float* bias, int relu, const float* mask, int64_t ldmask, int accumulate,
                  void* stream) {
   if (!c) return PIB_ERR_ARG;
   GemmArgs g{A, B, C, bias, mask, M, N, K, lda, ldb, ldc, ldmask, relu ? 1 : 0, accumulate ? 1 : 0,
-             0, nullptr};
+             0, nullptr, bf16x3};
   if (!A || !B || !C) return fail(c, PIB_ERR_ARG, "gemm_f32: null operand");
-  if (!gemm_shape_ok(g)) return fail(c, PIB_ERR_ARG, "gemm_f32: M and N must be multiples of 64, K of 16");
+  if (!gemm_shape_ok(g))
+    return fail(c, PIB_ERR_ARG, bf16x3 ? "gemm_bf16x3: M and N must be multiples of 64, K of 32"
+                                       : "gemm_f32: M and N must be multiples of 64, K of 16");
   const int64_t rowsA = ta ? K : M, colsA = ta ? M : K, rowsB = tb ? N : K, colsB = tb ? K : N;
   (void)rowsA; (void)rowsB;
   if (lda < colsA || ldb < colsB || ldc < N || (mask && ldmask < N))
@@ -464,6 +466,20 @@ int pib_gemm_f32(pib_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, co
   hipError_t e = launch_gemm_f32(ta != 0, tb != 0, g, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "gemm_f32 launch");
   return PIB_OK;
+}
+
+int pib_gemm_f32(pib_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, const float* A,
+                 int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
+                 const float* bias, int relu, const float* mask, int64_t ldmask, int accumulate,
+                 void* stream) {
+  return gemm_impl(c, 0, ta, tb, M, N, K, A, lda, B, ldb, C, ldc, bias, relu, mask, ldmask, accumulate, stream);
+}
+
+int pib_gemm_bf16x3(pib_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, const float* A,
+                    int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
+                    const float* bias, int relu, const float* mask, int64_t ldmask, int accumulate,
+                    void* stream) {
+  return gemm_impl(c, 1, ta, tb, M, N, K, A, lda, B, ldb, C, ldc, bias, relu, mask, ldmask, accumulate, stream);
 }
 
 int pib_flow_set_basis(pib_ctx* c, int N, int P, const double* U, const double* sigma,

@@ -19,6 +19,7 @@ struct GemmArgs {
   // epilogue (deterministic)
   int64_t kchunk;
   float* partial;
+  int bf16x3;  // 1: BF16x3 split on the bf16 matrix cores (K a multiple of 32)
 };
 
 hipError_t launch_gemm_f32(bool ta, bool tb, const GemmArgs& g, hipStream_t s);

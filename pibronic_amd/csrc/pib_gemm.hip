@@ -20,6 +20,7 @@ namespace pib {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTM = 64, kTN = 64, kTK = 16, kPad = 4;
+constexpr int kTK3 = 32, kLd3 = kTK3 + 8;  // BF16x3: K slab, LDS row of 40 bf16 = 80 B (16 B aligned)
 
 template <bool TA, bool TB>
 __global__ void __launch_bounds__(256) k_gemm_f32(GemmArgs g) {
@@ -121,22 +122,290 @@ __global__ void k_gemm_reduce(GemmArgs g, int splits) {
 
 int gemm_splits(const GemmArgs& g, int num_cu) {
   const int64_t tiles = (g.M / kTM) * (g.N / kTN);
+  const int64_t kq = g.bf16x3 ? kTK3 : kTK;
   int s = 1;
   // split while the grid is under one work-group per CU and each split keeps >= 256 of K
-  while (tiles * s * 2 <= num_cu && g.K % (kTK * s * 2) == 0 && g.K / (s * 2) >= 256) s *= 2;
+  while (tiles * s * 2 <= num_cu && g.K % (kq * s * 2) == 0 && g.K / (s * 2) >= 256) s *= 2;
   return s;
 }
 
+// ---------------------------------------------------------------------------------------
+// BF16x3 variant: each FP32 operand is split into bf16 hi + lo (hi = RNE(x), lo = RNE(x - hi))
+// while it is staged into LDS, and C += A_hi B_hi + A_hi B_lo + A_lo B_hi on the bf16 matrix
+// cores (v_mfma_f32_16x16x32_bf16) with FP32 accumulation; the dropped lo*lo term leaves
+// ~2^-16 relative error per product instead of bf16's 2^-8. Same tiling as k_gemm_f32 with
+// K slabs of 32. MFMA lane layout: lane l supplies A[l % 16][8 (l / 16) + e] and
+// B[8 (l / 16) + e][l % 16], e = 0..7; the result layout is k_gemm_f32's.
+// ---------------------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned uvec4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short f2bf(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (operands are finite)
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmArgs g) {
+  __shared__ __align__(16) unsigned short Ah[kTM][kLd3], Al[kTM][kLd3];
+  __shared__ __align__(16) unsigned short Bh[kTN][kLd3], Bl[kTN][kLd3];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int64_t m0 = (int64_t)blockIdx.y * kTM, n0 = (int64_t)blockIdx.x * kTN;
+  const int64_t kbeg = g.partial ? (int64_t)blockIdx.z * g.kchunk : 0;
+  const int64_t kend = g.partial ? kbeg + g.kchunk : g.K;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int64_t k0 = kbeg; k0 < kend; k0 += kTK3) {
+#pragma unroll
+    for (int e = tid; e < kTM * kTK3; e += 256) {
+      int i, k;
+      if (TA) { k = e / kTM; i = e % kTM; } else { i = e / kTK3; k = e % kTK3; }
+      const int64_t gi = m0 + i, gk = k0 + k;
+      const float x = TA ? g.A[gk * g.lda + gi] : g.A[gi * g.lda + gk];
+      const unsigned short h = f2bf(x);
+      Ah[i][k] = h;
+      Al[i][k] = f2bf(x - bf2f(h));
+    }
+#pragma unroll
+    for (int e = tid; e < kTK3 * kTN; e += 256) {
+      int j, k;
+      if (TB) { j = e / kTK3; k = e % kTK3; } else { k = e / kTN; j = e % kTN; }
+      const int64_t gj = n0 + j, gk = k0 + k;
+      const float x = TB ? g.B[gj * g.ldb + gk] : g.B[gk * g.ldb + gj];
+      const unsigned short h = f2bf(x);
+      Bh[j][k] = h;
+      Bl[j][k] = f2bf(x - bf2f(h));
+    }
+    __syncthreads();
+    const int kq = 8 * (lane >> 4);
+    bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int r = wr * 32 + i * 16 + (lane & 15);
+      ah[i] = __builtin_bit_cast(bf16x8, *(const uvec4*)&Ah[r][kq]);
+      al[i] = __builtin_bit_cast(bf16x8, *(const uvec4*)&Al[r][kq]);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = wc * 32 + j * 16 + (lane & 15);
+      bh[j] = __builtin_bit_cast(bf16x8, *(const uvec4*)&Bh[c][kq]);
+      bl[j] = __builtin_bit_cast(bf16x8, *(const uvec4*)&Bl[c][kq]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+  if (g.partial) {
+    float* P = g.partial + (int64_t)blockIdx.z * g.M * g.N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int64_t col = n0 + wc * 32 + j * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+          P[row * g.N + col] = acc[i][j][r];
+        }
+      }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int64_t col = n0 + wc * 32 + j * 16 + (lane & 15);
+      const float bv = g.bias ? g.bias[col] : 0.f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+        float v = acc[i][j][r] + bv;
+        if (g.relu) v = v > 0.f ? v : 0.f;
+        if (g.mask) v = g.mask[row * g.ldmask + col] > 0.f ? v : 0.f;
+        if (g.accumulate) v += g.C[row * g.ldc + col];
+        g.C[row * g.ldc + col] = v;
+      }
+    }
+}
+
+// Pipelined BF16x3 variant for 16-byte aligned operands: float4 global loads, and the next K
+// slab is loaded into registers while the matrix cores work on the current one.
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) k_gemm_bf16x3_v(GemmArgs g) {
+  __shared__ __align__(16) unsigned short Ah[kTM][kLd3], Al[kTM][kLd3];
+  __shared__ __align__(16) unsigned short Bh[kTN][kLd3], Bl[kTN][kLd3];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int64_t m0 = (int64_t)blockIdx.y * kTM, n0 = (int64_t)blockIdx.x * kTN;
+  const int64_t kbeg = g.partial ? (int64_t)blockIdx.z * g.kchunk : 0;
+  const int64_t kend = g.partial ? kbeg + g.kchunk : g.K;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  // slab element groups (4 floats each) of this thread: q = tid + 256 r, r = 0, 1
+  float4 ra[2], rb[2];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int q = tid + 256 * r;
+      if (TA) {  // A stored K x M: 4 consecutive m
+        const int k = q / 16, i = (q % 16) * 4;
+        ra[r] = *(const float4*)(g.A + (k0 + k) * g.lda + m0 + i);
+      } else {   // A stored M x K: 4 consecutive k
+        const int i = q / 8, k = (q % 8) * 4;
+        ra[r] = *(const float4*)(g.A + (m0 + i) * g.lda + k0 + k);
+      }
+      if (TB) {  // B stored N x K: 4 consecutive k
+        const int j = q / 8, k = (q % 8) * 4;
+        rb[r] = *(const float4*)(g.B + (n0 + j) * g.ldb + k0 + k);
+      } else {   // B stored K x N: 4 consecutive n
+        const int k = q / 16, j = (q % 16) * 4;
+        rb[r] = *(const float4*)(g.B + (k0 + k) * g.ldb + n0 + j);
+      }
+    }
+  };
+  auto split = [](float x, unsigned short& h, unsigned short& l) {
+    h = f2bf(x);
+    l = f2bf(x - bf2f(h));
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int q = tid + 256 * r;
+      const float av[4] = {ra[r].x, ra[r].y, ra[r].z, ra[r].w};
+      const float bv[4] = {rb[r].x, rb[r].y, rb[r].z, rb[r].w};
+      unsigned short h[4], l[4];
+      if (TA) {
+        const int k = q / 16, i = (q % 16) * 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) { split(av[e], h[e], l[e]); Ah[i + e][k] = h[e]; Al[i + e][k] = l[e]; }
+      } else {
+        const int i = q / 8, k = (q % 8) * 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) split(av[e], h[e], l[e]);
+        *(uint2*)&Ah[i][k] = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+        *(uint2*)&Al[i][k] = make_uint2(l[0] | ((unsigned)l[1] << 16), l[2] | ((unsigned)l[3] << 16));
+      }
+      if (TB) {
+        const int j = q / 8, k = (q % 8) * 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) split(bv[e], h[e], l[e]);
+        *(uint2*)&Bh[j][k] = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+        *(uint2*)&Bl[j][k] = make_uint2(l[0] | ((unsigned)l[1] << 16), l[2] | ((unsigned)l[3] << 16));
+      } else {
+        const int k = q / 16, j = (q % 16) * 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) { split(bv[e], h[e], l[e]); Bh[j + e][k] = h[e]; Bl[j + e][k] = l[e]; }
+      }
+    }
+  };
+
+  load(kbeg);
+  for (int64_t k0 = kbeg; k0 < kend; k0 += kTK3) {
+    store();
+    __syncthreads();
+    if (k0 + kTK3 < kend) load(k0 + kTK3);  // in flight during the matrix-core work below
+    const int kq = 8 * (lane >> 4);
+    bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int r = wr * 32 + i * 16 + (lane & 15);
+      ah[i] = __builtin_bit_cast(bf16x8, *(const uvec4*)&Ah[r][kq]);
+      al[i] = __builtin_bit_cast(bf16x8, *(const uvec4*)&Al[r][kq]);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = wc * 32 + j * 16 + (lane & 15);
+      bh[j] = __builtin_bit_cast(bf16x8, *(const uvec4*)&Bh[c][kq]);
+      bl[j] = __builtin_bit_cast(bf16x8, *(const uvec4*)&Bl[c][kq]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+  if (g.partial) {
+    float* P = g.partial + (int64_t)blockIdx.z * g.M * g.N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int64_t col = n0 + wc * 32 + j * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+          P[row * g.N + col] = acc[i][j][r];
+        }
+      }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int64_t col = n0 + wc * 32 + j * 16 + (lane & 15);
+      const float bv = g.bias ? g.bias[col] : 0.f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+        float v = acc[i][j][r] + bv;
+        if (g.relu) v = v > 0.f ? v : 0.f;
+        if (g.mask) v = g.mask[row * g.ldmask + col] > 0.f ? v : 0.f;
+        if (g.accumulate) v += g.C[row * g.ldc + col];
+        g.C[row * g.ldc + col] = v;
+      }
+    }
+}
+
+static bool aligned16(const void* p, int64_t ld) {
+  return ((uintptr_t)p % 16) == 0 && ld % 4 == 0;
+}
+
 bool gemm_shape_ok(const GemmArgs& g) {
-  return g.M > 0 && g.N > 0 && g.K > 0 && g.M % kTM == 0 && g.N % kTN == 0 && g.K % kTK == 0;
+  const int64_t kq = g.bf16x3 ? kTK3 : kTK;
+  return g.M > 0 && g.N > 0 && g.K > 0 && g.M % kTM == 0 && g.N % kTN == 0 && g.K % kq == 0;
 }
 
 hipError_t launch_gemm_f32(bool ta, bool tb, const GemmArgs& g, hipStream_t s) {
   if (!gemm_shape_ok(g)) return hipErrorInvalidValue;
   const int splits = g.partial ? (int)(g.K / g.kchunk) : 1;
-  if (g.partial && (g.kchunk <= 0 || g.kchunk % kTK || g.K % g.kchunk)) return hipErrorInvalidValue;
+  const int64_t kq = g.bf16x3 ? kTK3 : kTK;
+  if (g.partial && (g.kchunk <= 0 || g.kchunk % kq || g.K % g.kchunk)) return hipErrorInvalidValue;
   const dim3 grid((unsigned)(g.N / kTN), (unsigned)(g.M / kTM), (unsigned)splits);
-  if (!ta && !tb) hipLaunchKernelGGL((k_gemm_f32<false, false>), grid, dim3(256), 0, s, g);
+  if (g.bf16x3 && aligned16(g.A, g.lda) && aligned16(g.B, g.ldb)) {
+    if (!ta && !tb) hipLaunchKernelGGL((k_gemm_bf16x3_v<false, false>), grid, dim3(256), 0, s, g);
+    else if (!ta && tb) hipLaunchKernelGGL((k_gemm_bf16x3_v<false, true>), grid, dim3(256), 0, s, g);
+    else if (ta && !tb) hipLaunchKernelGGL((k_gemm_bf16x3_v<true, false>), grid, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((k_gemm_bf16x3_v<true, true>), grid, dim3(256), 0, s, g);
+  } else if (g.bf16x3) {
+    if (!ta && !tb) hipLaunchKernelGGL((k_gemm_bf16x3<false, false>), grid, dim3(256), 0, s, g);
+    else if (!ta && tb) hipLaunchKernelGGL((k_gemm_bf16x3<false, true>), grid, dim3(256), 0, s, g);
+    else if (ta && !tb) hipLaunchKernelGGL((k_gemm_bf16x3<true, false>), grid, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((k_gemm_bf16x3<true, true>), grid, dim3(256), 0, s, g);
+  } else if (!ta && !tb) hipLaunchKernelGGL((k_gemm_f32<false, false>), grid, dim3(256), 0, s, g);
   else if (!ta && tb) hipLaunchKernelGGL((k_gemm_f32<false, true>), grid, dim3(256), 0, s, g);
   else if (ta && !tb) hipLaunchKernelGGL((k_gemm_f32<true, false>), grid, dim3(256), 0, s, g);
   else hipLaunchKernelGGL((k_gemm_f32<true, true>), grid, dim3(256), 0, s, g);

@@ -34,8 +34,15 @@ from . import _native
 
 
 class Flow:
-    def __init__(self, data, layers=8, hidden=256, smax=2.0, seed=0):
+    """gemm: "f32" (FP32 MFMA) or "bf16x3" (bf16 matrix cores with a hi/lo operand split,
+    ~2^-16 relative error per product term) for the coupling networks' dense layers. The
+    whitening transforms always run in FP32. Either way log q is exact for the map computed."""
+
+    def __init__(self, data, layers=8, hidden=256, smax=2.0, seed=0, gemm="f32"):
         import torch
+        if gemm not in ("f32", "bf16x3"):
+            raise ValueError("gemm must be 'f32' or 'bf16x3'")
+        self.gemm = gemm
         self.data = data
         self.ctx = data.context()
         self.N, self.P = data.modes, data.beads
@@ -102,17 +109,20 @@ class Flow:
         if B % 64:
             raise ValueError("flow batches must be a multiple of 64 samples")
 
+    def _gemm(self, *args, **kw):
+        fn = self.ctx.gemm_bf16x3 if self.gemm == "bf16x3" else self.ctx.gemm_f32
+        fn(*args, **kw)
+
     # ------------------------------------------------------------------ MLP of one layer
     def _mlp(self, layer, zp, B, h1, h2, o, stream):
         a_off, _ = self.halves(layer)
         H, h, D = self.H, self.hidden, self.D
-        ctx = self.ctx
-        ctx.gemm_f32(zp[:, a_off:], self.p(layer, "W1"), h1, B, h, H, D, h, h,
-                     bias=self.p(layer, "b1"), relu=True, stream=stream)
-        ctx.gemm_f32(h1, self.p(layer, "W2"), h2, B, h, h, h, h, h,
-                     bias=self.p(layer, "b2"), relu=True, stream=stream)
-        ctx.gemm_f32(h2, self.p(layer, "W3"), o, B, D, h, h, D, D,
-                     bias=self.p(layer, "b3"), stream=stream)
+        self._gemm(zp[:, a_off:], self.p(layer, "W1"), h1, B, h, H, D, h, h,
+                   bias=self.p(layer, "b1"), relu=True, stream=stream)
+        self._gemm(h1, self.p(layer, "W2"), h2, B, h, h, h, h, h,
+                   bias=self.p(layer, "b2"), relu=True, stream=stream)
+        self._gemm(h2, self.p(layer, "W3"), o, B, D, h, h, D, D,
+                   bias=self.p(layer, "b3"), stream=stream)
 
     # ------------------------------------------------------------------ directions
     def forward(self, R0, stream=None):
@@ -190,20 +200,20 @@ class Flow:
             ctx.flow_coupling_inverse_backward(B, D, a_off, b_off, self.smax, ys[l], os_[l], dz,
                                                dout, c, stream)
             # o = h2 W3 + b3
-            ctx.gemm_f32(h2s[l], dout, self.p(l, "W3", g), h, D, B, h, D, D, trans_a=True, stream=stream)
+            self._gemm(h2s[l], dout, self.p(l, "W3", g), h, D, B, h, D, D, trans_a=True, stream=stream)
             ctx.colsum_f32(B, D, dout, D, self.p(l, "b3", g), stream)
-            ctx.gemm_f32(dout, self.p(l, "W3"), dh2, B, h, D, D, D, h, trans_b=True,
+            self._gemm(dout, self.p(l, "W3"), dh2, B, h, D, D, D, h, trans_b=True,
                          mask=h2s[l], ldmask=h, stream=stream)
             # h2 = relu(h1 W2 + b2)
-            ctx.gemm_f32(h1s[l], dh2, self.p(l, "W2", g), h, h, B, h, h, h, trans_a=True, stream=stream)
+            self._gemm(h1s[l], dh2, self.p(l, "W2", g), h, h, B, h, h, h, trans_a=True, stream=stream)
             ctx.colsum_f32(B, h, dh2, h, self.p(l, "b2", g), stream)
-            ctx.gemm_f32(dh2, self.p(l, "W2"), dh1, B, h, h, h, h, h, trans_b=True,
+            self._gemm(dh2, self.p(l, "W2"), dh1, B, h, h, h, h, h, trans_b=True,
                          mask=h1s[l], ldmask=h, stream=stream)
             # h1 = relu(y_A W1 + b1)
-            ctx.gemm_f32(ys[l][:, a_off:], dh1, self.p(l, "W1", g), H, h, B, D, h, h, trans_a=True,
+            self._gemm(ys[l][:, a_off:], dh1, self.p(l, "W1", g), H, h, B, D, h, h, trans_a=True,
                          stream=stream)
             ctx.colsum_f32(B, h, dh1, h, self.p(l, "b1", g), stream)
-            ctx.gemm_f32(dh1, self.p(l, "W1"), dz[:, a_off:], B, H, h, h, h, D, trans_b=True,
+            self._gemm(dh1, self.p(l, "W1"), dz[:, a_off:], B, H, h, h, h, D, trans_b=True,
                          accumulate=True, stream=stream)
         return loss
 
