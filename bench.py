@@ -280,6 +280,9 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0, help="0 = min(16, os.cpu_count())")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--ring-bf16x3", action="store_true",
+                    help="wide kernel with M and ring products as BF16x3 on the bf16 matrix cores "
+                         "(PIB_EVAL_BF16X3); the error against FP64 on the same samples is reported")
     ap.add_argument("--fp32", action="store_true",
                     help="reduced-precision kernel (PIB_EVAL_FP32); the error against FP64 on the "
                          "same samples is reported")
@@ -346,7 +349,8 @@ def main():
     stream = torch.cuda.current_stream()
     K, W = args.steps, args.warmup
     sums = torch.zeros((K + W, _native.NUM_SUMS), dtype=torch.float64, device="cuda")
-    flags = (_native.EVAL_PM if pm else 0) | (_native.EVAL_FP32 if args.fp32 else 0)
+    flags = ((_native.EVAL_PM if pm else 0) | (_native.EVAL_FP32 if args.fp32 else 0)
+             | (_native.EVAL_BF16X3 if args.ring_bf16x3 else 0))
     ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
     ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
 
@@ -386,12 +390,11 @@ def main():
     achieved = fl["eval"] * B / (eval_ms * 1e-3) / 1e12
     traffic, tdetail = None, "profile only for the default workload"
     fp32_err = None
-    if args.fp32:
+    if args.fp32 or args.ring_bf16x3:
         from pibronic_amd.precision import precision_report
-        data.fp32 = True
-        fp32_err = precision_report(data, dev["q"], B, pm=pm)
-        data.fp32 = False
-    if not pm and B == 100_000 and not args.fp32:
+        fp32_err = precision_report(data, dev["q"], B, pm=pm,
+                                    variant="fp32" if args.fp32 else "bf16x3")
+    if not pm and B == 100_000 and not (args.fp32 or args.ring_bf16x3):
         traffic, tdetail = profiled_traffic(
             os.path.join(ROOT, "profiles", f"k_eval_fp64_A{A}_N{N}_P{P}_B{B}.md"),
             f"void pib::k_eval_fp64<{A}, {A}, false, false>")
@@ -410,7 +413,8 @@ def main():
         "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": elapsed / K * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if args.fp32 else "f64", "data": "synthetic (seeded model, device Philox samples)",
+        "dtype": "f32" if args.fp32 else ("bf16x3/f32 ring, f64 eig" if args.ring_bf16x3 else "f64"),
+        "data": "synthetic (seeded model, device Philox samples)",
         "config": {"workload": f"{name}: A={A} N={N} P={P} T={T:g}K block={B}"
                                + (" +-dbeta" if pm else ""),
                    "samples_per_step": B, "lanes_per_sample": lanes,
@@ -421,10 +425,11 @@ def main():
         "harmonic": harm,
         "learned": learned,
         "learned_over_harmonic_ess_per_s": (learned["ess_per_s"] / harm["ess_per_s"]) if learned else None,
-        "fp32_error_vs_fp64": fp32_err,
+        "reduced_precision_error_vs_fp64": fp32_err,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "k_eval_fp32" if args.fp32 else "k_eval_fp64", "flops_per_sample": fl["eval"],
+                     "kernel": ("k_eval_fp32" if args.fp32 else "k_eval_wide" if (A > 8 or args.ring_bf16x3)
+                                else "k_eval_fp64"), "flops_per_sample": fl["eval"],
                      "algorithmic_bytes": B * N * P * 8 + B * (5 if pm else 3) * 8,
                      "traffic_detail": tdetail},
         "cpu_baseline": None,

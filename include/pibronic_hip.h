@@ -52,6 +52,9 @@ extern "C" {
                               parameters for near-degenerate pairs (gap < 1e-6 |V|_F), FP64
                               log-scale and rho log-sums; error vs FP64 is the caller's to
                               report (the outputs have the same meaning)                  */
+#define PIB_EVAL_BF16X3 16 /* wide kernel with M and the ring products as BF16x3 splits on
+                              the bf16 matrix cores, FP32 accumulation, renormalised every
+                              bead (A, A_rho <= 16); Jacobi and O-factors stay FP64        */
 
 /* rows of the per-sample output matrix written by pib_eval_block */
 #define PIB_OUT_RHO 0  /* s_rho = rho / prod_p S_p             (BoxResult.scaled_rho)       */

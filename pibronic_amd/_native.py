@@ -20,7 +20,7 @@ PIB_ERR_STATE = -3
 PIB_ERR_UNSUPPORTED = -4
 
 SET_RHO, SET_VIB, SET_VIB_PLUS, SET_VIB_MINUS = 0, 1, 2, 3
-EVAL_PM, EVAL_NOSCALE, EVAL_WIDE, EVAL_FP32 = 1, 2, 4, 8
+EVAL_PM, EVAL_NOSCALE, EVAL_WIDE, EVAL_FP32, EVAL_BF16X3 = 1, 2, 4, 8, 16
 BATCH_LANES, BATCH_WIDE, BATCH_SEQUENTIAL = 0, 1, 2
 OUT_RHO, OUT_G, OUT_GP, OUT_GM, OUT_LOGS = 0, 1, 2, 3, 4
 NUM_OUT = 5

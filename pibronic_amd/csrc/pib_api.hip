@@ -19,8 +19,10 @@ namespace pib {
 
 int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* wide_out) {
   const bool pm = (flags & PIB_EVAL_PM) != 0;
-  if (flags & ~(PIB_EVAL_PM | PIB_EVAL_NOSCALE | PIB_EVAL_WIDE | PIB_EVAL_FP32))
+  if (flags & ~(PIB_EVAL_PM | PIB_EVAL_NOSCALE | PIB_EVAL_WIDE | PIB_EVAL_FP32 | PIB_EVAL_BF16X3))
     return fail(c, PIB_ERR_ARG, "eval_block: unknown flag");
+  if ((flags & PIB_EVAL_FP32) && (flags & PIB_EVAL_BF16X3))
+    return fail(c, PIB_ERR_ARG, "eval_block: PIB_EVAL_FP32 and PIB_EVAL_BF16X3 are different kernels");
   if ((flags & PIB_EVAL_FP32) && (flags & PIB_EVAL_WIDE))
     return fail(c, PIB_ERR_UNSUPPORTED, "eval_block: the FP32 kernel is the lanes-per-sample kernel (no PIB_EVAL_WIDE)");
   if (!c->coupling_set || c->P < 3) return fail(c, PIB_ERR_STATE, "eval_block: coupling or beads not set");
@@ -34,7 +36,10 @@ int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* w
   const int AR = c->set_A[SET_RHO];
   const bool fits_wide = c->A <= kWideMaxStates && AR <= kWideMaxStates;
   const bool fp32 = (flags & PIB_EVAL_FP32) != 0;
-  const bool wide = !fp32 && fits_wide && ((flags & PIB_EVAL_WIDE) || !eval_supported(c->A, AR));
+  const bool bf = (flags & PIB_EVAL_BF16X3) != 0;
+  if (bf && !fits_wide)
+    return fail(c, PIB_ERR_UNSUPPORTED, "eval_block: BF16x3 ring products are in the wide kernel (A, A_rho <= 16)");
+  const bool wide = !fp32 && fits_wide && (bf || (flags & PIB_EVAL_WIDE) || !eval_supported(c->A, AR));
   if (fp32)
     for (int s = 0; s < need; ++s)
       if (!c->pref_positive[s])
@@ -306,7 +311,7 @@ int pib_eval_block(pib_ctx* c, const double* q, int64_t B, int flags, double* ou
   PIB_HIP(c, hipSetDevice(c->device));
   const int pm = (flags & PIB_EVAL_PM) ? 1 : 0;
   hipError_t e = (flags & PIB_EVAL_FP32) ? launch_eval_fp32(c->A, a.AR, pm, a, (hipStream_t)stream)
-                 : wide                  ? launch_eval_wide(pm, a, (hipStream_t)stream)
+                 : wide ? launch_eval_wide(pm, a, (hipStream_t)stream, (flags & PIB_EVAL_BF16X3) != 0)
                                          : launch_eval_fp64(c->A, a.AR, pm, a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "eval_block launch");
   return PIB_OK;

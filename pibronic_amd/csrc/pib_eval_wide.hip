@@ -86,6 +86,44 @@ __device__ __forceinline__ double group_max(double x) {
   return x;
 }
 
+// BF16x3 ring products: FP32 values split into bf16 hi + lo, products on the bf16 matrix cores
+// (v_mfma_f32_16x16x16_bf16: lane l supplies A[l%16][4(l/16)+e] and B[4(l/16)+e][l%16],
+// result register r = D[4(l/16)+r][l%16]), FP32 accumulation.
+typedef float f4w __attribute__((ext_vector_type(4)));
+typedef short s4w __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short f2bf_w(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ void split_bf(float x, short& h, short& l) {
+  const unsigned short hb = f2bf_w(x);
+  h = (short)hb;
+  l = (short)f2bf_w(x - __uint_as_float((unsigned)hb << 16));
+}
+__device__ __forceinline__ f4w mfma_bf16x3(const s4w& ah, const s4w& al, const s4w& bh, const s4w& bl,
+                                           f4w c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, c, 0, 0, 0);
+}
+// scale a 16x16 FP32 matrix (one wave, 4 values per lane) by a power of two so that its largest
+// magnitude lies in [0.5, 1); the exponent (the same on every lane) goes to e
+__device__ __forceinline__ void renorm_wave(f4w& R, int& e) {
+  float m = fmaxf(fmaxf(fabsf(R[0]), fabsf(R[1])), fmaxf(fabsf(R[2]), fabsf(R[3])));
+#pragma unroll
+  for (int s = 1; s < kWave; s <<= 1) m = fmaxf(m, __shfl_xor(m, s, kWave));
+  int ex = 0;
+  (void)frexpf(m, &ex);
+  if (m > 0.0f && ex != 0) {
+    const float f = ldexpf(1.0f, -ex);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) R[r] *= f;
+    e += ex;
+  }
+}
+
 template <int K>
 struct HiBit {
   static constexpr int v = (K >= 8) ? 8 : (K >= 4) ? 4 : (K >= 2) ? 2 : 1;
@@ -124,8 +162,9 @@ __device__ __forceinline__ void hestenes_step(double (&v)[kW], double& al, int j
   al = a2;
 }
 
-// One sample per wave; L is the wave's private LDS tile set.
-template <bool PM, bool DBG>
+// One sample per wave; L is the wave's private LDS tile set. BF: ring products (and M) as
+// BF16x3 on the bf16 matrix cores with FP32 accumulation, renormalised every bead.
+template <bool PM, bool DBG, bool BF = false>
 __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t sample, WideLds& L) {
   const int lane = threadIdx.x & (kWave - 1);
   const int g = lane / kW, j = lane % kW;
@@ -149,6 +188,15 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
     Rg[r] = (g + 4 * r == j) ? 1.0 : 0.0;
     Rp[r] = Rg[r];
     Rm[r] = Rg[r];
+  }
+  // BF16x3 ring states in the FP32 accumulator layout: reg r of lane l = R^T[4(l/16)+r][l%16]
+  f4w Fg, Fp, Fm;
+  int xg = 0, xp = 0, xm = 0;  // power-of-two exponents of the BF16x3 ring states
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    Fg[r] = (4 * g + r == j) ? 1.0f : 0.0f;
+    Fp[r] = Fg[r];
+    Fm[r] = Fg[r];
   }
   double rp = 1.0;  // rho: product over this group's beads of O_rho,j / S
   double smant = 1.0;
@@ -279,6 +327,59 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
     // ring steps, beads in order (pimc.py:1198-1206)
     const int nb = min(kBeadsPerChunk, P - p0);
     for (int b = 0; b < nb; ++b) {
+      if constexpr (BF) {
+        // M = (U diag(d)) U^T: A[i][k] = U[i][k] d_k, B[k][i'] = U[i'][k], k = 4(l/16) + e
+        s4w ah, al, bh, bl;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int k = 4 * g + e;
+          const double u = L.U[b][j * kURow + k];
+          short h, l;
+          split_bf((float)(u * L.D[b][k]), h, l);
+          ah[e] = h;
+          al[e] = l;
+          split_bf((float)u, h, l);
+          bh[e] = h;
+          bl[e] = l;
+        }
+        const f4w M = mfma_bf16x3(ah, al, bh, bl, f4w{0.f, 0.f, 0.f, 0.f});
+        if (DBG) {
+          const int rec = eval_debug_record(A, AR);
+          double* dr = args.dbg + ((int64_t)sample * P + p0 + b) * rec + AR + A + 1;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = 4 * g + r;
+            if (row < A && j < A) dr[row * A + j] = (double)M[r];
+          }
+        }
+        // Z = M R^T: the A operand M[j][4(l/16)+e] is M's own register e (M symmetric), the
+        // B operand is the ring state's register e
+        s4w mh, ml;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          short h, l;
+          split_bf(M[e], h, l);
+          mh[e] = h;
+          ml[e] = l;
+        }
+#pragma unroll
+        for (int w = 0; w < (PM ? 3 : 1); ++w) {
+          f4w& F = (w == 0) ? Fg : (w == 1) ? Fp : Fm;
+          int& fe = (w == 0) ? xg : (w == 1) ? xp : xm;
+          s4w rh, rl;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            short h, l;
+            split_bf(F[e], h, l);
+            rh[e] = h;
+            rl[e] = l;
+          }
+          const f4w Z = mfma_bf16x3(mh, ml, rh, rl, f4w{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+          for (int r = 0; r < 4; ++r) F[r] = Z[r] * (float)L.O[w][b][4 * g + r];
+          renorm_wave(F, fe);
+        }
+      } else {
       d4 M = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int kb = 0; kb < 4; ++kb) {
@@ -313,18 +414,31 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
           Rm[r] = Zm[r] * L.O[2][b][g + 4 * r];
         }
       }
+      }
     }
   }
 
-  // traces: lane l holds diagonal element (j, j) iff j % 4 == l / 16, in register j / 4
-  const bool diag = (j & 3) == g;
+  // traces: FP64 layout: lane l holds (j, j) iff j % 4 == l / 16, in register j / 4;
+  // FP32 layout: iff j / 4 == l / 16, in register j % 4
   double tg = 0.0, tp = 0.0, tm = 0.0;
+  if constexpr (BF) {
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    if (diag && (j >> 2) == r) {
-      tg = Rg[r];
-      tp = Rp[r];
-      tm = Rm[r];
+    for (int r = 0; r < 4; ++r) {
+      if ((j >> 2) == g && (j & 3) == r) {
+        tg = (double)Fg[r];
+        tp = (double)Fp[r];
+        tm = (double)Fm[r];
+      }
+    }
+  } else {
+    const bool diag = (j & 3) == g;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (diag && (j >> 2) == r) {
+        tg = Rg[r];
+        tp = Rp[r];
+        tm = Rm[r];
+      }
     }
   }
   double ls = log(smant) + (double)sexp * 0.69314718055994530942;
@@ -342,6 +456,11 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
       tm += shfl_xor_d(tm, m);
     }
   }
+  if (BF) {  // exponents are uniform over the wave
+    tg = ldexp(tg, xg);
+    tp = ldexp(tp, xp);
+    tm = ldexp(tm, xm);
+  }
   if (lane == 0) {
     const int64_t st = args.out_stride;
     args.out[OUT_RHO * st + sample] = rho;
@@ -354,13 +473,13 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
   }
 }
 
-template <bool PM, bool DBG>
+template <bool PM, bool DBG, bool BF>
 __global__ void __launch_bounds__(kWave * kWideWaves) k_eval_wide(const EvalArgs args) {
   __shared__ WideLds lds_all[kWideWaves];
   const int wv = threadIdx.x / kWave;
   const int64_t sample = (int64_t)blockIdx.x * kWideWaves + wv;
   if (sample >= args.B) return;  // whole wave; no work-group barrier is used
-  eval_wide_sample<PM, DBG>(args, sample, lds_all[wv]);
+  eval_wide_sample<PM, DBG, BF>(args, sample, lds_all[wv]);
 }
 
 // Persistent batched launch (see k_eval_batched): a tile is a range of samples ("waves") of
@@ -398,19 +517,20 @@ hipError_t launch_eval_wide_batched(int pm, const EvalArgs* probs, const BatchTi
   return hipGetLastError();
 }
 
-hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s) {
+hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s, bool bf16x3) {
   const dim3 grid((unsigned)((a.B + kWideWaves - 1) / kWideWaves));
   const dim3 block(kWave * kWideWaves);
   const bool dbg = a.dbg != nullptr;
-  if (pm && dbg)
-    hipLaunchKernelGGL((k_eval_wide<true, true>), grid, block, 0, s, a);
-  else if (pm)
-    hipLaunchKernelGGL((k_eval_wide<true, false>), grid, block, 0, s, a);
-  else if (dbg)
-    hipLaunchKernelGGL((k_eval_wide<false, true>), grid, block, 0, s, a);
-  else
-    hipLaunchKernelGGL((k_eval_wide<false, false>), grid, block, 0, s, a);
-  return hipGetLastError();
+#define PIB_WIDE(pm_, dbg_, bf_) \
+  if ((pm != 0) == pm_ && dbg == dbg_ && bf16x3 == bf_) { \
+    hipLaunchKernelGGL((k_eval_wide<pm_, dbg_, bf_>), grid, block, 0, s, a); \
+    return hipGetLastError(); \
+  }
+  PIB_WIDE(false, false, false) PIB_WIDE(true, false, false) PIB_WIDE(false, true, false)
+  PIB_WIDE(true, true, false) PIB_WIDE(false, false, true) PIB_WIDE(true, false, true)
+  PIB_WIDE(false, true, true) PIB_WIDE(true, true, true)
+#undef PIB_WIDE
+  return hipErrorInvalidValue;
 }
 
 }  // namespace pib

@@ -102,7 +102,7 @@ struct BatchTile {
 
 // launchers (return hipError_t of the launch)
 hipError_t launch_eval_fp64(int A, int AR, int pm, const EvalArgs& a, hipStream_t s);
-hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s);
+hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s, bool bf16x3 = false);
 hipError_t launch_eval_fp32(int A, int AR, int pm, const EvalArgs& a, hipStream_t s);
 hipError_t launch_eval_batched(int A, int AR, int pm, const EvalArgs* probs, const BatchTile* tiles,
                                int ntiles, int* counter, int waves, hipStream_t s);

@@ -250,6 +250,8 @@ class BoxData:
         self.force_wide = False
         # True: the reduced-precision kernel (FP32 products, PIB_EVAL_FP32; A <= 8)
         self.fp32 = False
+        # True: the wide kernel with BF16x3 ring products (PIB_EVAL_BF16X3; A <= 16)
+        self.ring_bf16x3 = False
         self.seed = None
         self.device = 0
         self._ctx = None
@@ -429,7 +431,8 @@ class BoxData:
         """One pib_eval_block call on device buffers (torch tensors)."""
         import torch
         flags = ((_native.EVAL_PM if pm else 0) | (_native.EVAL_NOSCALE if noscale else 0)
-                 | (_native.EVAL_WIDE if self.force_wide else 0) | (_native.EVAL_FP32 if self.fp32 else 0))
+                 | (_native.EVAL_WIDE if self.force_wide else 0) | (_native.EVAL_FP32 if self.fp32 else 0)
+                 | (_native.EVAL_BF16X3 if self.ring_bf16x3 else 0))
         if stream is None:
             stream = torch.cuda.current_stream(self.device)
         self.context().eval_block(q, B, flags, out, out_stride, debug, stream)

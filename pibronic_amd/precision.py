@@ -1,4 +1,5 @@
-"""Error of the reduced-precision (FP32) kernel against the FP64 kernel, per result.
+"""Error of a reduced-precision kernel (FP32, or BF16x3 ring products) against the FP64
+kernel, per result.
 
 Both kernels run on the same bead coordinates; the report gives, for every per-sample output
 and for the estimator weight r = g / rho, the largest and the median relative difference, the
@@ -16,22 +17,29 @@ def _stats(a32, a64):
     return {"max_rel": float(rel.max()), "median_rel": float(np.median(rel))}
 
 
-def precision_report(data, q_dev, B, pm=None):
-    """FP32 vs FP64 on the device coordinates q_dev (B, N, P) of `data` (a BoxData)."""
+def precision_report(data, q_dev, B, pm=None, variant="fp32"):
+    """variant ("fp32": PIB_EVAL_FP32, "bf16x3": PIB_EVAL_BF16X3) vs the FP64 kernel of the
+    same family on the device coordinates q_dev (B, N, P) of `data` (a BoxData)."""
     import torch
+    if variant not in ("fp32", "bf16x3"):
+        raise ValueError(variant)
+    attr = "fp32" if variant == "fp32" else "ring_bf16x3"
     pm = data.pm if pm is None else pm
     dev = q_dev.device
     outs = {}
-    saved = data.fp32
+    saved = (data.fp32, data.ring_bf16x3, data.force_wide)
     try:
-        for fp32 in (False, True):
-            data.fp32 = fp32
+        data.fp32 = data.ring_bf16x3 = False
+        if variant == "bf16x3":
+            data.force_wide = True  # compare with the FP64 wide kernel
+        for low in (False, True):
+            setattr(data, attr, low)
             out = torch.empty((_native.NUM_OUT, B), dtype=torch.float64, device=dev)
             data.evaluate_device(q_dev, B, pm, out, B)
-            outs[fp32] = out
+            outs[low] = out
         torch.cuda.synchronize(dev)
     finally:
-        data.fp32 = saved
+        data.fp32, data.ring_bf16x3, data.force_wide = saved
     o64, o32 = outs[False].cpu().numpy(), outs[True].cpu().numpy()
     keys = [("s_rho", _native.OUT_RHO), ("s_g", _native.OUT_G)]
     if pm:

@@ -121,3 +121,60 @@ def test_more_than_16_surfaces_is_rejected(tmp_path):
     q, _ = data.draw_host_sample(np.random.default_rng(1), 8)
     with pytest.raises(_native.NativeError, match="no FP64 kernel"):
         pimc.evaluate_samples(data, q, pm=False)
+
+
+# ------------------------------------------------------------------ BF16x3 ring products
+# Stated bounds for PIB_EVAL_BF16X3 (M and the ring chain on the bf16 matrix cores with a
+# hi/lo split and FP32 accumulation; Jacobi and O-factors FP64), against the FP64 wide kernel
+# on the same coordinates: r = g/rho median relative error <= 1e-3 and largest <= 5e-2; the
+# block's Z estimate moves by less than 0.05 of its standard error; rho and the log scale
+# are FP64 paths and agree to 1e-12.
+def _check_bf(rep):
+    print(rep)
+    assert rep["r"]["median_rel"] <= 1e-3, rep
+    assert rep["r"]["max_rel"] <= 5e-2, rep
+    assert abs(rep["z_shift_sigma"]) < 0.05, rep
+    assert rep["s_rho"]["max_rel"] <= 1e-12, rep
+    assert rep["log_scale_max_abs"] <= 1e-9, rep
+
+
+@pytest.mark.parametrize("pm", [False, True])
+def test_bf16x3_ring_large_surface_workload(tmp_path, pm):
+    _announce(f"bf16x3 ring A12 N24 P128 pm={pm}")
+    import torch
+    from pibronic_amd.precision import precision_report
+    cfg = synthetic.WORKLOADS["A12_N24_P128_T250"]
+    FS = stage_dict_models(tmp_path, synthetic.generate(12, 24, quadratic=True, off=cfg["off"], seed=0))
+    B = 512
+    data = make_data(FS, cfg["T"], cfg["P"], B, B, pm=pm)
+    q = torch.empty((B, 24, cfg["P"]), dtype=torch.float64, device="cuda")
+    data.context().sample_block(3, 0, B, q)
+    _check_bf(precision_report(data, q, B, variant="bf16x3"))
+
+
+@pytest.mark.parametrize("A", [2, 5, 16])
+def test_bf16x3_ring_surface_counts(tmp_path, A):
+    _announce(f"bf16x3 ring A={A}")
+    import torch
+    from pibronic_amd.precision import precision_report
+    FS = stage_dict_models(tmp_path, random_model(A, 3, seed=A, quad="full"))
+    B = 256
+    data = make_data(FS, 300.0, 20, B, B, pm=True)
+    q = torch.empty((B, 3, 20), dtype=torch.float64, device="cuda")
+    data.context().sample_block(4, 0, B, q)
+    _check_bf(precision_report(data, q, B, variant="bf16x3"))
+
+
+def test_bf16x3_ring_diagonal_model(tmp_path):
+    """V = 0: M = I exactly, so g == rho up to the split representation of the ring state
+    (hi + lo bf16 parts carry ~17 bits: ~1e-5 relative over a few beads)."""
+    _announce("bf16x3 ring g == rho")
+    m = synthetic.generate(10, 3, quadratic=False, seed=9)
+    m[VMK.E] = np.diag(np.diag(m[VMK.E]))
+    m[VMK.G1] = np.stack([np.diag(np.diag(m[VMK.G1][j])) for j in range(3)])
+    FS = stage_dict_models(tmp_path, m)
+    data = make_data(FS, 300.0, 8, 64, 64)
+    data.ring_bf16x3 = True
+    q, _ = data.draw_host_sample(np.random.default_rng(4), 64)
+    r = pimc.evaluate_samples(data, q, pm=False)
+    assert rel_err(r["s_g"], r["s_rho"]) < 1e-4
