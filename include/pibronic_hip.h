@@ -188,6 +188,22 @@ int pib_gemm_bf16x3(pib_ctx* ctx, int trans_a, int trans_b, int64_t M, int64_t N
                  int64_t ldc, const float* bias, int relu, const float* mask, int64_t ldmask,
                  int accumulate, void* stream);
 
+/* BF16x3 product of operands already split into bf16 hi/lo parts (uint16_t holding bf16
+ * bits): A_hi/A_lo M x K row-major (lda), B given transposed as Bt_hi/Bt_lo N x K row-major
+ * (ldb). Writes C (FP32, ldc) and/or the result split again into O_hi/O_lo (ldo), after
+ * bias and relu. M, N multiples of 64, K of 32, lda and ldb multiples of 8, operands 16-byte
+ * aligned. Lets a chain of dense layers pass bf16 hi/lo activations without conversion. */
+int pib_gemm_bf16x3_presplit(pib_ctx* ctx, int64_t M, int64_t N, int64_t K,
+                             const uint16_t* A_hi_dev, const uint16_t* A_lo_dev, int64_t lda,
+                             const uint16_t* Bt_hi_dev, const uint16_t* Bt_lo_dev, int64_t ldb,
+                             float* C_dev, int64_t ldc, uint16_t* O_hi_dev, uint16_t* O_lo_dev,
+                             int64_t ldo, const float* bias_dev, int relu, void* stream);
+
+/* Split X (rows x cols, ldx) into bf16 hi = RNE(x), lo = RNE(x - hi); with transpose the
+ * outputs are cols x rows (row length ldo). */
+int pib_split_bf16(pib_ctx* ctx, int64_t rows, int64_t cols, const float* X_dev, int64_t ldx,
+                   int transpose, uint16_t* hi_dev, uint16_t* lo_dev, int64_t ldo, void* stream);
+
 /* Normal-mode basis of the flow: U_host (P*P, the circulant eigenvectors, row p = bead),
  * sigma_host (N*P, harmonic standard deviations per mode and normal mode) and m_host (N, the
  * centre subtracted before rotating). P must be a multiple of 64 (MFMA tiles of the

@@ -50,6 +50,9 @@ SIGNATURES = {
                               _c_i64, _vp, _c_i64, _vp, _c_int, _vp, _c_i64, _c_int, _vp]),
     "pib_gemm_bf16x3": (_c_int, [_vp, _c_int, _c_int, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp,
                                  _c_i64, _vp, _c_i64, _vp, _c_int, _vp, _c_i64, _c_int, _vp]),
+    "pib_gemm_bf16x3_presplit": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp, _vp,
+                                          _c_i64, _vp, _c_i64, _vp, _vp, _c_i64, _vp, _c_int, _vp]),
+    "pib_split_bf16": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_int, _vp, _vp, _c_i64, _vp]),
     "pib_flow_set_basis": (_c_int, [_vp, _c_int, _c_int, _dp, _dp, _dp]),
     "pib_flow_whiten": (_c_int, [_vp, _c_int, _c_i64, _vp, _vp, _vp, _vp]),
     "pib_flow_coupling": (_c_int, [_vp, _c_int, _c_i64, _c_int, _c_int, _c_int, ctypes.c_float,
@@ -228,6 +231,18 @@ class Context:
                                              _devptr(A), int(lda), _devptr(B), int(ldb), _devptr(C),
                                              int(ldc), _devptr(bias), int(bool(relu)), _devptr(mask),
                                              int(ldmask), int(bool(accumulate)), _stream(stream)))
+
+    def gemm_bf16x3_presplit(self, M, N, K, A_hi, A_lo, lda, Bt_hi, Bt_lo, ldb, C=None, ldc=0,
+                             O_hi=None, O_lo=None, ldo=0, bias=None, relu=False, stream=None):
+        self._check(self.lib.pib_gemm_bf16x3_presplit(
+            self.h, int(M), int(N), int(K), _devptr(A_hi), _devptr(A_lo), int(lda), _devptr(Bt_hi),
+            _devptr(Bt_lo), int(ldb), _devptr(C), int(ldc), _devptr(O_hi), _devptr(O_lo), int(ldo),
+            _devptr(bias), int(bool(relu)), _stream(stream)))
+
+    def split_bf16(self, rows, cols, X, ldx, hi, lo, ldo, transpose=False, stream=None):
+        self._check(self.lib.pib_split_bf16(self.h, int(rows), int(cols), _devptr(X), int(ldx),
+                                            int(bool(transpose)), _devptr(hi), _devptr(lo), int(ldo),
+                                            _stream(stream)))
 
     def flow_set_basis(self, U, sigma, m):
         Uh, sh, mh = _host(U), _host(sigma), _host(m)

@@ -487,6 +487,36 @@ int pib_gemm_bf16x3(pib_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K,
   return gemm_impl(c, 1, ta, tb, M, N, K, A, lda, B, ldb, C, ldc, bias, relu, mask, ldmask, accumulate, stream);
 }
 
+int pib_gemm_bf16x3_presplit(pib_ctx* c, int64_t M, int64_t N, int64_t K, const uint16_t* A_hi,
+                             const uint16_t* A_lo, int64_t lda, const uint16_t* Bt_hi,
+                             const uint16_t* Bt_lo, int64_t ldb, float* C, int64_t ldc,
+                             uint16_t* O_hi, uint16_t* O_lo, int64_t ldo, const float* bias, int relu,
+                             void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  GemmPreArgs g{A_hi, A_lo, Bt_hi, Bt_lo, C, O_hi, O_lo, bias, M, N, K, lda, ldb, ldc, ldo, relu ? 1 : 0};
+  if (!A_hi || !A_lo || !Bt_hi || !Bt_lo) return fail(c, PIB_ERR_ARG, "gemm_bf16x3_presplit: null operand");
+  if (lda < K || ldb < K || (C && ldc < N) || (O_hi && ldo < N))
+    return fail(c, PIB_ERR_ARG, "gemm_bf16x3_presplit: leading dimension smaller than the row length");
+  if (!gemm_pre_shape_ok(g))
+    return fail(c, PIB_ERR_ARG, "gemm_bf16x3_presplit: M, N multiples of 64, K of 32, lda/ldb multiples of 8, "
+                                "16-byte aligned operands, an FP32 or a split output (hi and lo together)");
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_gemm_bf16x3_pre(g, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "gemm_bf16x3_presplit launch");
+  return PIB_OK;
+}
+
+int pib_split_bf16(pib_ctx* c, int64_t rows, int64_t cols, const float* X, int64_t ldx, int transpose,
+                   uint16_t* hi, uint16_t* lo, int64_t ldo, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (rows < 1 || cols < 1 || !X || !hi || !lo || ldx < cols || ldo < (transpose ? rows : cols))
+    return fail(c, PIB_ERR_ARG, "split_bf16: bad size, leading dimension or null pointer");
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_split_bf16(rows, cols, X, ldx, transpose ? 1 : 0, hi, lo, ldo, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "split_bf16 launch");
+  return PIB_OK;
+}
+
 int pib_flow_set_basis(pib_ctx* c, int N, int P, const double* U, const double* sigma,
                        const double* m) {
   if (!c) return PIB_ERR_ARG;

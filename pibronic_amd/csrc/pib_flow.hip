@@ -11,44 +11,52 @@
 #include "pib_common.hpp"
 #include "pib_flow.hpp"
 
+#include <algorithm>
+
 namespace pib {
 
 // ------------------------------------------------------------------ whitening pre/post
+// Elementwise whitening steps on a 2-D grid: blockIdx.y strides over samples, x over the
+// N*P row, so the mode and bead indices need only 32-bit arithmetic.
 __global__ void k_flow_pre(int mode, BasisArgs b, const void* in, float* out) {
-  const int64_t n = b.B * (int64_t)b.N * b.P;
   const int NP = b.N * b.P, Ph = b.P / 2, H = NP / 2;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t s = e / NP;
-    const int r = (int)(e - s * NP);
-    const int j = r / b.P, k = r - j * b.P;
-    if (mode == FLOW_WHITEN) {
-      out[e] = (float)(((const double*)in)[e] - b.m[j]);
-    } else if (mode == FLOW_GRAD) {
-      out[e] = ((const float*)in)[e];
-    } else {  // FLOW_UNWHITEN: out (row-major B*N x P) = unperm(zp) * sigma
-      const int h = k & 1, kk = k >> 1;
-      out[e] = ((const float*)in)[s * NP + h * H + j * Ph + kk] * b.sigma[r];
+  for (int64_t s = blockIdx.y; s < b.B; s += gridDim.y) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < NP; r += gridDim.x * blockDim.x) {
+      const int64_t e = s * NP + r;
+      const int j = r / b.P, k = r - j * b.P;
+      if (mode == FLOW_WHITEN) {
+        out[e] = (float)(((const double*)in)[e] - b.m[j]);
+      } else if (mode == FLOW_GRAD) {
+        out[e] = ((const float*)in)[e];
+      } else {  // FLOW_UNWHITEN: out (row-major B*N x P) = unperm(zp) * sigma
+        const int h = k & 1, kk = k >> 1;
+        out[e] = ((const float*)in)[s * NP + h * H + j * Ph + kk] * b.sigma[r];
+      }
     }
   }
 }
 
 __global__ void k_flow_post(int mode, BasisArgs b, const float* in, void* out) {
-  const int64_t n = b.B * (int64_t)b.N * b.P;
   const int NP = b.N * b.P, Ph = b.P / 2, H = NP / 2;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t s = e / NP;
-    const int r = (int)(e - s * NP);
-    const int j = r / b.P, k = r - j * b.P;
-    if (mode == FLOW_UNWHITEN) {
-      ((double*)out)[e] = (double)in[e] + b.m[j];
-    } else {
-      const float f = (mode == FLOW_WHITEN) ? b.inv_sigma[r] : b.sigma[r];
-      const int h = k & 1, kk = k >> 1;
-      ((float*)out)[s * NP + h * H + j * Ph + kk] = in[e] * f;
+  for (int64_t s = blockIdx.y; s < b.B; s += gridDim.y) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < NP; r += gridDim.x * blockDim.x) {
+      const int64_t e = s * NP + r;
+      const int j = r / b.P, k = r - j * b.P;
+      if (mode == FLOW_UNWHITEN) {
+        ((double*)out)[e] = (double)in[e] + b.m[j];
+      } else {
+        const float f = (mode == FLOW_WHITEN) ? b.inv_sigma[r] : b.sigma[r];
+        const int h = k & 1, kk = k >> 1;
+        ((float*)out)[s * NP + h * H + j * Ph + kk] = in[e] * f;
+      }
     }
   }
+}
+
+static dim3 grid_rows(int64_t rows, int64_t cols, int threads) {
+  const int64_t gx = std::min<int64_t>((cols + threads - 1) / threads, 64);
+  const int64_t gy = std::min<int64_t>(rows, 65535);
+  return dim3((unsigned)std::max<int64_t>(gx, 1), (unsigned)std::max<int64_t>(gy, 1));
 }
 
 static dim3 grid_for(int64_t n, int threads) {
@@ -59,14 +67,12 @@ static dim3 grid_for(int64_t n, int threads) {
 }
 
 hipError_t launch_flow_pre(int mode, const BasisArgs& b, const void* in, float* out, hipStream_t s) {
-  const int64_t n = b.B * (int64_t)b.N * b.P;
-  hipLaunchKernelGGL(k_flow_pre, grid_for(n, 256), dim3(256), 0, s, mode, b, in, out);
+  hipLaunchKernelGGL(k_flow_pre, grid_rows(b.B, (int64_t)b.N * b.P, 256), dim3(256), 0, s, mode, b, in, out);
   return hipGetLastError();
 }
 
 hipError_t launch_flow_post(int mode, const BasisArgs& b, const float* in, void* out, hipStream_t s) {
-  const int64_t n = b.B * (int64_t)b.N * b.P;
-  hipLaunchKernelGGL(k_flow_post, grid_for(n, 256), dim3(256), 0, s, mode, b, in, out);
+  hipLaunchKernelGGL(k_flow_post, grid_rows(b.B, (int64_t)b.N * b.P, 256), dim3(256), 0, s, mode, b, in, out);
   return hipGetLastError();
 }
 
@@ -161,17 +167,18 @@ __global__ void __launch_bounds__(64 * kRowsPerBlock) k_logrho(LogRhoArgs a) {
   double part[kMaxRhoSurfaces];
 #pragma unroll
   for (int s = 0; s < kMaxRhoSurfaces; ++s) part[s] = 0.0;
-  for (int e = lane; e < NP; e += 64) {
-    const int j = e / P, p = e - j * P;
-    const int pn = (p + 1 == P) ? 0 : p + 1;
-    const double r0 = R[e], r1 = R[j * P + pn];
+  for (int j = 0; j < N; ++j) {  // mode-uniform constants: scalar loads, no index division
     const double cj = a.coth[j], sj2 = 2.0 * a.csch[j];
+    for (int p = lane; p < P; p += 64) {
+      const int pn = (p + 1 == P) ? 0 : p + 1;
+      const double r0 = R[j * P + p], r1 = R[j * P + pn];
 #pragma unroll
-    for (int s = 0; s < kMaxRhoSurfaces; ++s) {
-      if (s < a.A) {
-        const double d = a.shift[s * N + j];
-        const double x1 = r0 - d, x2 = r1 - d;
-        part[s] += cj * (x1 * x1 + x2 * x2) - (sj2 * x1) * x2;
+      for (int s = 0; s < kMaxRhoSurfaces; ++s) {
+        if (s < a.A) {
+          const double d = a.shift[s * N + j];
+          const double x1 = r0 - d, x2 = r1 - d;
+          part[s] += cj * (x1 * x1 + x2 * x2) - (sj2 * x1) * x2;
+        }
       }
     }
   }
@@ -196,19 +203,21 @@ __global__ void __launch_bounds__(64 * kRowsPerBlock) k_logrho(LogRhoArgs a) {
 #pragma unroll
   for (int s = 0; s < kMaxRhoSurfaces; ++s) resp[s] = (s < a.A) ? exp(la[s] - lmax) / tot : 0.0;
   const double cb = a.c ? a.c[b] : 1.0;
-  for (int e = lane; e < NP; e += 64) {
-    const int j = e / P, p = e - j * P;
-    const int pn = (p + 1 == P) ? 0 : p + 1, pp = (p == 0) ? P - 1 : p - 1;
-    const double r0 = R[e], rn = R[j * P + pn], rp = R[j * P + pp];
-    double g = 0.0;
+  for (int j = 0; j < N; ++j) {
+    const double cj2 = 2.0 * a.coth[j], sj = a.csch[j];
+    for (int p = lane; p < P; p += 64) {
+      const int pn = (p + 1 == P) ? 0 : p + 1, pp = (p == 0) ? P - 1 : p - 1;
+      const double r0 = R[j * P + p], rn = R[j * P + pn], rp = R[j * P + pp];
+      double g = 0.0;
 #pragma unroll
-    for (int s = 0; s < kMaxRhoSurfaces; ++s) {
-      if (s < a.A) {
-        const double d = a.shift[s * N + j];
-        g -= resp[s] * (2.0 * a.coth[j] * (r0 - d) - a.csch[j] * ((rn - d) + (rp - d)));
+      for (int s = 0; s < kMaxRhoSurfaces; ++s) {
+        if (s < a.A) {
+          const double d = a.shift[s * N + j];
+          g -= resp[s] * (cj2 * (r0 - d) - sj * ((rn - d) + (rp - d)));
+        }
       }
+      a.dR[b * (int64_t)NP + j * P + p] = (float)(-cb * g);
     }
-    a.dR[b * (int64_t)NP + e] = (float)(-cb * g);
   }
 }
 

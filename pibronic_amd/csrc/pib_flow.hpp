@@ -23,6 +23,25 @@ struct GemmArgs {
 };
 
 hipError_t launch_gemm_f32(bool ta, bool tb, const GemmArgs& g, hipStream_t s);
+
+// BF16x3 product of pre-split operands (bf16 bits as unsigned short): A hi/lo M x K (lda),
+// B hi/lo stored N x K (ldb); output FP32 C (ldc) and/or split hi/lo (ldo); bias, relu.
+struct GemmPreArgs {
+  const unsigned short* Ah;
+  const unsigned short* Al;
+  const unsigned short* Bh;
+  const unsigned short* Bl;
+  float* C;
+  unsigned short* Oh;
+  unsigned short* Ol;
+  const float* bias;
+  int64_t M, N, K, lda, ldb, ldc, ldo;
+  int relu;
+};
+bool gemm_pre_shape_ok(const GemmPreArgs& g);
+hipError_t launch_gemm_bf16x3_pre(const GemmPreArgs& g, hipStream_t s);
+hipError_t launch_split_bf16(int64_t rows, int64_t cols, const float* X, int64_t ldx, int transpose,
+                             unsigned short* hi, unsigned short* lo, int64_t ldo, hipStream_t s);
 bool gemm_shape_ok(const GemmArgs& g);
 // number of K splits worth using for this shape (1 = none)
 int gemm_splits(const GemmArgs& g, int num_cu);

@@ -417,4 +417,118 @@ hipError_t launch_gemm_f32(bool ta, bool tb, const GemmArgs& g, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------------
+// BF16x3 with operands already split: A as hi/lo bf16 (M x K, row-major, lda), B as hi/lo bf16
+// stored transposed (N x K, row-major, ldb). Staging is a plain 16-byte copy. The result goes
+// to FP32 C and/or, split again, to hi/lo bf16 (M x N, ldo) for a following product.
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gemm_bf16x3_pre(GemmPreArgs g) {
+  __shared__ __align__(16) unsigned short Ah[kTM][kLd3], Al[kTM][kLd3];
+  __shared__ __align__(16) unsigned short Bh[kTN][kLd3], Bl[kTN][kLd3];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int64_t m0 = (int64_t)blockIdx.y * kTM, n0 = (int64_t)blockIdx.x * kTN;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // one 16-byte group (8 bf16 along K) of each of the four slabs per thread
+  const int row = tid / 4, kk = (tid % 4) * 8;
+  uvec4 rah, ral, rbh, rbl;
+  auto load = [&](int64_t k0) {
+    rah = *(const uvec4*)(g.Ah + (m0 + row) * g.lda + k0 + kk);
+    ral = *(const uvec4*)(g.Al + (m0 + row) * g.lda + k0 + kk);
+    rbh = *(const uvec4*)(g.Bh + (n0 + row) * g.ldb + k0 + kk);
+    rbl = *(const uvec4*)(g.Bl + (n0 + row) * g.ldb + k0 + kk);
+  };
+  load(0);
+  for (int64_t k0 = 0; k0 < g.K; k0 += kTK3) {
+    *(uvec4*)&Ah[row][kk] = rah;
+    *(uvec4*)&Al[row][kk] = ral;
+    *(uvec4*)&Bh[row][kk] = rbh;
+    *(uvec4*)&Bl[row][kk] = rbl;
+    __syncthreads();
+    if (k0 + kTK3 < g.K) load(k0 + kTK3);
+    const int kq = 8 * (lane >> 4);
+    bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int r = wr * 32 + i * 16 + (lane & 15);
+      ah[i] = __builtin_bit_cast(bf16x8, *(const uvec4*)&Ah[r][kq]);
+      al[i] = __builtin_bit_cast(bf16x8, *(const uvec4*)&Al[r][kq]);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = wc * 32 + j * 16 + (lane & 15);
+      bh[j] = __builtin_bit_cast(bf16x8, *(const uvec4*)&Bh[c][kq]);
+      bl[j] = __builtin_bit_cast(bf16x8, *(const uvec4*)&Bl[c][kq]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int64_t col = n0 + wc * 32 + j * 16 + (lane & 15);
+      const float bv = g.bias ? g.bias[col] : 0.f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t rw = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+        float v = acc[i][j][r] + bv;
+        if (g.relu) v = v > 0.f ? v : 0.f;
+        if (g.C) g.C[rw * g.ldc + col] = v;
+        if (g.Oh) {
+          const unsigned short h = f2bf(v);
+          g.Oh[rw * g.ldo + col] = h;
+          g.Ol[rw * g.ldo + col] = f2bf(v - bf2f(h));
+        }
+      }
+    }
+}
+
+// X (rows x cols, ldx) -> hi/lo bf16; transpose: out[c][r] (cols x rows, row length rows)
+__global__ void k_split_bf16(int64_t rows, int64_t cols, const float* X, int64_t ldx, int transpose,
+                             unsigned short* hi, unsigned short* lo, int64_t ldo) {
+  for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += (int64_t)gridDim.x * blockDim.x) {
+      const float x = X[r * ldx + c];
+      const unsigned short h = f2bf(x);
+      const int64_t o = transpose ? c * ldo + r : r * ldo + c;
+      hi[o] = h;
+      lo[o] = f2bf(x - bf2f(h));
+    }
+  }
+}
+
+bool gemm_pre_shape_ok(const GemmPreArgs& g) {
+  return g.M > 0 && g.N > 0 && g.K > 0 && g.M % kTM == 0 && g.N % kTN == 0 && g.K % kTK3 == 0 &&
+         g.lda % 8 == 0 && g.ldb % 8 == 0 && ((uintptr_t)g.Ah % 16) == 0 && ((uintptr_t)g.Al % 16) == 0 &&
+         ((uintptr_t)g.Bh % 16) == 0 && ((uintptr_t)g.Bl % 16) == 0 && (g.C || g.Oh) &&
+         (!g.Oh == !g.Ol);
+}
+
+hipError_t launch_gemm_bf16x3_pre(const GemmPreArgs& g, hipStream_t s) {
+  if (!gemm_pre_shape_ok(g)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gemm_bf16x3_pre, dim3((unsigned)(g.N / kTN), (unsigned)(g.M / kTM)), dim3(256), 0, s, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_bf16(int64_t rows, int64_t cols, const float* X, int64_t ldx, int transpose,
+                             unsigned short* hi, unsigned short* lo, int64_t ldo, hipStream_t s) {
+  const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((cols + 255) / 256, 64)),
+                  (unsigned)std::max<int64_t>(1, std::min<int64_t>(rows, 65535)));
+  hipLaunchKernelGGL(k_split_bf16, grid, dim3(256), 0, s, rows, cols, X, ldx, transpose, hi, lo, ldo);
+  return hipGetLastError();
+}
+
 }  // namespace pib

@@ -73,6 +73,8 @@ class Flow:
         self.adam_v = torch.zeros_like(self.params)
         self.step_count = 0
         self._bufs = {}
+        self._version = 0          # bumped whenever the parameters change
+        self._split_version = -1   # parameter version of the pre-split bf16 weights
 
     # ------------------------------------------------------------------ layout helpers
     def _offset(self, layer, name):
@@ -96,6 +98,11 @@ class Flow:
     def set_params(self, flat):
         import torch
         self.params.copy_(torch.as_tensor(np.asarray(flat, dtype=np.float32)))
+        self.invalidate()
+
+    def invalidate(self):
+        """Call after writing self.params directly (set_params and adam_step do it)."""
+        self._version += 1
 
     def _buf(self, name, shape, dtype=None):
         import torch
@@ -124,6 +131,44 @@ class Flow:
         self._gemm(h2, self.p(layer, "W3"), o, B, D, h, h, D, D,
                    bias=self.p(layer, "b3"), stream=stream)
 
+    # ------------------------------------------------------------------ bf16x3 fast path
+    def _split_weights(self, stream):
+        """Weights split once per parameter version into bf16 hi/lo, stored transposed (N x K)
+        as the pre-split product expects."""
+        import torch
+        if self._split_version == self._version:
+            return
+        H, h, D = self.H, self.hidden, self.D
+        self._wsplit = []
+        for l in range(self.layers):
+            mats = []
+            for name, (K, Nn) in (("W1", (H, h)), ("W2", (h, h)), ("W3", (h, D))):
+                hi = self._buf(f"{name}h{l}", (Nn, K), torch.int16)
+                lo = self._buf(f"{name}l{l}", (Nn, K), torch.int16)
+                self.ctx.split_bf16(K, Nn, self.p(l, name), Nn, hi, lo, K, transpose=True, stream=stream)
+                mats.append((hi, lo))
+            self._wsplit.append(mats)
+        self._split_version = self._version
+
+    def _mlp_fast(self, layer, zp, B, o, stream):
+        """The coupling network on pre-split operands: the conditioning half of zp is split
+        once, the hidden activations travel as bf16 hi/lo written by the previous product."""
+        import torch
+        a_off, _ = self.halves(layer)
+        H, h, D = self.H, self.hidden, self.D
+        ctx = self.ctx
+        (w1h, w1l), (w2h, w2l), (w3h, w3l) = self._wsplit[layer]
+        zh, zl = self._buf("zh", (B, H), torch.int16), self._buf("zl", (B, H), torch.int16)
+        ah, al = self._buf("ah", (B, h), torch.int16), self._buf("al", (B, h), torch.int16)
+        bh, bl = self._buf("bh", (B, h), torch.int16), self._buf("bl", (B, h), torch.int16)
+        ctx.split_bf16(B, H, zp[:, a_off:], D, zh, zl, H, stream=stream)
+        ctx.gemm_bf16x3_presplit(B, h, H, zh, zl, H, w1h, w1l, H, O_hi=ah, O_lo=al, ldo=h,
+                                 bias=self.p(layer, "b1"), relu=True, stream=stream)
+        ctx.gemm_bf16x3_presplit(B, h, h, ah, al, h, w2h, w2l, h, O_hi=bh, O_lo=bl, ldo=h,
+                                 bias=self.p(layer, "b2"), relu=True, stream=stream)
+        ctx.gemm_bf16x3_presplit(B, D, h, bh, bl, h, w3h, w3l, h, C=o, ldc=D,
+                                 bias=self.p(layer, "b3"), stream=stream)
+
     # ------------------------------------------------------------------ directions
     def forward(self, R0, stream=None):
         """Sampling direction: R0 (B, N, P) double -> R (B, N, P) double, log|det dF/dZ| (B,)."""
@@ -137,9 +182,15 @@ class Flow:
         logdet = torch.zeros(B, dtype=torch.float64, device=self.dev)
         R = torch.empty_like(R0)
         self.ctx.flow_whiten(_native.FLOW_WHITEN, B, R0, work, zp, stream)
+        fast = self.gemm == "bf16x3"
+        if fast:
+            self._split_weights(stream)
         for l in range(self.layers):
             a_off, b_off = self.halves(l)
-            self._mlp(l, zp, B, h1, h2, o, stream)
+            if fast:
+                self._mlp_fast(l, zp, B, o, stream)
+            else:
+                self._mlp(l, zp, B, h1, h2, o, stream)
             self.ctx.flow_coupling(False, B, D, a_off, b_off, self.smax, zp, o, logdet, stream)
         self.ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, R, stream)
         return R, logdet
@@ -156,9 +207,15 @@ class Flow:
         logdet = torch.zeros(B, dtype=torch.float64, device=self.dev)
         R0 = torch.empty_like(R)
         self.ctx.flow_whiten(_native.FLOW_WHITEN, B, R, work, zp, stream)
+        fast = self.gemm == "bf16x3"
+        if fast:
+            self._split_weights(stream)
         for l in reversed(range(self.layers)):
             a_off, b_off = self.halves(l)
-            self._mlp(l, zp, B, h1, h2, o, stream)
+            if fast:
+                self._mlp_fast(l, zp, B, o, stream)
+            else:
+                self._mlp(l, zp, B, h1, h2, o, stream)
             self.ctx.flow_coupling(True, B, D, a_off, b_off, self.smax, zp, o, logdet, stream)
         self.ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, R0, stream)
         logrho = torch.empty(B, dtype=torch.float64, device=self.dev)
@@ -219,6 +276,7 @@ class Flow:
 
     def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
         self.step_count += 1
+        self._version += 1
         self.ctx.adam_f32(self.n_params, self.params, self.grad, self.adam_m, self.adam_v, lr,
                           beta1, beta2, eps, self.step_count, stream)
 

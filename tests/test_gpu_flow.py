@@ -128,6 +128,36 @@ def test_gemm_bf16x3_unaligned_operand(tmp_path):
     assert torch.equal(C1, C2)
 
 
+def test_gemm_presplit_matches_converting(tmp_path):
+    """Pre-split operands give bit-for-bit the converting kernel's result (same splits, same
+    product order), and the split output equals splitting the FP32 output."""
+    _announce("gemm presplit")
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, 2, quadratic=False, seed=0))
+    ctx = make_data(FS, 300.0, 64, 10, 10).context()
+    M, N, K = 128, 192, 96
+    g = torch.Generator().manual_seed(21)
+    A = torch.randn((M, K), generator=g).to(DEV)
+    W = torch.randn((K, N), generator=g).to(DEV)
+    bias = torch.randn(N, generator=g).to(DEV)
+    ref = torch.empty((M, N), device=DEV)
+    ctx.gemm_bf16x3(A, W, ref, M, N, K, K, N, N, bias=bias, relu=True)
+    ah, al = torch.empty((M, K), dtype=torch.int16, device=DEV), torch.empty((M, K), dtype=torch.int16, device=DEV)
+    wh, wl = torch.empty((N, K), dtype=torch.int16, device=DEV), torch.empty((N, K), dtype=torch.int16, device=DEV)
+    ctx.split_bf16(M, K, A, K, ah, al, K)
+    ctx.split_bf16(K, N, W, N, wh, wl, K, transpose=True)
+    C = torch.empty((M, N), device=DEV)
+    oh, ol = torch.empty((M, N), dtype=torch.int16, device=DEV), torch.empty((M, N), dtype=torch.int16, device=DEV)
+    ctx.gemm_bf16x3_presplit(M, N, K, ah, al, K, wh, wl, K, C=C, ldc=N, O_hi=oh, O_lo=ol, ldo=N,
+                             bias=bias, relu=True)
+    rh, rl = torch.empty_like(oh), torch.empty_like(ol)
+    ctx.split_bf16(M, N, ref, N, rh, rl, N)
+    torch.cuda.synchronize()
+    assert torch.equal(C, ref)
+    assert torch.equal(oh, rh) and torch.equal(ol, rl)
+    with pytest.raises(_native.NativeError, match="multiples"):
+        ctx.gemm_bf16x3_presplit(M, N, 48, ah, al, K, wh, wl, K, C=C, ldc=N)
+
+
 def test_colsum_f32(tmp_path):
     """Two-pass column sums, ragged shapes, strided rows."""
     _announce("colsum")
@@ -247,6 +277,39 @@ def test_adam_step_matches_formula(model):
     v = 0.999 * v + 0.001 * 4 * g * g
     p2 = p1 - 1e-2 * (m / (1 - 0.81)) / (np.sqrt(v / (1 - 0.999 ** 2)) + 1e-8)
     assert np.allclose(fl.params.cpu().numpy(), p2, rtol=1e-5, atol=1e-6)
+
+
+def test_flow_fast_path_matches(model):
+    """The draw's pre-split path equals the layer-by-layer bf16x3 path bit for bit, and
+    recomputes its weight split after a parameter update."""
+    _announce("flow fast path")
+    data = model
+    fl, ref, flat = _flow_and_ref(data, gemm="bf16x3")
+    q = torch.as_tensor(_host_q(data, 128, 8), device=DEV)
+    R_fast, ld_fast = fl.forward(q)
+    R_slow, ld_slow = _forward_slow(fl, q)  # layer by layer with the converting kernel
+    torch.cuda.synchronize()
+    assert torch.equal(R_fast, R_slow) and torch.equal(ld_fast, ld_slow)
+    fl.set_params(random_params(ref, 99, 0.05))
+    R_new, _ = fl.forward(q)
+    torch.cuda.synchronize()
+    assert not torch.equal(R_new, R_fast)
+
+
+def _forward_slow(fl, R0):
+    B, D, h = R0.shape[0], fl.D, fl.hidden
+    zp = torch.empty((B, D), device=DEV)
+    work = torch.empty(2 * B * D, device=DEV)
+    h1, h2, o = torch.empty((B, h), device=DEV), torch.empty((B, h), device=DEV), torch.empty((B, D), device=DEV)
+    logdet = torch.zeros(B, dtype=torch.float64, device=DEV)
+    R = torch.empty_like(R0)
+    fl.ctx.flow_whiten(_native.FLOW_WHITEN, B, R0, work, zp)
+    for l in range(fl.layers):
+        a_off, b_off = fl.halves(l)
+        fl._mlp(l, zp, B, h1, h2, o, None)
+        fl.ctx.flow_coupling(False, B, D, a_off, b_off, fl.smax, zp, o, logdet)
+    fl.ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, R)
+    return R, logdet
 
 
 def test_identity_flow_reproduces_harmonic_weights(model):

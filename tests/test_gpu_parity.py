@@ -232,7 +232,8 @@ def test_counter_based_sampler_is_split_invariant(tmp_path):
 
 def test_statistics_match_oracle_sampling(tmp_path):
     """Independent runs agree statistically: device-sampled GPU run vs host-sampled oracle
-    run of the same model; Z_MC and the basic E within 4 combined standard errors."""
+    run of the same model; Z_MC and the jackknife E within 2 combined standard errors
+    (measured with these seeds: 0.08 and 0.8 sigma)."""
     _announce("statistical agreement A2 N3 P12")
     FS = stage_dict_models(tmp_path, synthetic.generate(2, 3, quadratic=False, seed=0))
     X = 200_000
@@ -255,8 +256,43 @@ def test_statistics_match_oracle_sampling(tmp_path):
     Zg, eg, Eg, sEg = est(res.scaled_rho, res.scaled_g, res.scaled_gofr_plus, res.scaled_gofr_minus)
     Zo, eo, Eo, sEo = est(o["s_rho"], o["s_g"], o["s_gP"], o["s_gM"])
     print(f"Z gpu {Zg:.6f}+-{eg:.2e} oracle {Zo:.6f}+-{eo:.2e}; E gpu {Eg:.6f}+-{sEg:.2e} oracle {Eo:.6f}+-{sEo:.2e}")
-    assert abs(Zg - Zo) < 4 * np.hypot(eg, eo)
-    assert abs(Eg - Eo) < 4 * np.hypot(sEg, sEo)
+    assert abs(Zg - Zo) < 2 * np.hypot(eg, eo)
+    assert abs(Eg - Eo) < 2 * np.hypot(sEg, sEo)
+
+
+def test_estimates_agree_within_jackknife_error(tmp_path):
+    """Z(beta)/Z_rho and the jackknife <E> of block_compute_pm on host-seeded samples agree
+    with the oracle's on the same samples far inside 2 sigma of the jackknife error (the
+    criterion the estimates are judged by); the difference is roundoff only."""
+    _announce("estimates within 2 sigma A4 N8 P64")
+    FS = stage_dict_models(tmp_path, synthetic.generate(4, 8, quadratic=True, seed=0))
+    X, B = 8000, 2000
+    data = make_data(FS, 300.0, 64, X, B, pm=True, seed=31)
+    data.sampler = "host"
+    res = pimc.block_compute_pm(data, pimc.BoxResultPM(data=data))
+    s = oracle_setup(FS, 300.0, 64)
+    # the oracle on exactly the samples the host sampler produced, block by block
+    o = {"s_rho": [], "s_g": [], "s_gP": [], "s_gM": []}
+    for k in range(X // B):
+        qk, _ = data.draw_host_sample(np.random.default_rng([31, k * B]), B)  # draw_sample's host stream
+        ok = po.evaluate(s, qk, pm=True)
+        for key in o:
+            o[key].append(ok[key])
+    o = {k: np.concatenate(v) for k, v in o.items()}
+
+    def est(rho, g, gp, gm):
+        r, d1, d2 = so.basic_terms(po.DELTA_BETA, rho, g, gp, gm)
+        n = len(r)
+        jk = [so.jackknife_term(n, t) for t in (r, d1, d2)]
+        basic = so.estimate_basic(n, 300.0, r, d1, d2)
+        j = so.estimate_jackknife(n, 300.0, po.DELTA_BETA, basic, *jk)
+        return r.mean(), r.std() / np.sqrt(n - 1), j["E"], j["E error"]
+
+    Zg, eg, Eg, sEg = est(res.scaled_rho, res.scaled_g, res.scaled_gofr_plus, res.scaled_gofr_minus)
+    Zo, eo, Eo, sEo = est(o["s_rho"], o["s_g"], o["s_gP"], o["s_gM"])
+    print(f"Z gpu {Zg:.10e}+-{eg:.2e} oracle {Zo:.10e}; E gpu {Eg:.10e}+-{sEg:.2e} oracle {Eo:.10e}")
+    assert abs(Zg - Zo) < 2 * eo and abs(Eg - Eo) < 2 * sEo
+    assert abs(Zg - Zo) <= 1e-9 * abs(Zo) and abs(Eg - Eo) <= 1e-8 * max(abs(Eo), sEo)
 
 
 def test_errors_are_reported(tmp_path):
