@@ -136,6 +136,7 @@ def learned_leg(data, args, log):
     t0 = time.perf_counter()
     fl = flow.Flow(data, layers=args.flow_layers, hidden=args.flow_hidden, seed=args.seed,
                    gemm=args.flow_gemm)
+    fl.use_fused = bool(args.flow_fused)
     hist = learn.train(fl, args.train_steps, args.train_batch, args.lr, args.seed + 1,
                        log=lambda st: log(f"[bench] train step {st['step']} ess {st['ess_frac']:.3e}"),
                        log_every=max(1, args.train_steps // 5))
@@ -148,23 +149,35 @@ def learned_leg(data, args, log):
     ctx = data.context()
     stream = torch.cuda.current_stream()
 
-    def step(i):
+    e0 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    e1 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+
+    def step(i, timed):
+        if timed:
+            e0[i - W].record(stream)
         R, logq = fl.draw(B, args.seed + 2, (1 << 40) + i * B, stream)
+        if timed:
+            e1[i - W].record(stream)
         data.evaluate_device(R, B, False, out, B, stream=stream)
         ctx.block_sums_q(out, B, B, False, 1.0, logq, sums[i], stream)
 
     for i in range(W):
-        step(i)
+        step(i, False)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for i in range(W, W + K):
-        step(i)
+        step(i, True)
     torch.cuda.synchronize()
     el = time.perf_counter() - t1
+    draw_ms = sum(a.elapsed_time(b) for a, b in zip(e0, e1)) / K
+    H, hdn, D = fl.H, fl.hidden, fl.D
+    mlp_flops = 2.0 * B * fl.layers * (H * hdn + hdn * hdn + hdn * D)
     res = weight_summary(sums[W:].sum(dim=0).cpu().numpy(), el)
     res.update(train_seconds=train_s, train_steps=args.train_steps, train_batch=args.train_batch,
                samples_per_step=B, layers=args.flow_layers, hidden=args.flow_hidden,
-               gemm=args.flow_gemm,
+               gemm=args.flow_gemm, flow_draw_ms=draw_ms,
+               flow_mlp_tflops=mlp_flops / (draw_ms * 1e-3) / 1e12,
+               flow_mlp_flops_per_draw=mlp_flops,
                train_final_batch_ess=hist[-1]["ess_frac"] if hist else None)
     return res
 
@@ -295,6 +308,8 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--flow-gemm", default="bf16x3", choices=["f32", "bf16x3"],
                     help="precision of the flow's dense layers (FP32 MFMA or BF16x3 split)")
+    ap.add_argument("--flow-fused", type=int, default=0,
+                    help="1: one launch per coupling layer in the draws (bf16x3, hidden 256)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this many MiB 3 times "

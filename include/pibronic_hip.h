@@ -230,6 +230,19 @@ int pib_flow_coupling_inverse_backward(pib_ctx* ctx, int64_t B, int D, int a_off
                                        float* dz_dev, float* dout_dev, const double* c_dev,
                                        void* stream);
 
+/* One whole coupling layer in one launch: the MLP of the layer (D/2 -> hidden -> hidden -> D,
+ * ReLU) as BF16x3 products on weights already split and transposed (pib_split_bf16 with
+ * transpose: W1 as hidden x D/2, W2 as hidden x hidden, W3 as D x hidden, bf16 hi/lo), then
+ * the coupling of pib_flow_coupling on z (B x D) and logdet. Same arithmetic as the
+ * layer-by-layer calls (pib_gemm_bf16x3_presplit x3 + pib_flow_coupling); the activations and
+ * the MLP output stay on chip. Supported: hidden 256, D/2 in {256, 512}, B multiple of 32. */
+int pib_flow_layer_fused(pib_ctx* ctx, int inverse, int64_t B, int D, int hidden, int a_off,
+                         int b_off, float smax, float* z_dev, double* logdet_dev,
+                         const uint16_t* W1_hi_dev, const uint16_t* W1_lo_dev,
+                         const uint16_t* W2_hi_dev, const uint16_t* W2_lo_dev,
+                         const uint16_t* W3_hi_dev, const uint16_t* W3_lo_dev, const float* b1_dev,
+                         const float* b2_dev, const float* b3_dev, void* stream);
+
 /* log rho(R) of the sampling model (unscaled, double, B) and, if dR_dev is not NULL,
  * dR = -c * d log rho / dR (float, B*N*P; c_dev NULL means c = 1). Uses the PIB_SET_RHO
  * constants and P. Same function as build_o_matrix + build_denominator

@@ -53,6 +53,8 @@ SIGNATURES = {
     "pib_gemm_bf16x3_presplit": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp, _vp,
                                           _c_i64, _vp, _c_i64, _vp, _vp, _c_i64, _vp, _c_int, _vp]),
     "pib_split_bf16": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_int, _vp, _vp, _c_i64, _vp]),
+    "pib_flow_layer_fused": (_c_int, [_vp, _c_int, _c_i64, _c_int, _c_int, _c_int, _c_int, ctypes.c_float,
+                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pib_flow_set_basis": (_c_int, [_vp, _c_int, _c_int, _dp, _dp, _dp]),
     "pib_flow_whiten": (_c_int, [_vp, _c_int, _c_i64, _vp, _vp, _vp, _vp]),
     "pib_flow_coupling": (_c_int, [_vp, _c_int, _c_i64, _c_int, _c_int, _c_int, ctypes.c_float,
@@ -243,6 +245,15 @@ class Context:
         self._check(self.lib.pib_split_bf16(self.h, int(rows), int(cols), _devptr(X), int(ldx),
                                             int(bool(transpose)), _devptr(hi), _devptr(lo), int(ldo),
                                             _stream(stream)))
+
+    def flow_layer_fused(self, inverse, B, D, hidden, a_off, b_off, smax, z, logdet, w, b1, b2, b3,
+                         stream=None):
+        """w = ((W1h, W1l), (W2h, W2l), (W3h, W3l)) pre-split transposed weights."""
+        (w1h, w1l), (w2h, w2l), (w3h, w3l) = w
+        self._check(self.lib.pib_flow_layer_fused(
+            self.h, int(bool(inverse)), int(B), int(D), int(hidden), int(a_off), int(b_off), float(smax),
+            _devptr(z), _devptr(logdet), _devptr(w1h), _devptr(w1l), _devptr(w2h), _devptr(w2l),
+            _devptr(w3h), _devptr(w3l), _devptr(b1), _devptr(b2), _devptr(b3), _stream(stream)))
 
     def flow_set_basis(self, U, sigma, m):
         Uh, sh, mh = _host(U), _host(sigma), _host(m)

@@ -19,7 +19,7 @@ BUILD_DIR = os.path.join(ROOT, "_build")
 LIB_NAME = "libpibronic_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 SOURCES = ["pib_api.hip", "pib_batch.hip", "pib_eval.hip", "pib_eval_wide.hip", "pib_eval_fp32.hip", "pib_sample.hip", "pib_reduce.hip", "pib_gemm.hip",
-           "pib_flow.hip"]
+           "pib_flow.hip", "pib_flow_fused.hip"]
 ARCH = "gfx950"
 
 COMMON_FLAGS = [

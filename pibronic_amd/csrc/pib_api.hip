@@ -517,6 +517,28 @@ int pib_split_bf16(pib_ctx* c, int64_t rows, int64_t cols, const float* X, int64
   return PIB_OK;
 }
 
+int pib_flow_layer_fused(pib_ctx* c, int inverse, int64_t B, int D, int hidden, int a_off, int b_off,
+                         float smax, float* z, double* logdet, const uint16_t* W1_hi,
+                         const uint16_t* W1_lo, const uint16_t* W2_hi, const uint16_t* W2_lo,
+                         const uint16_t* W3_hi, const uint16_t* W3_lo, const float* b1, const float* b2,
+                         const float* b3, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (!z || !logdet || !W1_hi || !W1_lo || !W2_hi || !W2_lo || !W3_hi || !W3_lo || !b1 || !b2 || !b3)
+    return fail(c, PIB_ERR_ARG, "flow_layer_fused: null pointer");
+  if (!fused_layer_supported(B, D, hidden))
+    return fail(c, PIB_ERR_UNSUPPORTED, "flow_layer_fused: needs hidden 256, D/2 in {256, 512}, B a multiple of 32");
+  const int H = D / 2;
+  if (!((a_off == 0 && b_off == H) || (a_off == H && b_off == 0)))
+    return fail(c, PIB_ERR_ARG, "flow_layer_fused: the halves must be (0, D/2) or (D/2, 0)");
+  if (!(smax > 0.f)) return fail(c, PIB_ERR_ARG, "flow_layer_fused: smax must be > 0");
+  FusedLayerArgs a{B, D, hidden, a_off, b_off, inverse ? 1 : 0, smax, z, logdet,
+                   W1_hi, W1_lo, W2_hi, W2_lo, W3_hi, W3_lo, b1, b2, b3};
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_flow_layer(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "flow_layer_fused launch");
+  return PIB_OK;
+}
+
 int pib_flow_set_basis(pib_ctx* c, int N, int P, const double* U, const double* sigma,
                        const double* m) {
   if (!c) return PIB_ERR_ARG;

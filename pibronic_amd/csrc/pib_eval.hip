@@ -33,21 +33,28 @@ __device__ __forceinline__ double& UT(double (&a)[A][A], int i, int j) {
   return i <= j ? a[i][j] : a[j][i];
 }
 
-// 1/x and 1/sqrt(x) from the hardware approximations refined by two Newton steps each
-// (relative error ~1 ulp). Used only inside Jacobi rotations, whose orthogonality depends on
-// c^2 + s^2 = 1 (kept to ~1 ulp), not on t being correctly rounded.
+// 1/x and 1/sqrt(x) from the hardware approximations (relative error ~2^-29) refined by
+// Newton steps, each of which squares the relative error: one step gives ~2^-50, two ~1 ulp.
+// Used only inside Jacobi rotations: the rotation's orthogonality depends on c^2 + s^2 = 1,
+// so c = 1/sqrt(1 + t^2) takes two steps; t (and the square root inside it) only has to be
+// close to the annihilating angle -- an error of 2^-50 leaves a_pq ~2^-50 |a_pq| behind --
+// and takes one.
+template <int STEPS>
 __device__ __forceinline__ double rcp_nr(double x) {
   double y = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-x, y, 1.0);
-  return fma(y, e, y);
+#pragma unroll
+  for (int it = 0; it < STEPS; ++it) {
+    const double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+  }
+  return y;
 }
 
+template <int STEPS>
 __device__ __forceinline__ double rsq_nr(double x) {
   double y = __builtin_amdgcn_rsq(x);
 #pragma unroll
-  for (int it = 0; it < 2; ++it) {
+  for (int it = 0; it < STEPS; ++it) {
     const double e = fma(-(x * y), y, 1.0);  // 1 - x y^2
     y = fma(0.5 * y, e, y);
   }
@@ -66,11 +73,11 @@ __device__ __forceinline__ void jacobi_rotate(double (&a)[A][A], double (&u)[A][
   // safe arguments instead of branches: x == 0 gives r = 0 * rsq(1) = 0, and den == 0
   // implies h == 0, so t = 0 * rcp(1) = 0
   const double x = fma(d, d, h * h);
-  const double r = x * rsq_nr(x > 0.0 ? x : 1.0);
+  const double r = x * rsq_nr<1>(x > 0.0 ? x : 1.0);
   const double den = fabs(d) + r;
-  double t = h * rcp_nr(den > 0.0 ? den : 1.0);
+  double t = h * rcp_nr<1>(den > 0.0 ? den : 1.0);
   t = d < 0.0 ? -t : t;
-  const double c = rsq_nr(fma(t, t, 1.0));
+  const double c = rsq_nr<2>(fma(t, t, 1.0));
   const double s = t * c;
   a[p][p] = fma(-t, apq, a[p][p]);
   a[q][q] = fma(t, apq, a[q][q]);

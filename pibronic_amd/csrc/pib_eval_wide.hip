@@ -43,18 +43,24 @@ struct WideLds {
   double O[3][kBeadsPerChunk][kW];       // scaled O-factors (vib, vib+, vib-)
 };
 
+// hardware 1/x and 1/sqrt(x) (~2^-29) with Newton steps (see rcp_nr in the per-lane kernel):
+// two steps where orthogonality or an output depends on it, one inside the rotation angle
+template <int STEPS = 2>
 __device__ __forceinline__ double rcp_nr_w(double x) {
   double y = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-x, y, 1.0);
-  return fma(y, e, y);
+#pragma unroll
+  for (int it = 0; it < STEPS; ++it) {
+    const double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+  }
+  return y;
 }
 
+template <int STEPS = 2>
 __device__ __forceinline__ double rsq_nr_w(double x) {
   double y = __builtin_amdgcn_rsq(x);
 #pragma unroll
-  for (int it = 0; it < 2; ++it) {
+  for (int it = 0; it < STEPS; ++it) {
     const double e = fma(-(x * y), y, 1.0);
     y = fma(0.5 * y, e, y);
   }
@@ -146,20 +152,26 @@ __device__ __forceinline__ void hestenes_step(double (&v)[kW], double& al, int j
   // root, |t| <= 1); gam == 0 gives t = 0, an exact no-op (see jacobi_rotate)
   const double d = aqq - app, h = 2.0 * gam;
   const double x = fma(d, d, h * h);
-  const double r = x * rsq_nr_w(x > 0.0 ? x : 1.0);
+  const double r = x * rsq_nr_w<1>(x > 0.0 ? x : 1.0);
   const double den = fabs(d) + r;
-  double t = h * rcp_nr_w(den > 0.0 ? den : 1.0);
+  double t = h * rcp_nr_w<1>(den > 0.0 ? den : 1.0);
   t = d < 0.0 ? -t : t;
-  const double c = rsq_nr_w(fma(t, t, 1.0));
+  const double c = rsq_nr_w<2>(fma(t, t, 1.0));
   const double s = t * c;
   busy |= (gam * gam > tol2 * (app * aqq));
   const double sg = low ? -s : s;  // p: c g_p - s g_q ; q: s g_p + c g_q
 #pragma unroll
   for (int i = 0; i < kW; ++i) v[i] = fma(c, v[i], sg * pc[i]);
+  // the rotation maps the Gram block to diag(app - t gam, aqq + t gam) (as a_pp - t a_pq in
+  // the two-sided method); the norms are recomputed exactly at every sweep start
+  al = low ? fma(-t, gam, al) : fma(t, gam, al);
+}
+
+__device__ __forceinline__ double column_norm2(const double (&v)[kW]) {
   double a2 = 0.0;
 #pragma unroll
   for (int i = 0; i < kW; ++i) a2 = fma(v[i], v[i], a2);
-  al = a2;
+  return a2;
 }
 
 // One sample per wave; L is the wave's private LDS tile set. BF: ring products (and M) as
@@ -282,9 +294,8 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
 #pragma unroll
     for (int i = 0; i < kW; ++i) v[i] += (i == j) ? sig : 0.0;
     double al = 0.0;
-#pragma unroll
-    for (int i = 0; i < kW; ++i) al = fma(v[i], v[i], al);
     for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
+      al = column_norm2(v);
       bool busy = false;
       hestenes_step<1>(v, al, j, tol2, busy);
       hestenes_step<2>(v, al, j, tol2, busy);
@@ -303,6 +314,7 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
       hestenes_step<15>(v, al, j, tol2, busy);
       if (!__any(busy)) break;
     }
+    al = column_norm2(v);
     const double rs = rsq_nr_w(al);
     const double lam = al * rs - sig;
     const double inv = 1.0 / S;

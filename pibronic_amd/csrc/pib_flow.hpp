@@ -88,6 +88,20 @@ hipError_t launch_colsum_f32(int64_t M, int64_t N, const float* X, int64_t ldx, 
                              double* partial, hipStream_t s);
 
 hipError_t launch_coupling(int inverse, const CouplingArgs& a, hipStream_t s);
+
+// One coupling layer fused (MLP on pre-split weights + coupling + logdet), see
+// pib_flow_fused.hip. Supported: hidden 256, D/2 in {256, 512}, B a multiple of 32.
+struct FusedLayerArgs {
+  int64_t B;
+  int D, hidden, a_off, b_off, inverse;
+  float smax;
+  float* z;          // (B, D) flow state, transformed half updated in place
+  double* logdet;    // (B,)
+  const unsigned short *W1h, *W1l, *W2h, *W2l, *W3h, *W3l;  // transposed (N x K) bf16 hi/lo
+  const float *b1, *b2, *b3;
+};
+bool fused_layer_supported(int64_t B, int D, int hidden);
+hipError_t launch_flow_layer(const FusedLayerArgs& a, hipStream_t s);
 hipError_t launch_coupling_inverse_backward(const CouplingArgs& a, hipStream_t s);
 
 struct LogRhoArgs {

@@ -74,6 +74,7 @@ class Flow:
         self.step_count = 0
         self._bufs = {}
         self._version = 0          # bumped whenever the parameters change
+        self.use_fused = False     # True: one launch per coupling layer in bf16x3 draws where supported
         self._split_version = -1   # parameter version of the pre-split bf16 weights
 
     # ------------------------------------------------------------------ layout helpers
@@ -169,6 +170,16 @@ class Flow:
         ctx.gemm_bf16x3_presplit(B, D, h, bh, bl, h, w3h, w3l, h, C=o, ldc=D,
                                  bias=self.p(layer, "b3"), stream=stream)
 
+    def _fused_ok(self, B):
+        return self.use_fused and self.hidden == 256 and self.H in (256, 512) and B % 32 == 0
+
+    def _layer_fused(self, inverse, layer, zp, B, logdet, stream):
+        """The whole coupling layer in one launch (pib_flow_layer_fused)."""
+        a_off, b_off = self.halves(layer)
+        self.ctx.flow_layer_fused(inverse, B, self.D, self.hidden, a_off, b_off, self.smax, zp, logdet,
+                                  self._wsplit[layer], self.p(layer, "b1"), self.p(layer, "b2"),
+                                  self.p(layer, "b3"), stream)
+
     # ------------------------------------------------------------------ directions
     def forward(self, R0, stream=None):
         """Sampling direction: R0 (B, N, P) double -> R (B, N, P) double, log|det dF/dZ| (B,)."""
@@ -187,6 +198,9 @@ class Flow:
             self._split_weights(stream)
         for l in range(self.layers):
             a_off, b_off = self.halves(l)
+            if fast and self._fused_ok(B):
+                self._layer_fused(False, l, zp, B, logdet, stream)
+                continue
             if fast:
                 self._mlp_fast(l, zp, B, o, stream)
             else:
@@ -212,6 +226,9 @@ class Flow:
             self._split_weights(stream)
         for l in reversed(range(self.layers)):
             a_off, b_off = self.halves(l)
+            if fast and self._fused_ok(B):
+                self._layer_fused(True, l, zp, B, logdet, stream)
+                continue
             if fast:
                 self._mlp_fast(l, zp, B, o, stream)
             else:

@@ -306,4 +306,4 @@ def test_errors_are_reported(tmp_path):
     with pytest.raises(_native.NativeError, match="3 or more beads"):
         ctx.set_beads(2, 0.1)
     with pytest.raises(_native.NativeError, match="unknown flag"):
-        ctx.eval_block(0, 10, 8, 0, 10)
+        ctx.eval_block(0, 10, 32, 0, 10)
