@@ -62,6 +62,9 @@ int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* w
     a.set[s].csch = (const double*)c->csch[s].p;
     a.set[s].pref = (const double*)c->pref[s].p;
     a.set[s].shift = (const double*)c->shift[s].p;
+    a.set[s].th = (const double*)c->th[s].p;
+    a.set[s].lin = (const double*)c->lin[s].p;
+    a.set[s].k2 = (const double*)c->k2[s].p;
     if (s >= need) a.set[s] = a.set[SET_VIB];
   }
   a.energy = (const double*)c->energy.p;
@@ -138,6 +141,9 @@ int pib_ctx_destroy(pib_ctx* c) {
     release(c->tc_f[s]);
     release(c->cs_f[s]);
     release(c->lpref_f[s]);
+    release(c->th[s]);
+    release(c->lin[s]);
+    release(c->k2[s]);
   }
   release(c->s_cdf);
   release(c->s_chol);
@@ -249,6 +255,22 @@ int pib_set_omatrix_set(pib_ctx* c, int slot, int A_set, int N, const double* co
   if ((r = upload(c, c->csch[slot], csch, N * sizeof(double)))) return r;
   if ((r = upload(c, c->pref[slot], pref, A_set * sizeof(double)))) return r;
   if ((r = upload(c, c->shift[slot], shift, (size_t)A_set * N * sizeof(double)))) return r;
+  {  // expanded exponent constants (FP64 lanes kernel)
+    std::vector<double> th(N), lin((size_t)A_set * N), k2(A_set);
+    for (int j = 0; j < N; ++j) th[j] = coth[j] - csch[j];
+    for (int a = 0; a < A_set; ++a) {
+      double acc = 0.0;
+      for (int j = 0; j < N; ++j) {
+        const double d = shift[(size_t)a * N + j];
+        lin[(size_t)a * N + j] = -2.0 * th[j] * d;
+        acc += th[j] * d * d;
+      }
+      k2[a] = 2.0 * acc;
+    }
+    if ((r = upload(c, c->th[slot], th.data(), N * sizeof(double)))) return r;
+    if ((r = upload(c, c->lin[slot], lin.data(), (size_t)A_set * N * sizeof(double)))) return r;
+    if ((r = upload(c, c->k2[slot], k2.data(), A_set * sizeof(double)))) return r;
+  }
   {  // FP32 kernel constants
     std::vector<float> tc(N), cs(N), lp(A_set);
     for (int j = 0; j < N; ++j) {

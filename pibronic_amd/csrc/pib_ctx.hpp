@@ -33,6 +33,7 @@ struct pib_ctx {
   int set_A[pib::NUM_SETS] = {0, 0, 0, 0};
   int set_N[pib::NUM_SETS] = {0, 0, 0, 0};
   pib::DevBuf coth[pib::NUM_SETS], csch[pib::NUM_SETS], pref[pib::NUM_SETS], shift[pib::NUM_SETS];
+  pib::DevBuf th[pib::NUM_SETS], lin[pib::NUM_SETS], k2[pib::NUM_SETS];
   pib::DevBuf tc_f[pib::NUM_SETS], cs_f[pib::NUM_SETS], lpref_f[pib::NUM_SETS];
   bool pref_positive[pib::NUM_SETS] = {false, false, false, false};
   // beads

@@ -13,10 +13,16 @@
 // then combined by a log2(G)-level ordered tree of A x A products over cross-lane moves
 // (chunk 0 * chunk 1 * ... keeps the reference's left-to-right order).
 //
-// Numerics: the file is compiled with -ffp-contract=off so the O-matrix exponent is the
-// reference's expression term by term (coth*(x1^2 + x2^2) - ((2*csch)*x1)*x2, summed over
-// modes in order), which matters because that expression cancels when tau*omega is small.
-// Linear algebra (V, Jacobi, M, ring) uses explicit fma() where it helps accuracy.
+// Numerics: the O-matrix exponent sum_j coth (x1^2 + x2^2) - 2 csch x1 x2 (x1, x2 the shifted
+// coordinates of beads p, p+1) is evaluated as
+//   sum_j th (x^2 + x'^2) + csch (x - x')^2 + lin_a (x + x') + k2_a,   th = coth - csch,
+// which needs one FMA per surface and mode instead of nine operations. The reference's
+// expression cancels when tau*omega is small (coth*x^2 ~ 1e3 against a result ~1e1); this
+// form does not, so it sits about 100x closer to the exact value of the same rounded
+// constants, and the O-factors differ from the reference's by the reference's own rounding
+// (<= 1e-11 relative at A=12, N=24, P=128; ~1e-13 at the small workloads). V is built from
+// the unshifted coordinates (the reference's shift/unshift leaves an ulp of drift in q,
+// which moves V by ~1e-16). Linear algebra (V, Jacobi, M, ring) uses explicit fma().
 #include "pib_common.hpp"
 #include "pib_kernels.hpp"
 
@@ -137,12 +143,6 @@ __device__ __forceinline__ void jacobi_eigh(double (&a)[A][A], double (&u)[A][A]
   }
 }
 
-// O-matrix exponent term of one mode, in the reference's evaluation order
-// (pibronic/pimc/pimc.py:1106): coth*(x1**2 + x2**2) - 2.*csch*x1*x2.
-__device__ __forceinline__ double oterm(double coth, double csch2, double x1, double x2) {
-  return coth * (x1 * x1 + x2 * x2) - (csch2 * x1) * x2;
-}
-
 // Occupancy: for A <= 4 the VGPR budget is capped at 168 (three waves per SIMD). The kernel
 // is latency-bound (long dependent FP64 chains); at A = 4 with +-dbeta this is 16% faster
 // than the compiler's default of 256 VGPRs / two waves (measured on MI355X). Larger A keeps
@@ -155,7 +155,6 @@ struct EvalOccupancy {
 // The samples of one wave: wave w evaluates samples [w * 64/G, (w + 1) * 64/G) of `args`.
 template <int A, int AR, bool PM, bool DBG>
 __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
-  constexpr bool SEP = (AR != A);  // qTempTensor path (pibronic/pimc/pimc.py:1094-1096)
   const int lane = threadIdx.x & (kWave - 1);
   const int G = args.G, K = args.K, N = args.N, P = args.P;
   const int gl = lane & (G - 1);
@@ -167,15 +166,20 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
 
   // read-only constants: restrict-qualified so the compiler can keep them in scalar loads
   struct RO {
-    const double* __restrict__ coth;
+    const double* __restrict__ th;
     const double* __restrict__ csch;
+    const double* __restrict__ lin;
+    const double* __restrict__ k2;
     const double* __restrict__ pref;
-    const double* __restrict__ shift;
   };
-  const RO sr{args.set[SET_RHO].coth, args.set[SET_RHO].csch, args.set[SET_RHO].pref, args.set[SET_RHO].shift};
-  const RO sv{args.set[SET_VIB].coth, args.set[SET_VIB].csch, args.set[SET_VIB].pref, args.set[SET_VIB].shift};
-  const RO sp{args.set[SET_VIB_PLUS].coth, args.set[SET_VIB_PLUS].csch, args.set[SET_VIB_PLUS].pref, args.set[SET_VIB_PLUS].shift};
-  const RO sm{args.set[SET_VIB_MINUS].coth, args.set[SET_VIB_MINUS].csch, args.set[SET_VIB_MINUS].pref, args.set[SET_VIB_MINUS].shift};
+  const RO sr{args.set[SET_RHO].th, args.set[SET_RHO].csch, args.set[SET_RHO].lin, args.set[SET_RHO].k2,
+              args.set[SET_RHO].pref};
+  const RO sv{args.set[SET_VIB].th, args.set[SET_VIB].csch, args.set[SET_VIB].lin, args.set[SET_VIB].k2,
+              args.set[SET_VIB].pref};
+  const RO sp{args.set[SET_VIB_PLUS].th, args.set[SET_VIB_PLUS].csch, args.set[SET_VIB_PLUS].lin,
+              args.set[SET_VIB_PLUS].k2, args.set[SET_VIB_PLUS].pref};
+  const RO sm{args.set[SET_VIB_MINUS].th, args.set[SET_VIB_MINUS].csch, args.set[SET_VIB_MINUS].lin,
+              args.set[SET_VIB_MINUS].k2, args.set[SET_VIB_MINUS].pref};
   const double* __restrict__ Ev = args.energy;
   const double* __restrict__ Lv = args.linear;
   const double* __restrict__ Qd = args.qdiag;
@@ -201,7 +205,10 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
   const int pend = live ? min(pbeg + K, P) : pbeg;
   for (int p = pbeg; p < pend; ++p) {
     const int pn = (p + 1 == P) ? 0 : p + 1;
+    // O-matrix exponents (pimc.py:1087-1129) in the expanded cancellation-free form of OSet:
+    // the surface-independent parts t* once per mode, one FMA per surface and mode
     double er[AR], ev[A], ep[A], em[A], v[A][A];
+    double tr = 0.0, tv = 0.0, tp = 0.0, tm = 0.0;
 #pragma unroll
     for (int a = 0; a < AR; ++a) er[a] = 0.0;
 #pragma unroll
@@ -214,79 +221,44 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
     }
     for (int j = 0; j < N; ++j) {
       const double x0 = qs[j * P + p], xn = qs[j * P + pn];
-      // sampling model: q -= d_rho; O_rho; q += d_rho (pimc.py:1091-1127)
-      double q1[A], q1n[A];
-      {
-        const double c = sr.coth[j], s2 = 2.0 * sr.csch[j];
-#pragma unroll
-        for (int a = 0; a < AR; ++a) {
-          const double d = sr.shift[a * N + j];
-          const double x1 = x0 - d, x2 = xn - d;
-          const double t = oterm(c, s2, x1, x2);
-          er[a] += t;  // 0 + t_0 == t_0: same sums as the reference
-          if (!SEP) { q1[a] = x1 + d; q1n[a] = x2 + d; }
-        }
-        if (SEP) {
-#pragma unroll
-          for (int a = 0; a < A; ++a) { q1[a] = x0; q1n[a] = xn; }
-        }
+      const double sx = x0 + xn, dx = x0 - xn;
+      const double q2s = fma(x0, x0, xn * xn), d2 = dx * dx;
+      tr = fma(sr.csch[j], d2, fma(sr.th[j], q2s, tr));
+      tv = fma(sv.csch[j], d2, fma(sv.th[j], q2s, tv));
+      if (PM) {
+        tp = fma(sp.csch[j], d2, fma(sp.th[j], q2s, tp));
+        tm = fma(sm.csch[j], d2, fma(sm.th[j], q2s, tm));
       }
-      // coupled model at tau (and tau +- dbeta/P): q -= d_vib; O; q += d_vib
-      double q2[A];
-      {
-        const double c = sv.coth[j], s2 = 2.0 * sv.csch[j];
-        double cP = 0, sP2 = 0, cM = 0, sM2 = 0;
-        if (PM) { cP = sp.coth[j]; sP2 = 2.0 * sp.csch[j]; cM = sm.coth[j]; sM2 = 2.0 * sm.csch[j]; }
 #pragma unroll
-        for (int a = 0; a < A; ++a) {
-          const double d = sv.shift[a * N + j];
-          const double y1 = q1[a] - d, y2 = q1n[a] - d;
-          const double t = oterm(c, s2, y1, y2);
-          ev[a] += t;
-          q2[a] = y1 + d;
-          if (PM) {
-            const double q2n = y2 + d;
-            const double z1 = q2[a] - d, z2 = q2n - d;
-            const double tp = oterm(cP, sP2, z1, z2);
-            ep[a] += tp;
-            const double w1 = (z1 + d) - d, w2 = (z2 + d) - d;
-            const double tm = oterm(cM, sM2, w1, w2);
-            em[a] += tm;
-          }
+      for (int a = 0; a < AR; ++a) er[a] = fma(sr.lin[a * N + j], sx, er[a]);
+#pragma unroll
+      for (int a = 0; a < A; ++a) {
+        ev[a] = fma(sv.lin[a * N + j], sx, ev[a]);
+        if (PM) {
+          ep[a] = fma(sp.lin[a * N + j], sx, ep[a]);
+          em[a] = fma(sm.lin[a * N + j], sx, em[a]);
         }
       }
       // residual coupling, lower triangle b >= c (eigh UPLO='L'):
-      // V[b][c] = sum_j (q_c*Qh[j,j,b,c])*q_b + sum_j L[j,b,c]*q_b  (pimc.py:1147-1160)
+      // V[b][c] = sum_j (q_j Qh[j,j,b,c] + L[j,b,c]) q_j  (pimc.py:1147-1160)
 #pragma unroll
       for (int b = 0; b < A; ++b)
 #pragma unroll
         for (int c = 0; c <= b; ++c) {
-          const double lq = Qd ? fma(Qd[(j * A + b) * A + c], q2[c], Lv[(j * A + b) * A + c])
+          const double lq = Qd ? fma(Qd[(j * A + b) * A + c], x0, Lv[(j * A + b) * A + c])
                                : Lv[(j * A + b) * A + c];
-          v[b][c] = fma(q2[b], lq, v[b][c]);
+          v[b][c] = fma(x0, lq, v[b][c]);
         }
     }
-    // off-diagonal quadratic mode pairs (j != k): recompute the V-stage coordinate copies
+    // off-diagonal quadratic mode pairs (j != k)
     for (int t = 0; t < args.npair; ++t) {
       const int jj = pairs[2 * t], kk = pairs[2 * t + 1];
-      double qj[A], qk[A];
-#pragma unroll
-      for (int a = 0; a < A; ++a) {
-        double vj = qs[jj * P + p], vk = qs[kk * P + p];
-        if (!SEP) {
-          const double dj = sr.shift[a * N + jj], dk = sr.shift[a * N + kk];
-          vj = (vj - dj) + dj;
-          vk = (vk - dk) + dk;
-        }
-        const double ej = sv.shift[a * N + jj], ek = sv.shift[a * N + kk];
-        qj[a] = (vj - ej) + ej;
-        qk[a] = (vk - ek) + ek;
-      }
+      const double xj = qs[jj * P + p], xk = qs[kk * P + p];
       const double* Qh = qpair + (int64_t)t * A * A;
 #pragma unroll
       for (int b = 0; b < A; ++b)
 #pragma unroll
-        for (int c = 0; c <= b; ++c) v[b][c] = fma(qk[c] * Qh[b * A + c], qj[b], v[b][c]);
+        for (int c = 0; c <= b; ++c) v[b][c] = fma(xk * Qh[b * A + c], xj, v[b][c]);
     }
 
     // O-factors, per-bead scaling S_p = max over both O's (pimc.py:1076-1084)
@@ -294,16 +266,16 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
     double S = 0.0;
 #pragma unroll
     for (int a = 0; a < AR; ++a) {
-      Or[a] = exp(-0.5 * er[a]) * sr.pref[a];
+      Or[a] = exp(-0.5 * ((tr + er[a]) + sr.k2[a])) * sr.pref[a];
       S = fmax(S, Or[a]);
     }
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-      Ov[a] = exp(-0.5 * ev[a]) * sv.pref[a];
+      Ov[a] = exp(-0.5 * ((tv + ev[a]) + sv.k2[a])) * sv.pref[a];
       S = fmax(S, Ov[a]);
       if (PM) {
-        Op[a] = exp(-0.5 * ep[a]) * sp.pref[a];
-        Om[a] = exp(-0.5 * em[a]) * sm.pref[a];
+        Op[a] = exp(-0.5 * ((tp + ep[a]) + sp.k2[a])) * sp.pref[a];
+        Om[a] = exp(-0.5 * ((tm + em[a]) + sm.k2[a])) * sm.pref[a];
       }
     }
     if (args.noscale) S = 1.0;

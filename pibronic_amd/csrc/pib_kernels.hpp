@@ -12,6 +12,15 @@ struct OSet {
   const double* csch;   // N
   const double* pref;   // A_set, O-matrix prefactor F_a
   const double* shift;  // A_set * N, state shift d_aj
+  // The exponent of O_a at bead p in the cancellation-free form
+  //   sum_j th_j ((x_jp - d_aj)^2 + (x_jp' - d_aj)^2) + csch_j (x_jp - x_jp')^2
+  //   = sum_j [th_j (x_jp^2 + x_jp'^2) + csch_j (x_jp - x_jp')^2 + lin_aj (x_jp + x_jp')] + k2_a
+  // (p' = p + 1 around the ring), with th = coth - csch = tanh(tau w / 2) (the difference of
+  // the two rounded constants: exact for tau w < 1.317, and the coefficient the unexpanded
+  // form effectively uses), lin_aj = -2 th_j d_aj and k2_a = 2 sum_j th_j d_aj^2.
+  const double* th;     // N
+  const double* lin;    // A_set * N
+  const double* k2;     // A_set
 };
 
 enum : int { SET_RHO = 0, SET_VIB = 1, SET_VIB_PLUS = 2, SET_VIB_MINUS = 3, NUM_SETS = 4 };

@@ -4,9 +4,11 @@ Every test runs the product entry points (pibronic_amd.pimc -> libpibronic_hip.s
 GPU and checks against the CPU oracle on the same host-seeded coordinates.
 
 Tolerances (FP64 path): per-sample s_rho, s_g, s_gP, s_gM within 1e-10 relative of the
-oracle. The O-matrix exponent is evaluated in the reference's operation order, so the
-remaining differences are exp() implementations (1 ulp), the eigen-solver (Jacobi vs
-LAPACK) and the association order of the bead products.
+oracle. The remaining differences are the O-matrix exponent (evaluated in a
+cancellation-free form, closer to the exact value than the reference's expression; the
+difference is the reference's own rounding, pinned in test_exponent_form.py), exp()
+implementations (1 ulp), the eigen-solver (Jacobi vs LAPACK) and the association order of
+the bead products.
 """
 import os
 
@@ -51,13 +53,15 @@ def test_explicit_golden_vectors(explicit_fs):
     assert np.max(np.abs(r["M"] - M)) < 1e-14
     assert rel_err(r["s_g"], golden("numerator(g).npy")) < 1e-12
     assert rel_err(r["s_rho"], golden("denominator(rho).npy")) < 1e-12
-    # and against the oracle at the same inputs
+    # and against the oracle at the same inputs: the oracle's exponent (the reference's
+    # expression) carries up to 9e-13 of rounding on these coordinates, the kernel's 2e-14
+    # (test_exponent_form.py), so O agrees to ~5e-13
     s = oracle_setup(explicit_fs, 300.0, 5)
     o = po.evaluate(s, q, scale=False, keep=True)
-    assert rel_err(r["O_rho"], np.diagonal(o["O_rho"], axis1=2, axis2=3)) < 1e-14
-    assert rel_err(r["O_vib"], np.diagonal(o["O_vib"], axis1=2, axis2=3)) < 1e-14
-    assert rel_err(r["s_g"], o["s_g"]) < 1e-13
-    assert rel_err(r["s_rho"], o["s_rho"]) < 1e-13
+    assert rel_err(r["O_rho"], np.diagonal(o["O_rho"], axis1=2, axis2=3)) < 1e-12
+    assert rel_err(r["O_vib"], np.diagonal(o["O_vib"], axis1=2, axis2=3)) < 1e-12
+    assert rel_err(r["s_g"], o["s_g"]) < 3e-12  # products of five such O-factors
+    assert rel_err(r["s_rho"], o["s_rho"]) < 3e-12
 
 
 def _parity(FS, T, P, B, pm, seed, lanes=None, tol=RTOL):
