@@ -17,6 +17,50 @@ using namespace pib;
 
 namespace pib {
 
+// Per-mode constant records of the lanes kernel (layout: ModeRec). Rebuilt by the coupling
+// and O-matrix set setters once the coupling and the rho and vib sets agree in shape; the
+// +-dbeta entries stay zero until those sets are given.
+int rebuild_mode_table(pib_ctx* c) {
+  c->mode_tab_ok = false;
+  const int A = c->A, N = c->N, AR = c->set_A[SET_RHO];
+  if (!c->coupling_set || !AR || c->set_A[SET_VIB] != A || c->set_N[SET_RHO] != N || c->set_N[SET_VIB] != N)
+    return PIB_OK;
+  const ModeRec R(A, AR);
+  const size_t AA = (size_t)A * A;
+  std::vector<double> tab((size_t)N * R.len(), 0.0);
+  const bool pm = c->set_A[SET_VIB_PLUS] == A && c->set_A[SET_VIB_MINUS] == A &&
+                  c->set_N[SET_VIB_PLUS] == N && c->set_N[SET_VIB_MINUS] == N;
+  for (int j = 0; j < N; ++j) {
+    double* t = tab.data() + (size_t)j * R.len();
+    t[0] = c->h_th[SET_RHO][j];
+    t[1] = c->h_cs[SET_RHO][j];
+    t[2] = c->h_th[SET_VIB][j];
+    t[3] = c->h_cs[SET_VIB][j];
+    for (int a = 0; a < AR; ++a) t[R.lin_rho() + a] = c->h_lin[SET_RHO][(size_t)a * N + j];
+    for (int a = 0; a < A; ++a) t[R.lin_vib() + a] = c->h_lin[SET_VIB][(size_t)a * N + j];
+    int e = 0;
+    for (int b = 0; b < A; ++b)
+      for (int d = 0; d <= b; ++d, ++e) {
+        t[R.lin_q() + e] = c->h_linear[(size_t)j * AA + b * A + d];
+        t[R.quad() + e] = c->h_qdiag.empty() ? 0.0 : c->h_qdiag[(size_t)j * AA + b * A + d];
+      }
+    if (pm) {
+      t[R.pm() + 0] = c->h_th[SET_VIB_PLUS][j];
+      t[R.pm() + 1] = c->h_cs[SET_VIB_PLUS][j];
+      t[R.pm() + 2] = c->h_th[SET_VIB_MINUS][j];
+      t[R.pm() + 3] = c->h_cs[SET_VIB_MINUS][j];
+      for (int a = 0; a < A; ++a) {
+        t[R.lin_plus() + a] = c->h_lin[SET_VIB_PLUS][(size_t)a * N + j];
+        t[R.lin_minus() + a] = c->h_lin[SET_VIB_MINUS][(size_t)a * N + j];
+      }
+    }
+  }
+  int r;
+  if ((r = upload(c, c->mode_tab, tab.data(), tab.size() * sizeof(double)))) return r;
+  c->mode_tab_ok = true;
+  return PIB_OK;
+}
+
 int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* wide_out) {
   const bool pm = (flags & PIB_EVAL_PM) != 0;
   if (flags & ~(PIB_EVAL_PM | PIB_EVAL_NOSCALE | PIB_EVAL_WIDE | PIB_EVAL_FP32 | PIB_EVAL_BF16X3))
@@ -49,6 +93,7 @@ int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* w
                                             " kernel for A=" + std::to_string(c->A) +
                                             ", A_rho=" + std::to_string(AR));
   if (B < 0) return fail(c, PIB_ERR_ARG, "eval_block: B < 0 or out_stride < B");
+  if (!c->mode_tab_ok) return fail(c, PIB_ERR_STATE, "eval_block: per-mode constants not built");
   EvalArgs a{};
   a.B = B;
   a.N = c->N;
@@ -73,6 +118,7 @@ int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* w
   a.npair = c->npair;
   a.pair = (const int*)c->pair.p;
   a.qpair = (const double*)c->qpair.p;
+  a.mode_tab = (const double*)c->mode_tab.p;
   a.A = c->A;
   a.AR = AR;
   a.wide_lq = (const double2*)c->w_lq.p;
@@ -145,6 +191,7 @@ int pib_ctx_destroy(pib_ctx* c) {
     release(c->lin[s]);
     release(c->k2[s]);
   }
+  release(c->mode_tab);
   release(c->s_cdf);
   release(c->s_chol);
   release(c->s_shift);
@@ -200,6 +247,8 @@ int pib_set_coupling(pib_ctx* c, int A, int N, const double* E, const double* L,
   }
   c->has_qdiag = any_diag;
   if (any_diag && (r = upload(c, c->qdiag, qd.data(), qd.size() * sizeof(double)))) return r;
+  c->h_linear.assign(L, L + (size_t)N * AA);
+  if (any_diag) c->h_qdiag = qd; else c->h_qdiag.clear();
   c->npair = (int)(pairs.size() / 2);
   if (c->npair) {
     if ((r = upload(c, c->pair, pairs.data(), pairs.size() * sizeof(int)))) return r;
@@ -238,7 +287,7 @@ int pib_set_coupling(pib_ctx* c, int A, int N, const double* E, const double* L,
   c->A = A;
   c->N = N;
   c->coupling_set = true;
-  return PIB_OK;
+  return rebuild_mode_table(c);
 }
 
 int pib_set_omatrix_set(pib_ctx* c, int slot, int A_set, int N, const double* coth,
@@ -270,6 +319,9 @@ int pib_set_omatrix_set(pib_ctx* c, int slot, int A_set, int N, const double* co
     if ((r = upload(c, c->th[slot], th.data(), N * sizeof(double)))) return r;
     if ((r = upload(c, c->lin[slot], lin.data(), (size_t)A_set * N * sizeof(double)))) return r;
     if ((r = upload(c, c->k2[slot], k2.data(), A_set * sizeof(double)))) return r;
+    c->h_th[slot] = th;
+    c->h_cs[slot].assign(csch, csch + N);
+    c->h_lin[slot] = lin;
   }
   {  // FP32 kernel constants
     std::vector<float> tc(N), cs(N), lp(A_set);
@@ -289,7 +341,7 @@ int pib_set_omatrix_set(pib_ctx* c, int slot, int A_set, int N, const double* co
   }
   c->set_A[slot] = A_set;
   c->set_N[slot] = N;
-  return PIB_OK;
+  return rebuild_mode_table(c);
 }
 
 int pib_set_beads(pib_ctx* c, int P, double tau) {

@@ -34,6 +34,11 @@ struct pib_ctx {
   int set_N[pib::NUM_SETS] = {0, 0, 0, 0};
   pib::DevBuf coth[pib::NUM_SETS], csch[pib::NUM_SETS], pref[pib::NUM_SETS], shift[pib::NUM_SETS];
   pib::DevBuf th[pib::NUM_SETS], lin[pib::NUM_SETS], k2[pib::NUM_SETS];
+  // host copies feeding the per-mode constant records of the lanes kernel
+  std::vector<double> h_th[pib::NUM_SETS], h_cs[pib::NUM_SETS], h_lin[pib::NUM_SETS];
+  std::vector<double> h_linear, h_qdiag;
+  pib::DevBuf mode_tab;
+  bool mode_tab_ok = false;
   pib::DevBuf tc_f[pib::NUM_SETS], cs_f[pib::NUM_SETS], lpref_f[pib::NUM_SETS];
   bool pref_positive[pib::NUM_SETS] = {false, false, false, false};
   // beads

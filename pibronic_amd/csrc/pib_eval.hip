@@ -8,7 +8,7 @@
 // Mapping (gfx950, wave64): G lanes cooperate on one sample, 64/G samples per wave. Lane g
 // owns the contiguous bead chunk [g*K, min((g+1)*K, P)). Per bead it forms the diagonal
 // O-factors of every temperature set, the residual coupling V in registers, diagonalises V
-// with a cyclic Jacobi sweep (per lane, A x A in registers), builds M = exp(-tau V) and
+// with Jacobi sweeps (per lane, A x A in registers), builds M = exp(-tau V) and
 // multiplies M * diag(O_vib / S_p) into its running ring product. The G partial products are
 // then combined by a log2(G)-level ordered tree of A x A products over cross-lane moves
 // (chunk 0 * chunk 1 * ... keeps the reference's left-to-right order).
@@ -103,7 +103,7 @@ __device__ __forceinline__ void jacobi_rotate(double (&a)[A][A], double (&u)[A][
   }
 }
 
-// Symmetric eigen-decomposition by cyclic Jacobi sweeps. On entry a holds V (upper
+// Symmetric eigen-decomposition by Jacobi sweeps. On entry a holds V (upper
 // triangle); on exit its diagonal holds the eigenvalues and V = u diag(a) u^T. (A warm start
 // from the previous bead's eigenvectors saves about one sweep on average, but it keeps a
 // second A x A basis live across the mode loop; on gfx950 the lost occupancy costs more than
@@ -127,19 +127,81 @@ __device__ __forceinline__ void jacobi_eigh(double (&a)[A][A], double (&u)[A][A]
   // convergence takes the typical matrix from 1e-4 to below 1e-12 in one sweep
   const double tol2 = 1e-24 * fro2;
   constexpr int kMaxSweeps = (A <= 4) ? 12 : 20;
-  for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
-    double off = 0.0;
+  // round-robin (tournament) order: each round holds disjoint pairs, whose c/s chains do
+  // not depend on one another, so the scheduler overlaps them (the cyclic row order makes
+  // every rotation wait for the previous one); on the workload matrices it also converges in
+  // about one sweep less. Convergence is tested before every round, uniformly over the wave:
+  // converged lanes keep rotating (harmlessly) until every lane of the wave has converged.
+  constexpr int n = (A % 2 == 0) ? A : A + 1;  // odd A: one idle index per round
+  bool conv = false;
+  for (int sweep = 0; sweep < kMaxSweeps && !conv; ++sweep) {
 #pragma unroll
-    for (int i = 0; i < A; ++i)
+    for (int rnd = 0; rnd < n - 1; ++rnd) {
+      if (!conv) {
+        double off = 0.0;
 #pragma unroll
-      for (int j = i + 1; j < A; ++j) off = fma(a[i][j], a[i][j], off);
-    // the sweep count is uniform over the wave: converged lanes keep rotating (harmlessly)
-    // until every lane of the wave has converged, which keeps the sweep branch-free
-    if (__all(!(off > tol2))) break;
+        for (int i = 0; i < A; ++i)
 #pragma unroll
-    for (int p = 0; p < A - 1; ++p)
+          for (int j = i + 1; j < A; ++j) off = fma(a[i][j], a[i][j], off);
+        conv = __all(!(off > tol2));
+      }
+      if (!conv) {
 #pragma unroll
-      for (int q = p + 1; q < A; ++q) jacobi_rotate<A>(a, u, p, q);
+        for (int i = 0; i < n / 2; ++i) {
+          const int x = (i == 0) ? n - 1 : (rnd + i) % (n - 1);
+          const int y = (i == 0) ? rnd : (rnd - i + n - 1) % (n - 1);
+          if (x < A && y < A) jacobi_rotate<A>(a, u, x < y ? x : y, x < y ? y : x);
+        }
+      }
+    }
+  }
+}
+
+// Sum over modes for one bead p (next bead pn): the surface-independent parts t* of the
+// O-matrix exponents, the per-surface linear parts e*, and the lower triangle of the residual
+// coupling V[b][c] = sum_j (q_j 0.5 Q[j,j,b,c] + L[j,b,c]) q_j (pimc.py:1147-1160). Mode j's
+// constants are the record tab + j * ModeRec(A, AR).len().
+template <int A, int AR, bool PM, bool QD>
+__device__ __forceinline__ void mode_terms(const double* __restrict__ tab, const double* __restrict__ qs, int N,
+                                           int P, int p, int pn, double& tr, double& tv, double& tp,
+                                           double& tm, double (&er)[AR], double (&ev)[A], double (&ep)[A],
+                                           double (&em)[A], double (&v)[A][A]) {
+  constexpr ModeRec R(A, AR);
+  // the next mode's coordinates are loaded while this mode's arithmetic runs
+  double x0 = qs[p], xn = qs[pn];
+  for (int j = 0; j < N; ++j) {
+    const double* __restrict__ t = tab + j * R.len();
+    const int jn = (j + 1 < N) ? j + 1 : j;
+    const double x0_next = qs[jn * P + p], xn_next = qs[jn * P + pn];
+    const double sx = x0 + xn, dx = x0 - xn;
+    const double q2s = fma(x0, x0, xn * xn), d2 = dx * dx;
+    tr = fma(t[1], d2, fma(t[0], q2s, tr));
+    tv = fma(t[3], d2, fma(t[2], q2s, tv));
+    if (PM) {
+      tp = fma(t[R.pm() + 1], d2, fma(t[R.pm() + 0], q2s, tp));
+      tm = fma(t[R.pm() + 3], d2, fma(t[R.pm() + 2], q2s, tm));
+    }
+#pragma unroll
+    for (int a = 0; a < AR; ++a) er[a] = fma(t[R.lin_rho() + a], sx, er[a]);
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      ev[a] = fma(t[R.lin_vib() + a], sx, ev[a]);
+      if (PM) {
+        ep[a] = fma(t[R.lin_plus() + a], sx, ep[a]);
+        em[a] = fma(t[R.lin_minus() + a], sx, em[a]);
+      }
+    }
+    // one scalar operand per FMA (a VALU op reads at most one SGPR pair)
+    const double xx = x0 * x0;
+#pragma unroll
+    for (int b = 0, e = 0; b < A; ++b)
+#pragma unroll
+      for (int c = 0; c <= b; ++c, ++e) {
+        v[b][c] = fma(t[R.lin_q() + e], x0, v[b][c]);
+        if (QD) v[b][c] = fma(t[R.quad() + e], xx, v[b][c]);
+      }
+    x0 = x0_next;
+    xn = xn_next;
   }
 }
 
@@ -166,22 +228,14 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
 
   // read-only constants: restrict-qualified so the compiler can keep them in scalar loads
   struct RO {
-    const double* __restrict__ th;
-    const double* __restrict__ csch;
-    const double* __restrict__ lin;
     const double* __restrict__ k2;
     const double* __restrict__ pref;
   };
-  const RO sr{args.set[SET_RHO].th, args.set[SET_RHO].csch, args.set[SET_RHO].lin, args.set[SET_RHO].k2,
-              args.set[SET_RHO].pref};
-  const RO sv{args.set[SET_VIB].th, args.set[SET_VIB].csch, args.set[SET_VIB].lin, args.set[SET_VIB].k2,
-              args.set[SET_VIB].pref};
-  const RO sp{args.set[SET_VIB_PLUS].th, args.set[SET_VIB_PLUS].csch, args.set[SET_VIB_PLUS].lin,
-              args.set[SET_VIB_PLUS].k2, args.set[SET_VIB_PLUS].pref};
-  const RO sm{args.set[SET_VIB_MINUS].th, args.set[SET_VIB_MINUS].csch, args.set[SET_VIB_MINUS].lin,
-              args.set[SET_VIB_MINUS].k2, args.set[SET_VIB_MINUS].pref};
+  const RO sr{args.set[SET_RHO].k2, args.set[SET_RHO].pref};
+  const RO sv{args.set[SET_VIB].k2, args.set[SET_VIB].pref};
+  const RO sp{args.set[SET_VIB_PLUS].k2, args.set[SET_VIB_PLUS].pref};
+  const RO sm{args.set[SET_VIB_MINUS].k2, args.set[SET_VIB_MINUS].pref};
   const double* __restrict__ Ev = args.energy;
-  const double* __restrict__ Lv = args.linear;
   const double* __restrict__ Qd = args.qdiag;
   const int* __restrict__ pairs = args.pair;
   const double* __restrict__ qpair = args.qpair;
@@ -219,37 +273,10 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
 #pragma unroll
       for (int c = 0; c <= a; ++c) v[a][c] = 0.0;
     }
-    for (int j = 0; j < N; ++j) {
-      const double x0 = qs[j * P + p], xn = qs[j * P + pn];
-      const double sx = x0 + xn, dx = x0 - xn;
-      const double q2s = fma(x0, x0, xn * xn), d2 = dx * dx;
-      tr = fma(sr.csch[j], d2, fma(sr.th[j], q2s, tr));
-      tv = fma(sv.csch[j], d2, fma(sv.th[j], q2s, tv));
-      if (PM) {
-        tp = fma(sp.csch[j], d2, fma(sp.th[j], q2s, tp));
-        tm = fma(sm.csch[j], d2, fma(sm.th[j], q2s, tm));
-      }
-#pragma unroll
-      for (int a = 0; a < AR; ++a) er[a] = fma(sr.lin[a * N + j], sx, er[a]);
-#pragma unroll
-      for (int a = 0; a < A; ++a) {
-        ev[a] = fma(sv.lin[a * N + j], sx, ev[a]);
-        if (PM) {
-          ep[a] = fma(sp.lin[a * N + j], sx, ep[a]);
-          em[a] = fma(sm.lin[a * N + j], sx, em[a]);
-        }
-      }
-      // residual coupling, lower triangle b >= c (eigh UPLO='L'):
-      // V[b][c] = sum_j (q_j Qh[j,j,b,c] + L[j,b,c]) q_j  (pimc.py:1147-1160)
-#pragma unroll
-      for (int b = 0; b < A; ++b)
-#pragma unroll
-        for (int c = 0; c <= b; ++c) {
-          const double lq = Qd ? fma(Qd[(j * A + b) * A + c], x0, Lv[(j * A + b) * A + c])
-                               : Lv[(j * A + b) * A + c];
-          v[b][c] = fma(x0, lq, v[b][c]);
-        }
-    }
+    if (Qd)
+      mode_terms<A, AR, PM, true>(args.mode_tab, qs, N, P, p, pn, tr, tv, tp, tm, er, ev, ep, em, v);
+    else
+      mode_terms<A, AR, PM, false>(args.mode_tab, qs, N, P, p, pn, tr, tv, tp, tm, er, ev, ep, em, v);
     // off-diagonal quadratic mode pairs (j != k)
     for (int t = 0; t < args.npair; ++t) {
       const int jj = pairs[2 * t], kk = pairs[2 * t + 1];

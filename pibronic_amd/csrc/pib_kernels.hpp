@@ -52,6 +52,7 @@ struct EvalArgs {
   int npair;              // off-diagonal quadratic mode pairs (j != k)
   const int* pair;        // 2*npair (j, k)
   const double* qpair;    // npair*A*A = 0.5*Q[j,k]
+  const double* mode_tab; // N records of ModeRec(A, AR).len() doubles (lanes kernel)
   double* out;            // NUM_OUT rows of out_stride doubles
   int64_t out_stride;
   double* dbg;            // optional per-(sample, bead) dump, see eval_debug_record()
@@ -69,6 +70,30 @@ struct EvalArgs {
 };
 
 __host__ __device__ inline int eval_debug_record(int A, int AR) { return AR + A + 1 + A * A; }
+
+// Per-mode constant record of the FP64 lanes kernel (mode j at mode_tab + j * record length),
+// so that one mode's constants come in a few wide scalar loads from one address:
+//   [0..4)              th_rho, csch_rho, th_vib, csch_vib
+//   [4, 4+AR)           lin_rho[a]
+//   [4+AR, 4+AR+A)      lin_vib[a]
+//   next T = A(A+1)/2   L[j][b][c], b >= c (row-major lower triangle)
+//   next T              0.5 Q[j][j][b][c], same order (zeros without quadratic terms)
+//   next 4              th_plus, csch_plus, th_minus, csch_minus  (+-dbeta sets)
+//   next 2A             lin_plus[a], lin_minus[a]
+// padded to an even count (16-byte records).
+struct ModeRec {
+  int A, AR;
+  __host__ __device__ constexpr ModeRec(int A_, int AR_) : A(A_), AR(AR_) {}
+  __host__ __device__ constexpr int tri() const { return A * (A + 1) / 2; }
+  __host__ __device__ constexpr int lin_rho() const { return 4; }
+  __host__ __device__ constexpr int lin_vib() const { return 4 + AR; }
+  __host__ __device__ constexpr int lin_q() const { return 4 + AR + A; }
+  __host__ __device__ constexpr int quad() const { return lin_q() + tri(); }
+  __host__ __device__ constexpr int pm() const { return quad() + tri(); }
+  __host__ __device__ constexpr int lin_plus() const { return pm() + 4; }
+  __host__ __device__ constexpr int lin_minus() const { return lin_plus() + A; }
+  __host__ __device__ constexpr int len() const { return (lin_minus() + A + 1) & ~1; }
+};
 
 struct SampleArgs {
   int64_t B;
