@@ -55,8 +55,14 @@ int rebuild_mode_table(pib_ctx* c) {
       }
     }
   }
+  // float copy, records padded to 16 bytes
+  std::vector<float> tabf((size_t)N * R.len_f(), 0.f);
+  for (int j = 0; j < N; ++j)
+    for (int i = 0; i < R.lin_minus() + A; ++i)
+      tabf[(size_t)j * R.len_f() + i] = (float)tab[(size_t)j * R.len() + i];
   int r;
   if ((r = upload(c, c->mode_tab, tab.data(), tab.size() * sizeof(double)))) return r;
+  if ((r = upload(c, c->mode_tab_f, tabf.data(), tabf.size() * sizeof(float)))) return r;
   c->mode_tab_ok = true;
   return PIB_OK;
 }
@@ -119,6 +125,7 @@ int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* w
   a.pair = (const int*)c->pair.p;
   a.qpair = (const double*)c->qpair.p;
   a.mode_tab = (const double*)c->mode_tab.p;
+  a.mode_tab_f = (const float*)c->mode_tab_f.p;
   a.A = c->A;
   a.AR = AR;
   a.wide_lq = (const double2*)c->w_lq.p;
@@ -129,6 +136,7 @@ int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* w
     a.setf[s].tc = (const float*)c->tc_f[src].p;
     a.setf[s].cs = (const float*)c->cs_f[src].p;
     a.setf[s].lpref = (const float*)c->lpref_f[src].p;
+    a.setf[s].k2 = (const float*)c->k2_f[src].p;
   }
   a.energy_f = (const float*)c->energy_f.p;
   a.linear_f = (const float*)c->linear_f.p;
@@ -190,8 +198,10 @@ int pib_ctx_destroy(pib_ctx* c) {
     release(c->th[s]);
     release(c->lin[s]);
     release(c->k2[s]);
+    release(c->k2_f[s]);
   }
   release(c->mode_tab);
+  release(c->mode_tab_f);
   release(c->s_cdf);
   release(c->s_chol);
   release(c->s_shift);
@@ -319,6 +329,8 @@ int pib_set_omatrix_set(pib_ctx* c, int slot, int A_set, int N, const double* co
     if ((r = upload(c, c->th[slot], th.data(), N * sizeof(double)))) return r;
     if ((r = upload(c, c->lin[slot], lin.data(), (size_t)A_set * N * sizeof(double)))) return r;
     if ((r = upload(c, c->k2[slot], k2.data(), A_set * sizeof(double)))) return r;
+    std::vector<float> k2f(k2.begin(), k2.end());
+    if ((r = upload(c, c->k2_f[slot], k2f.data(), A_set * sizeof(float)))) return r;
     c->h_th[slot] = th;
     c->h_cs[slot].assign(csch, csch + N);
     c->h_lin[slot] = lin;

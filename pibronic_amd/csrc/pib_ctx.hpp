@@ -37,7 +37,7 @@ struct pib_ctx {
   // host copies feeding the per-mode constant records of the lanes kernel
   std::vector<double> h_th[pib::NUM_SETS], h_cs[pib::NUM_SETS], h_lin[pib::NUM_SETS];
   std::vector<double> h_linear, h_qdiag;
-  pib::DevBuf mode_tab;
+  pib::DevBuf mode_tab, mode_tab_f, k2_f[pib::NUM_SETS];
   bool mode_tab_ok = false;
   pib::DevBuf tc_f[pib::NUM_SETS], cs_f[pib::NUM_SETS], lpref_f[pib::NUM_SETS];
   bool pref_positive[pib::NUM_SETS] = {false, false, false, false};

@@ -1,10 +1,10 @@
 // Reduced-precision variant of k_eval_fp64 (same mapping, same outputs, A <= 8).
 //
 // What runs in FP32: the O-matrix exponents, the coupling matrix V, the Jacobi sweeps, M and
-// the ring products. What stays FP64: the coordinate differences x - d (formed once per
-// mode from the FP64 samples), the rotation parameters of near-degenerate Jacobi pairs
-// (|a_qq - a_pp| < 1e-6 |V|_F), the per-sample log-scale sum_p log S_p and the per-surface
-// log-products of rho, and the final assembly of s_g, s_rho.
+// the ring products. What stays FP64: the bead coordinates as read (rounded to float once per
+// mode), the rotation parameters of near-degenerate Jacobi pairs (|a_qq - a_pp| < 1e-6 |V|_F),
+// the per-sample log-scale sum_p log S_p and the per-surface log-products of rho, and the
+// final assembly of s_g, s_rho.
 //
 // Range: the O-factors are formed in the log domain, log O_a = log F_a - e_a / 2, so that
 // S_p = max_a O_a and O_a / S_p = exp(log O_a - log S_p) never under/overflow before scaling.
@@ -12,8 +12,11 @@
 // floating point) and the exponents are carried as integers; s_g = trace * 2^exp in FP64.
 //
 // The O exponent coth (x1^2 + x2^2) - 2 csch x1 x2 cancels when tau w is small; here it is
-// evaluated as tanh(tau w / 2) (x1^2 + x2^2) + csch (x1 - x2)^2, a sum of non-negative terms
-// (the identity coth - csch = tanh(x/2)).
+// evaluated, as in the FP64 kernel, in the expanded form
+// sum_j th (x^2 + x'^2) + csch (x - x')^2 + lin_a (x + x') + k2_a (th = coth - csch), whose
+// FP32 rounding is no larger than that of the shifted form th ((x-d)^2 + (x'-d)^2) +
+// csch (x - x')^2 evaluated with x - d in FP64 (measured on the workload models: exponent
+// error <= 5e-5 absolute for |e| ~ 10-20).
 #include "pib_common.hpp"
 #include "pib_kernels.hpp"
 
@@ -26,8 +29,9 @@ __device__ __forceinline__ float& UTf(float (&a)[A][A], int i, int j) {
   return i <= j ? a[i][j] : a[j][i];
 }
 
-// tan of the Jacobi angle (see jacobi_rotate in the FP64 kernel): FP32, or FP64 when the
-// diagonal gap is below `gap` (near-degenerate pair).
+// tan of the Jacobi angle (see jacobi_rotate in the FP64 kernel): FP32 with the hardware
+// reciprocal and reciprocal square root (1 ulp), or FP64 when the diagonal gap is below
+// `gap` (near-degenerate pair).
 __device__ __forceinline__ float jacobi_t(float app, float aqq, float apq, float gap) {
   const float d = aqq - app;
   if (fabsf(d) < gap) {
@@ -38,9 +42,10 @@ __device__ __forceinline__ float jacobi_t(float app, float aqq, float apq, float
     return (float)(dd < 0.0 ? -t : t);
   }
   const float h = 2.0f * apq;
-  const float r = sqrtf(fmaf(d, d, h * h));
+  const float x = fmaf(d, d, h * h);
+  const float r = x > 0.0f ? x * __builtin_amdgcn_rsqf(x) : 0.0f;
   const float den = fabsf(d) + r;
-  float t = den > 0.0f ? h / den : 0.0f;
+  const float t = den > 0.0f ? h * __builtin_amdgcn_rcpf(den) : 0.0f;
   return d < 0.0f ? -t : t;
 }
 
@@ -49,7 +54,7 @@ __device__ __forceinline__ void jacobi_rotate_f(float (&a)[A][A], float (&u)[A][
                                                 float gap) {
   const float apq = a[p][q];
   const float t = jacobi_t(a[p][p], a[q][q], apq, gap);
-  const float c = 1.0f / sqrtf(fmaf(t, t, 1.0f));
+  const float c = __builtin_amdgcn_rsqf(fmaf(t, t, 1.0f));
   const float s = t * c;
   a[p][p] = fmaf(-t, apq, a[p][p]);
   a[q][q] = fmaf(t, apq, a[q][q]);
@@ -69,6 +74,8 @@ __device__ __forceinline__ void jacobi_rotate_f(float (&a)[A][A], float (&u)[A][
   }
 }
 
+// Jacobi sweeps in round-robin order with a convergence test before every round (as the
+// FP64 kernel's jacobi_eigh), to a relative off-diagonal norm of 1e-6.
 template <int A>
 __device__ __forceinline__ void jacobi_eigh_f(float (&a)[A][A], float (&u)[A][A]) {
 #pragma unroll
@@ -84,19 +91,76 @@ __device__ __forceinline__ void jacobi_eigh_f(float (&a)[A][A], float (&u)[A][A]
     for (int j = i + 1; j < A; ++j) fro2 = fmaf(2.0f * a[i][j], a[i][j], fro2);
   }
   const float gap = 1e-6f * sqrtf(fro2);
-  const float tol2 = 1e-12f * fro2;  // relative off-diagonal norm 1e-6
+  const float tol2 = 1e-12f * fro2;
   constexpr int kMaxSweeps = (A <= 4) ? 12 : 20;
-  for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
-    float off = 0.0f;
+  constexpr int n = (A % 2 == 0) ? A : A + 1;
+  bool conv = false;
+  for (int sweep = 0; sweep < kMaxSweeps && !conv; ++sweep) {
 #pragma unroll
-    for (int i = 0; i < A; ++i)
+    for (int rnd = 0; rnd < n - 1; ++rnd) {
+      if (!conv) {
+        float off = 0.0f;
 #pragma unroll
-      for (int j = i + 1; j < A; ++j) off = fmaf(a[i][j], a[i][j], off);
-    if (__all(!(off > tol2))) break;
+        for (int i = 0; i < A; ++i)
 #pragma unroll
-    for (int p = 0; p < A - 1; ++p)
+          for (int j = i + 1; j < A; ++j) off = fmaf(a[i][j], a[i][j], off);
+        conv = __all(!(off > tol2));
+      }
+      if (!conv) {
 #pragma unroll
-      for (int q = p + 1; q < A; ++q) jacobi_rotate_f<A>(a, u, p, q, gap);
+        for (int i = 0; i < n / 2; ++i) {
+          const int x = (i == 0) ? n - 1 : (rnd + i) % (n - 1);
+          const int y = (i == 0) ? rnd : (rnd - i + n - 1) % (n - 1);
+          if (x < A && y < A) jacobi_rotate_f<A>(a, u, x < y ? x : y, x < y ? y : x, gap);
+        }
+      }
+    }
+  }
+}
+
+// Sum over modes for one bead (the FP64 kernel's mode_terms in FP32, coordinates rounded to
+// float once per mode): surface-independent exponent parts t*, per-surface linear parts e*,
+// and the lower triangle of V.
+template <int A, int AR, bool PM, bool QD>
+__device__ __forceinline__ void mode_terms_f(const float* __restrict__ tab, const double* __restrict__ qs, int N,
+                                             int P, int p, int pn, float& tr, float& tv, float& tp,
+                                             float& tm, float (&er)[AR], float (&ev)[A], float (&ep)[A],
+                                             float (&em)[A], float (&v)[A][A]) {
+  constexpr ModeRec R(A, AR);
+  double x0d = qs[p], xnd = qs[pn];
+  for (int j = 0; j < N; ++j) {
+    const float* __restrict__ t = tab + j * R.len_f();
+    const int jn = (j + 1 < N) ? j + 1 : j;
+    const double x0_next = qs[jn * P + p], xn_next = qs[jn * P + pn];
+    const float x0 = (float)x0d, xn = (float)xnd;
+    const float sx = x0 + xn, dx = x0 - xn;
+    const float q2s = fmaf(x0, x0, xn * xn), d2 = dx * dx;
+    tr = fmaf(t[1], d2, fmaf(t[0], q2s, tr));
+    tv = fmaf(t[3], d2, fmaf(t[2], q2s, tv));
+    if (PM) {
+      tp = fmaf(t[R.pm() + 1], d2, fmaf(t[R.pm() + 0], q2s, tp));
+      tm = fmaf(t[R.pm() + 3], d2, fmaf(t[R.pm() + 2], q2s, tm));
+    }
+#pragma unroll
+    for (int a = 0; a < AR; ++a) er[a] = fmaf(t[R.lin_rho() + a], sx, er[a]);
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      ev[a] = fmaf(t[R.lin_vib() + a], sx, ev[a]);
+      if (PM) {
+        ep[a] = fmaf(t[R.lin_plus() + a], sx, ep[a]);
+        em[a] = fmaf(t[R.lin_minus() + a], sx, em[a]);
+      }
+    }
+    const float xx = x0 * x0;
+#pragma unroll
+    for (int b = 0, e = 0; b < A; ++b)
+#pragma unroll
+      for (int c = 0; c <= b; ++c, ++e) {
+        v[b][c] = fmaf(t[R.lin_q() + e], x0, v[b][c]);
+        if (QD) v[b][c] = fmaf(t[R.quad() + e], xx, v[b][c]);
+      }
+    x0d = x0_next;
+    xnd = xn_next;
   }
 }
 
@@ -144,7 +208,6 @@ __device__ __forceinline__ float shfl_down_f(float v, int d) { return __shfl_dow
 
 template <int A, int AR, bool PM>
 __global__ void __launch_bounds__(256) k_eval_fp32(const EvalArgs args) {
-  constexpr bool SEP = (AR != A);
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int G = args.G, K = args.K, N = args.N, P = args.P;
@@ -152,13 +215,10 @@ __global__ void __launch_bounds__(256) k_eval_fp32(const EvalArgs args) {
   const int64_t sample = wave * (kWave / G) + lane / G;
   const bool live = sample < args.B;
   const double* __restrict__ qs = args.q + (live ? sample : 0) * (int64_t)N * P;
-  const double* __restrict__ dr = args.set[SET_RHO].shift;
-  const double* __restrict__ dv = args.set[SET_VIB].shift;
   const OSetF fr = args.setf[SET_RHO], fv = args.setf[SET_VIB];
   const OSetF fp = args.setf[SET_VIB_PLUS], fm = args.setf[SET_VIB_MINUS];
   const float* __restrict__ Ev = args.energy_f;
-  const float* __restrict__ Lv = args.linear_f;
-  const float* __restrict__ Qd = args.qdiag_f;
+  const bool has_q = args.qdiag_f != nullptr;
   const float tau = (float)args.tau;
 
   float Ng[A][A], Np[A][A], Nm[A][A];
@@ -180,6 +240,7 @@ __global__ void __launch_bounds__(256) k_eval_fp32(const EvalArgs args) {
   for (int p = pbeg; p < pend; ++p) {
     const int pn = (p + 1 == P) ? 0 : p + 1;
     float er[AR], ev[A], epl[A], emi[A], v[A][A];
+    float tr = 0.f, tv = 0.f, tpl = 0.f, tmi = 0.f;
 #pragma unroll
     for (int a = 0; a < AR; ++a) er[a] = 0.0f;
 #pragma unroll
@@ -190,43 +251,10 @@ __global__ void __launch_bounds__(256) k_eval_fp32(const EvalArgs args) {
 #pragma unroll
       for (int c = 0; c <= a; ++c) v[a][c] = 0.0f;
     }
-    for (int j = 0; j < N; ++j) {
-      const double x0 = qs[j * P + p], xn = qs[j * P + pn];
-      {
-        const float tc = fr.tc[j], cs = fr.cs[j];
-#pragma unroll
-        for (int a = 0; a < AR; ++a) {
-          const float x1 = (float)(x0 - dr[a * N + j]), x2 = (float)(xn - dr[a * N + j]);
-          const float dx = x1 - x2;
-          er[a] = fmaf(tc, fmaf(x1, x1, x2 * x2), fmaf(cs, dx * dx, er[a]));
-        }
-      }
-      {
-        const float tc = fv.tc[j], cs = fv.cs[j];
-        float tP = 0.f, cP = 0.f, tM = 0.f, cM = 0.f;
-        if (PM) { tP = fp.tc[j]; cP = fp.cs[j]; tM = fm.tc[j]; cM = fm.cs[j]; }
-#pragma unroll
-        for (int a = 0; a < A; ++a) {
-          const float x1 = (float)(x0 - dv[a * N + j]), x2 = (float)(xn - dv[a * N + j]);
-          const float dx = x1 - x2, s2 = fmaf(x1, x1, x2 * x2), d2 = dx * dx;
-          ev[a] = fmaf(tc, s2, fmaf(cs, d2, ev[a]));
-          if (PM) {
-            epl[a] = fmaf(tP, s2, fmaf(cP, d2, epl[a]));
-            emi[a] = fmaf(tM, s2, fmaf(cM, d2, emi[a]));
-          }
-        }
-      }
-      // residual coupling, lower triangle (eigh UPLO='L'), at the sample coordinate
-      const float x = (float)x0;
-#pragma unroll
-      for (int b = 0; b < A; ++b)
-#pragma unroll
-        for (int c = 0; c <= b; ++c) {
-          const float lq = Qd ? fmaf(Qd[(j * A + b) * A + c], x, Lv[(j * A + b) * A + c])
-                              : Lv[(j * A + b) * A + c];
-          v[b][c] = fmaf(x, lq, v[b][c]);
-        }
-    }
+    if (has_q)
+      mode_terms_f<A, AR, PM, true>(args.mode_tab_f, qs, N, P, p, pn, tr, tv, tpl, tmi, er, ev, epl, emi, v);
+    else
+      mode_terms_f<A, AR, PM, false>(args.mode_tab_f, qs, N, P, p, pn, tr, tv, tpl, tmi, er, ev, epl, emi, v);
     for (int t = 0; t < args.npair; ++t) {
       const int jj = args.pair[2 * t], kk = args.pair[2 * t + 1];
       const float xj = (float)qs[jj * P + p], xk = (float)qs[kk * P + p];
@@ -242,16 +270,16 @@ __global__ void __launch_bounds__(256) k_eval_fp32(const EvalArgs args) {
     float lS = -INFINITY;
 #pragma unroll
     for (int a = 0; a < AR; ++a) {
-      lOr[a] = fmaf(-0.5f, er[a], fr.lpref[a]);
+      lOr[a] = fmaf(-0.5f, (tr + er[a]) + fr.k2[a], fr.lpref[a]);
       lS = fmaxf(lS, lOr[a]);
     }
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-      lOv[a] = fmaf(-0.5f, ev[a], fv.lpref[a]);
+      lOv[a] = fmaf(-0.5f, (tv + ev[a]) + fv.k2[a], fv.lpref[a]);
       lS = fmaxf(lS, lOv[a]);
       if (PM) {
-        lOp[a] = fmaf(-0.5f, epl[a], fp.lpref[a]);
-        lOm[a] = fmaf(-0.5f, emi[a], fm.lpref[a]);
+        lOp[a] = fmaf(-0.5f, (tpl + epl[a]) + fp.k2[a], fp.lpref[a]);
+        lOm[a] = fmaf(-0.5f, (tmi + emi[a]) + fm.k2[a], fm.lpref[a]);
       }
     }
     if (args.noscale) lS = 0.0f;

@@ -32,6 +32,7 @@ struct OSetF {
   const float* tc;     // N
   const float* cs;     // N
   const float* lpref;  // A_set
+  const float* k2;     // A_set, 2 sum_j th_j d_aj^2 (see OSet)
 };
 
 // Output rows of the per-sample result matrix (each row has out_stride entries).
@@ -53,6 +54,7 @@ struct EvalArgs {
   const int* pair;        // 2*npair (j, k)
   const double* qpair;    // npair*A*A = 0.5*Q[j,k]
   const double* mode_tab; // N records of ModeRec(A, AR).len() doubles (lanes kernel)
+  const float* mode_tab_f; // N records of ModeRec(A, AR).len_f() floats (FP32 kernel)
   double* out;            // NUM_OUT rows of out_stride doubles
   int64_t out_stride;
   double* dbg;            // optional per-(sample, bead) dump, see eval_debug_record()
@@ -93,6 +95,7 @@ struct ModeRec {
   __host__ __device__ constexpr int lin_plus() const { return pm() + 4; }
   __host__ __device__ constexpr int lin_minus() const { return lin_plus() + A; }
   __host__ __device__ constexpr int len() const { return (lin_minus() + A + 1) & ~1; }
+  __host__ __device__ constexpr int len_f() const { return (lin_minus() + A + 3) & ~3; }  // float copy
 };
 
 struct SampleArgs {
