@@ -308,8 +308,8 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--flow-gemm", default="bf16x3", choices=["f32", "bf16x3"],
                     help="precision of the flow's dense layers (FP32 MFMA or BF16x3 split)")
-    ap.add_argument("--flow-fused", type=int, default=0,
-                    help="1: one launch per coupling layer in the draws (bf16x3, hidden 256)")
+    ap.add_argument("--flow-fused", type=int, default=1,
+                    help="0: layer-by-layer products in the draws instead of one launch per coupling layer")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this many MiB 3 times "

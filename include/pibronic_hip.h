@@ -199,10 +199,15 @@ int pib_gemm_bf16x3_presplit(pib_ctx* ctx, int64_t M, int64_t N, int64_t K,
                              float* C_dev, int64_t ldc, uint16_t* O_hi_dev, uint16_t* O_lo_dev,
                              int64_t ldo, const float* bias_dev, int relu, void* stream);
 
-/* Split X (rows x cols, ldx) into bf16 hi = RNE(x), lo = RNE(x - hi); with transpose the
- * outputs are cols x rows (row length ldo). */
+/* Split X (rows x cols, ldx) into bf16 hi = RNE(x), lo = RNE(x - hi). mode bits:
+ * PIB_SPLIT_TRANSPOSE: the outputs are cols x rows (row length ldo);
+ * PIB_SPLIT_PERM32 (with TRANSPOSE, rows a multiple of 32): within every block of 32 output
+ * columns, column 32b + w is stored at 32b + 8((w % 16) / 4) + 4(w / 16) + w % 4, the operand
+ * order of pib_flow_layer_fused's second and third products. */
+#define PIB_SPLIT_TRANSPOSE 1
+#define PIB_SPLIT_PERM32 2
 int pib_split_bf16(pib_ctx* ctx, int64_t rows, int64_t cols, const float* X_dev, int64_t ldx,
-                   int transpose, uint16_t* hi_dev, uint16_t* lo_dev, int64_t ldo, void* stream);
+                   int mode, uint16_t* hi_dev, uint16_t* lo_dev, int64_t ldo, void* stream);
 
 /* Normal-mode basis of the flow: U_host (P*P, the circulant eigenvectors, row p = bead),
  * sigma_host (N*P, harmonic standard deviations per mode and normal mode) and m_host (N, the
@@ -231,11 +236,12 @@ int pib_flow_coupling_inverse_backward(pib_ctx* ctx, int64_t B, int D, int a_off
                                        void* stream);
 
 /* One whole coupling layer in one launch: the MLP of the layer (D/2 -> hidden -> hidden -> D,
- * ReLU) as BF16x3 products on weights already split and transposed (pib_split_bf16 with
- * transpose: W1 as hidden x D/2, W2 as hidden x hidden, W3 as D x hidden, bf16 hi/lo), then
- * the coupling of pib_flow_coupling on z (B x D) and logdet. Same arithmetic as the
- * layer-by-layer calls (pib_gemm_bf16x3_presplit x3 + pib_flow_coupling); the activations and
- * the MLP output stay on chip. Supported: hidden 256, D/2 in {256, 512}, B multiple of 32. */
+ * ReLU) as BF16x3 products, then the coupling of pib_flow_coupling on z (B x D) and logdet.
+ * Weights split by pib_split_bf16: W1 with PIB_SPLIT_TRANSPOSE (hidden x D/2), W2 and W3 with
+ * PIB_SPLIT_TRANSPOSE | PIB_SPLIT_PERM32 (hidden x hidden, D x hidden). Same arithmetic as the
+ * layer-by-layer calls (pib_gemm_bf16x3_presplit x3 + pib_flow_coupling) up to the FP32
+ * summation order; the activations and the MLP output never leave the chip. Supported:
+ * hidden 256, D/2 in {256, 512}, B a multiple of 128. */
 int pib_flow_layer_fused(pib_ctx* ctx, int inverse, int64_t B, int D, int hidden, int a_off,
                          int b_off, float smax, float* z_dev, double* logdet_dev,
                          const uint16_t* W1_hi_dev, const uint16_t* W1_lo_dev,

@@ -22,6 +22,7 @@ PIB_ERR_UNSUPPORTED = -4
 SET_RHO, SET_VIB, SET_VIB_PLUS, SET_VIB_MINUS = 0, 1, 2, 3
 EVAL_PM, EVAL_NOSCALE, EVAL_WIDE, EVAL_FP32, EVAL_BF16X3 = 1, 2, 4, 8, 16
 BATCH_LANES, BATCH_WIDE, BATCH_SEQUENTIAL = 0, 1, 2
+SPLIT_TRANSPOSE, SPLIT_PERM32 = 1, 2
 OUT_RHO, OUT_G, OUT_GP, OUT_GM, OUT_LOGS = 0, 1, 2, 3, 4
 NUM_OUT = 5
 (SUM_N, SUM_R, SUM_R2, SUM_ABSR, SUM_NEG, SUM_D1, SUM_D1SQ, SUM_D2, SUM_D2SQ,
@@ -241,14 +242,15 @@ class Context:
             _devptr(Bt_lo), int(ldb), _devptr(C), int(ldc), _devptr(O_hi), _devptr(O_lo), int(ldo),
             _devptr(bias), int(bool(relu)), _stream(stream)))
 
-    def split_bf16(self, rows, cols, X, ldx, hi, lo, ldo, transpose=False, stream=None):
+    def split_bf16(self, rows, cols, X, ldx, hi, lo, ldo, transpose=False, stream=None, perm32=False):
+        mode = (SPLIT_TRANSPOSE if transpose else 0) | (SPLIT_PERM32 if perm32 else 0)
         self._check(self.lib.pib_split_bf16(self.h, int(rows), int(cols), _devptr(X), int(ldx),
-                                            int(bool(transpose)), _devptr(hi), _devptr(lo), int(ldo),
-                                            _stream(stream)))
+                                            mode, _devptr(hi), _devptr(lo), int(ldo), _stream(stream)))
 
     def flow_layer_fused(self, inverse, B, D, hidden, a_off, b_off, smax, z, logdet, w, b1, b2, b3,
                          stream=None):
-        """w = ((W1h, W1l), (W2h, W2l), (W3h, W3l)) pre-split transposed weights."""
+        """w = ((W1h, W1l), (W2h, W2l), (W3h, W3l)) pre-split transposed weights, W2 and W3
+        split with perm32=True."""
         (w1h, w1l), (w2h, w2l), (w3h, w3l) = w
         self._check(self.lib.pib_flow_layer_fused(
             self.h, int(bool(inverse)), int(B), int(D), int(hidden), int(a_off), int(b_off), float(smax),

@@ -592,13 +592,18 @@ int pib_gemm_bf16x3_presplit(pib_ctx* c, int64_t M, int64_t N, int64_t K, const 
   return PIB_OK;
 }
 
-int pib_split_bf16(pib_ctx* c, int64_t rows, int64_t cols, const float* X, int64_t ldx, int transpose,
+int pib_split_bf16(pib_ctx* c, int64_t rows, int64_t cols, const float* X, int64_t ldx, int mode,
                    uint16_t* hi, uint16_t* lo, int64_t ldo, void* stream) {
   if (!c) return PIB_ERR_ARG;
-  if (rows < 1 || cols < 1 || !X || !hi || !lo || ldx < cols || ldo < (transpose ? rows : cols))
+  if (mode & ~(PIB_SPLIT_TRANSPOSE | PIB_SPLIT_PERM32))
+    return fail(c, PIB_ERR_ARG, "split_bf16: unknown mode bit");
+  if ((mode & PIB_SPLIT_PERM32) && (!(mode & PIB_SPLIT_TRANSPOSE) || rows % 32))
+    return fail(c, PIB_ERR_ARG, "split_bf16: PIB_SPLIT_PERM32 needs PIB_SPLIT_TRANSPOSE and rows a multiple of 32");
+  const bool tr = (mode & PIB_SPLIT_TRANSPOSE) != 0;
+  if (rows < 1 || cols < 1 || !X || !hi || !lo || ldx < cols || ldo < (tr ? rows : cols))
     return fail(c, PIB_ERR_ARG, "split_bf16: bad size, leading dimension or null pointer");
   PIB_HIP(c, hipSetDevice(c->device));
-  hipError_t e = launch_split_bf16(rows, cols, X, ldx, transpose ? 1 : 0, hi, lo, ldo, (hipStream_t)stream);
+  hipError_t e = launch_split_bf16(rows, cols, X, ldx, mode, hi, lo, ldo, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "split_bf16 launch");
   return PIB_OK;
 }

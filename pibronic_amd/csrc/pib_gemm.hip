@@ -497,13 +497,23 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3_pre(GemmPreArgs g) {
 }
 
 // X (rows x cols, ldx) -> hi/lo bf16; transpose: out[c][r] (cols x rows, row length rows)
-__global__ void k_split_bf16(int64_t rows, int64_t cols, const float* X, int64_t ldx, int transpose,
+// Position of input index k in a row permuted for register-resident products: within each
+// block of 32, index 32b + w goes to 32b + 8((w % 16) / 4) + 4(w / 16) + w % 4 (the order in
+// which two 16x16 accumulator tiles hand their rows to the next product's operand).
+__device__ __forceinline__ int64_t perm32(int64_t k) {
+  const int64_t b = k & ~(int64_t)31;
+  const int w = (int)(k & 31);
+  return b + 8 * ((w & 15) >> 2) + 4 * (w >> 4) + (w & 3);
+}
+
+__global__ void k_split_bf16(int64_t rows, int64_t cols, const float* X, int64_t ldx, int mode,
                              unsigned short* hi, unsigned short* lo, int64_t ldo) {
+  const bool transpose = (mode & 1) != 0, perm = (mode & 2) != 0;
   for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += (int64_t)gridDim.x * blockDim.x) {
       const float x = X[r * ldx + c];
       const unsigned short h = f2bf(x);
-      const int64_t o = transpose ? c * ldo + r : r * ldo + c;
+      const int64_t o = transpose ? c * ldo + (perm ? perm32(r) : r) : r * ldo + c;
       hi[o] = h;
       lo[o] = f2bf(x - bf2f(h));
     }
@@ -525,6 +535,7 @@ hipError_t launch_gemm_bf16x3_pre(const GemmPreArgs& g, hipStream_t s) {
 
 hipError_t launch_split_bf16(int64_t rows, int64_t cols, const float* X, int64_t ldx, int transpose,
                              unsigned short* hi, unsigned short* lo, int64_t ldo, hipStream_t s) {
+  if ((transpose & ~3) || ((transpose & 2) && (!(transpose & 1) || rows % 32))) return hipErrorInvalidValue;
   const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((cols + 255) / 256, 64)),
                   (unsigned)std::max<int64_t>(1, std::min<int64_t>(rows, 65535)));
   hipLaunchKernelGGL(k_split_bf16, grid, dim3(256), 0, s, rows, cols, X, ldx, transpose, hi, lo, ldo);

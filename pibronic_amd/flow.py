@@ -74,8 +74,9 @@ class Flow:
         self.step_count = 0
         self._bufs = {}
         self._version = 0          # bumped whenever the parameters change
-        self.use_fused = False     # True: one launch per coupling layer in bf16x3 draws where supported
+        self.use_fused = True      # bf16x3 draws and densities: one launch per coupling layer where supported
         self._split_version = -1   # parameter version of the pre-split bf16 weights
+        self._fsplit_version = -1  # parameter version of the fused layer's permuted weights
 
     # ------------------------------------------------------------------ layout helpers
     def _offset(self, layer, name):
@@ -151,6 +152,26 @@ class Flow:
             self._wsplit.append(mats)
         self._split_version = self._version
 
+    def _split_weights_fused(self, stream):
+        """The fused layer's operands: W1 as for the pre-split products, W2 and W3 split with
+        their input index permuted within blocks of 32 (PIB_SPLIT_PERM32)."""
+        import torch
+        self._split_weights(stream)
+        if self._fsplit_version == self._version:
+            return
+        h, D = self.hidden, self.D
+        self._wfused = []
+        for l in range(self.layers):
+            mats = [self._wsplit[l][0]]
+            for name, (K, Nn) in (("W2", (h, h)), ("W3", (h, D))):
+                hi = self._buf(f"{name}ph{l}", (Nn, K), torch.int16)
+                lo = self._buf(f"{name}pl{l}", (Nn, K), torch.int16)
+                self.ctx.split_bf16(K, Nn, self.p(l, name), Nn, hi, lo, K, transpose=True, perm32=True,
+                                    stream=stream)
+                mats.append((hi, lo))
+            self._wfused.append(mats)
+        self._fsplit_version = self._version
+
     def _mlp_fast(self, layer, zp, B, o, stream):
         """The coupling network on pre-split operands: the conditioning half of zp is split
         once, the hidden activations travel as bf16 hi/lo written by the previous product."""
@@ -171,13 +192,14 @@ class Flow:
                                  bias=self.p(layer, "b3"), stream=stream)
 
     def _fused_ok(self, B):
-        return self.use_fused and self.hidden == 256 and self.H in (256, 512) and B % 32 == 0
+        return self.use_fused and self.hidden == 256 and self.H in (256, 512) and B % 128 == 0
 
     def _layer_fused(self, inverse, layer, zp, B, logdet, stream):
         """The whole coupling layer in one launch (pib_flow_layer_fused)."""
         a_off, b_off = self.halves(layer)
+        self._split_weights_fused(stream)
         self.ctx.flow_layer_fused(inverse, B, self.D, self.hidden, a_off, b_off, self.smax, zp, logdet,
-                                  self._wsplit[layer], self.p(layer, "b1"), self.p(layer, "b2"),
+                                  self._wfused[layer], self.p(layer, "b1"), self.p(layer, "b2"),
                                   self.p(layer, "b3"), stream)
 
     # ------------------------------------------------------------------ directions

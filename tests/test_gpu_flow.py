@@ -128,6 +128,31 @@ def test_gemm_bf16x3_unaligned_operand(tmp_path):
     assert torch.equal(C1, C2)
 
 
+def test_split_perm32_layout(tmp_path):
+    """PIB_SPLIT_PERM32: input 32b + w of every output row sits at 32b + 8((w%16)/4) +
+    4(w/16) + w%4, values identical to the plain transposed split; bad modes are refused."""
+    _announce("split perm32")
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, 2, quadratic=False, seed=0))
+    ctx = make_data(FS, 300.0, 64, 10, 10).context()
+    K, N = 96, 48
+    W = torch.randn((K, N), generator=torch.Generator().manual_seed(3)).to(DEV)
+    th, tl = torch.empty((N, K), dtype=torch.int16, device=DEV), torch.empty((N, K), dtype=torch.int16, device=DEV)
+    ph, pl = torch.empty_like(th), torch.empty_like(tl)
+    ctx.split_bf16(K, N, W, N, th, tl, K, transpose=True)
+    ctx.split_bf16(K, N, W, N, ph, pl, K, transpose=True, perm32=True)
+    torch.cuda.synchronize()
+    k = np.arange(K)
+    w = k % 32
+    pos = (k - w) + 8 * ((w % 16) // 4) + 4 * (w // 16) + w % 4
+    assert sorted(pos.tolist()) == list(range(K))
+    assert torch.equal(ph[:, torch.as_tensor(pos, device=DEV)], th)
+    assert torch.equal(pl[:, torch.as_tensor(pos, device=DEV)], tl)
+    with pytest.raises(_native.NativeError, match="PERM32"):
+        ctx.split_bf16(K, N, W, N, ph, pl, K, transpose=False, perm32=True)
+    with pytest.raises(_native.NativeError, match="PERM32"):
+        ctx.split_bf16(40, N, W, N, ph, pl, K, transpose=True, perm32=True)
+
+
 def test_gemm_presplit_matches_converting(tmp_path):
     """Pre-split operands give bit-for-bit the converting kernel's result (same splits, same
     product order), and the split output equals splitting the FP32 output."""
@@ -314,23 +339,52 @@ def _forward_slow(fl, R0):
 
 @pytest.mark.parametrize("N,P", [(4, 128), (8, 128)])
 def test_fused_layer_matches_layer_by_layer(tmp_path, N, P):
-    """One-launch coupling layers (D/2 = 256 and 512) give the same R bit for bit as the
-    layer-by-layer pre-split path, log q to summation order, in both directions."""
+    """One-launch coupling layers (D/2 = 256 and 512) against the layer-by-layer pre-split
+    path. The fused layer sums the same BF16x3 products in another order (transposed tiles,
+    permuted inputs of the second and third products), so per layer z agrees to FP32 rounding
+    (measured 1.5e-6 of max|z|) and log|det| to the sum of those roundings over D/2 features;
+    the fused forward-inverse round trip of a layer returns its input to FP32 rounding. Through
+    the whole 4-layer flow (random weights, smax = 2, which amplify rounding ~1e4-fold in the
+    inverse direction) the forward R agrees to 8e-6 of max|R|."""
     _announce(f"fused coupling layer D={N * P}")
     FS = stage_dict_models(tmp_path, synthetic.generate(2, N, quadratic=False, seed=2))
     data = make_data(FS, 300.0, P, 64, 64)
     fl = flow.Flow(data, layers=4, hidden=256, gemm="bf16x3")
     ref = ref_for(data, 4, 256, 2.0)
     fl.set_params(random_params(ref, 5, 0.05))
-    q = torch.as_tensor(_host_q(data, 128, 3), device=DEV)
+    B, D = 256, fl.D
+    fl._split_weights(None)
+    z0 = torch.randn((B, D), generator=torch.Generator().manual_seed(1)).to(DEV)
+    o = torch.empty((B, D), device=DEV)
+    for l in range(2):
+        a_off, b_off = fl.halves(l)
+        for inverse in (False, True):
+            z1, l1 = z0.clone(), torch.zeros(B, dtype=torch.float64, device=DEV)
+            z2, l2 = z0.clone(), torch.zeros(B, dtype=torch.float64, device=DEV)
+            fl._layer_fused(inverse, l, z1, B, l1, None)
+            fl._mlp_fast(l, z2, B, o, None)
+            fl.ctx.flow_coupling(inverse, B, D, a_off, b_off, fl.smax, z2, o, l2, None)
+            torch.cuda.synchronize()
+            scale = float(z2.abs().max())
+            dz, dl = float((z1 - z2).abs().max()), float((l1 - l2).abs().max())
+            print(f"layer {l} inverse={inverse}: max |dz| {dz:.3e} (|z| <= {scale:.1f}), |d logdet| {dl:.3e}")
+            assert torch.equal(z1[:, a_off:a_off + fl.H], z0[:, a_off:a_off + fl.H])
+            assert dz <= 1e-5 * scale and dl <= 5e-4
+            if not inverse:
+                fl._layer_fused(True, l, z1, B, l1, None)
+                torch.cuda.synchronize()
+                assert float((z1 - z0).abs().max()) <= 1e-6 * scale
+                assert float(l1.abs().max()) <= 1e-4
+    q = torch.as_tensor(_host_q(data, B, 3), device=DEV)
+    fl.use_fused = True
     R1, ld1 = fl.forward(q)
-    lq1, R01 = fl.log_q(R1)
     fl.use_fused = False
     R2, ld2 = fl.forward(q)
-    lq2, R02 = fl.log_q(R2)
     torch.cuda.synchronize()
-    assert torch.equal(R1, R2) and torch.equal(R01, R02)
-    assert torch.allclose(ld1, ld2, rtol=0, atol=1e-9) and torch.allclose(lq1, lq2, rtol=0, atol=1e-9)
+    scale = float(R2.abs().max())
+    dR, dld = float((R1 - R2).abs().max()), float((ld1 - ld2).abs().max())
+    print(f"flow: max |dR| {dR:.3e} (|R| <= {scale:.2f}), |d logdet| {dld:.3e}")
+    assert dR <= 5e-5 * scale and dld <= 1e-2
 
 
 def test_identity_flow_reproduces_harmonic_weights(model):
