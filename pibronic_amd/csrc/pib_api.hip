@@ -670,6 +670,11 @@ int pib_flow_whiten(pib_ctx* c, int mode, int64_t B, const void* in, float* work
   PIB_HIP(c, hipSetDevice(c->device));
   const float* U = (const float*)c->f_U.p;
   hipError_t e;
+  if (whiten_fused_supported(P)) {
+    const int m = mode == PIB_FLOW_WHITEN ? FLOW_WHITEN : mode == PIB_FLOW_UNWHITEN ? FLOW_UNWHITEN : FLOW_GRAD;
+    if ((e = launch_whiten_fused(m, ba, in, out, U, c->num_cu, s)) != hipSuccess) return hip_fail(c, e, "flow whiten");
+    return PIB_OK;
+  }
   if (mode == PIB_FLOW_UNWHITEN) {
     if ((e = launch_flow_pre(FLOW_UNWHITEN, ba, in, Y, s)) != hipSuccess) return hip_fail(c, e, "flow pre");
     // X = Y U^T : B operand U transposed (B(k, p) = U[p][k])

@@ -76,6 +76,148 @@ hipError_t launch_flow_post(int mode, const BasisArgs& b, const float* in, void*
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ fused whitening
+// One kernel for each direction of the whitening transform: a work-group stages the P x P basis
+// (U, or U^T for the inverse, padded rows) in LDS once; then each wave walks blocks of 16 rows
+// (sample, mode) of its own: their input is gathered into the wave's LDS rows (R - m_j, or the
+// un-permuted zp * sigma), the product with the basis runs on the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32, 16 rows x P columns), and the result goes back through LDS to be
+// written with row-contiguous stores (the permuted even/odd halves, or R = X + m_j in double). Same arithmetic as pre + GEMM +
+// post up to the FP32 summation order.
+typedef float f32x4w __attribute__((ext_vector_type(4)));
+
+// LDS traffic of one wave ordered: every access before is complete and visible to every lane of
+// the same wave after it
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int P>
+struct WhitenLds {
+  static constexpr int ldu = P + 16;  // basis rows 16 floats apart mod 64 banks
+  static constexpr int ldx = P + 4;   // row tiles 4 floats apart
+  static constexpr size_t bytes = sizeof(float) * ((size_t)P * ldu + 2 * 64 * (size_t)ldx);
+};
+
+template <int P, int MODE>
+__global__ void __launch_bounds__(256) k_whiten(BasisArgs b, const void* in, void* out, const float* U) {
+  using L = WhitenLds<P>;
+  extern __shared__ __align__(16) float wsm[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: row indices in scalars
+  float* Us = wsm;                                  // [P][ldu]: B operand rows k, columns c
+  float* Xs = wsm + P * L::ldu + wave * 16 * L::ldx;  // this wave's 16 input rows
+  float* Ys = Xs + 64 * L::ldx;                     // this wave's 16 output rows
+  const int N = b.N, NP = N * P, Ph = P / 2, H = NP / 2;
+  constexpr bool inv = MODE == FLOW_UNWHITEN;
+  for (int e = tid; e < P * P; e += 256) {
+    const int r = e / P, c = e - r * P;
+    if (inv) Us[c * L::ldu + r] = U[e];  // B[k][c] = U[c][k]
+    else Us[r * L::ldu + c] = U[e];      // B[k][c] = U[k][c]
+  }
+  __syncthreads();
+  // each wave owns 16 rows (sample s, mode j) per step: it stages, multiplies and writes only its
+  // own rows, so the loop needs no work-group barrier; (s, j) advance without divisions
+  const int64_t rows = b.B * (int64_t)N;
+  for (int64_t rw = (int64_t)blockIdx.x * 4 + wave; rw * 16 < rows; rw += (int64_t)gridDim.x * 4) {
+    const int64_t gr0 = rw * 16, s0 = gr0 / N;
+    const int j0 = (int)(gr0 - s0 * N);
+    int64_t s = s0;
+    int j = j0;
+    for (int i = 0; i < 16; ++i) {
+      const int64_t gr = gr0 + i;
+#pragma unroll
+      for (int o = lane; o < P; o += 64) {
+        if (MODE == FLOW_WHITEN) {
+          Xs[i * L::ldx + o] = (float)(((const double*)in)[gr * P + o] - b.m[j]);
+        } else if (MODE == FLOW_GRAD) {
+          Xs[i * L::ldx + o] = ((const float*)in)[gr * P + o];
+        } else {  // o runs over (half h, kk); k = 2 kk + h
+          const int h = o / Ph, kk = o - h * Ph, k = 2 * kk + h;
+          Xs[i * L::ldx + k] = ((const float*)in)[s * NP + h * H + j * Ph + kk] * b.sigma[j * P + k];
+        }
+      }
+      if (++j == N) {
+        j = 0;
+        ++s;
+      }
+    }
+    wave_lds_sync();
+    // one column tile at a time, each a single in-place accumulator chain written to Ys at
+    // once (keeping all P/16 tiles live let the compiler rename accumulators between unrolled
+    // steps with a destination that partly overlaps the C operand, which gave wrong sums)
+#pragma unroll 1
+    for (int c = 0; c < P / 16; ++c) {
+      f32x4w acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k0 = 0; k0 < P; k0 += 4) {
+        const float a = Xs[(lane & 15) * L::ldx + k0 + (lane >> 4)];
+        const float bb = Us[(k0 + (lane >> 4)) * L::ldu + 16 * c + (lane & 15)];
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Ys[(4 * (lane >> 4) + r) * L::ldx + 16 * c + (lane & 15)] = acc[r];
+    }
+    wave_lds_sync();
+    s = s0;
+    j = j0;
+    for (int i = 0; i < 16; ++i) {
+      const int64_t gr = gr0 + i;
+#pragma unroll
+      for (int o = lane; o < P; o += 64) {
+        if (inv) {
+          ((double*)out)[gr * P + o] = (double)Ys[i * L::ldx + o] + b.m[j];
+        } else {
+          const int h = o / Ph, kk = o - h * Ph, k = 2 * kk + h;
+          const float f = (MODE == FLOW_WHITEN) ? b.inv_sigma[j * P + k] : b.sigma[j * P + k];
+          ((float*)out)[s * NP + h * H + j * Ph + kk] = Ys[i * L::ldx + k] * f;
+        }
+      }
+      if (++j == N) {
+        j = 0;
+        ++s;
+      }
+    }
+    wave_lds_sync();  // the next step overwrites this wave's rows
+  }
+}
+
+bool whiten_fused_supported(int P) { return P == 64 || P == 128; }
+
+template <int P, int MODE>
+static hipError_t launch_whiten_pm(const BasisArgs& b, const void* in, void* out, const float* U, int num_cu,
+                                   hipStream_t s) {
+  const size_t lds = WhitenLds<P>::bytes;
+  static bool attr_set = false;  // more than the default 64 KB of dynamic LDS when P = 128
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_whiten<P, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  const int64_t blocks = (b.B * (int64_t)b.N + 63) / 64;  // 4 waves x 16 rows
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)4 * num_cu));
+  hipLaunchKernelGGL((k_whiten<P, MODE>), dim3((unsigned)grid), dim3(256), lds, s, b, in, out, U);
+  return hipGetLastError();
+}
+
+template <int P>
+static hipError_t launch_whiten_p(int mode, const BasisArgs& b, const void* in, void* out, const float* U,
+                                  int num_cu, hipStream_t s) {
+  if (mode == FLOW_WHITEN) return launch_whiten_pm<P, FLOW_WHITEN>(b, in, out, U, num_cu, s);
+  if (mode == FLOW_UNWHITEN) return launch_whiten_pm<P, FLOW_UNWHITEN>(b, in, out, U, num_cu, s);
+  return launch_whiten_pm<P, FLOW_GRAD>(b, in, out, U, num_cu, s);
+}
+
+hipError_t launch_whiten_fused(int mode, const BasisArgs& b, const void* in, void* out, const float* U,
+                               int num_cu, hipStream_t s) {
+  if ((b.B * (int64_t)b.N) % 16) return hipErrorInvalidValue;
+  if (b.P == 64) return launch_whiten_p<64>(mode, b, in, out, U, num_cu, s);
+  if (b.P == 128) return launch_whiten_p<128>(mode, b, in, out, U, num_cu, s);
+  return hipErrorInvalidValue;
+}
+
 // ------------------------------------------------------------------ block reduction helper
 
 // ------------------------------------------------------------------ coupling layers

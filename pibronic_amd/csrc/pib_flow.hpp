@@ -65,6 +65,10 @@ enum : int {
 
 hipError_t launch_flow_pre(int mode, const BasisArgs& b, const void* in, float* out, hipStream_t s);
 hipError_t launch_flow_post(int mode, const BasisArgs& b, const float* in, void* out, hipStream_t s);
+// the whole transform (pre, product with U or U^T, post) in one kernel, for P in {64, 128}
+bool whiten_fused_supported(int P);
+hipError_t launch_whiten_fused(int mode, const BasisArgs& b, const void* in, void* out, const float* U,
+                               int num_cu, hipStream_t s);
 
 struct CouplingArgs {
   int64_t B;

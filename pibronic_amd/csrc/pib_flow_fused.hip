@@ -12,10 +12,14 @@
 // in the matching order (pib_split_bf16 with PIB_SPLIT_PERM32 stores input 32c + w at
 // 32c + 8((w%16)/4) + 4(w/16) + w%4). The hidden activations never leave the registers.
 //
-// Work-group: 8 waves, 16 samples each (128 samples). Weight tiles (16 output rows x K, hi
-// and lo) are staged through a double-buffered LDS ring shared by the 8 waves; the next tile
-// is fetched into registers while the current one is multiplied. The last product is taken
-// tile pair by tile pair (raw scales s and shifts t of the same 16 features), and its
+// Work-group: 128 samples; each wave holds kNT MFMA N tiles of 16 samples. kNT = 1 (eight waves,
+// two per SIMD) measured faster than kNT = 2 (four waves with the whole register file each:
+// 2.79 vs 2.09 ms per draw of 65536 samples at D = 512): the second wave per SIMD hides more
+// LDS and L2 latency than the doubled reuse of each weight read saves. Weight tiles (16 output rows x K, hi
+// and lo) are staged two at a time through a double-buffered LDS ring shared by the 8 waves;
+// the next pair is fetched into registers while the current one is multiplied. The last
+// product is taken tile pair by tile pair (raw scales s and shifts t of the same 16
+// features), and its
 // epilogue applies the coupling -- s = smax tanh(raw / smax), y = z e^s + t (or
 // z = (y - t) e^-s) -- to the lane's four features of its sample and accumulates sum s.
 #include <hip/hip_runtime.h>
@@ -31,9 +35,9 @@ typedef float f32x4f __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8f __attribute__((ext_vector_type(8)));
 typedef unsigned uvec4f __attribute__((ext_vector_type(4)));
 
-constexpr int kWaves = 8;               // waves per work-group
-constexpr int kSamplesPerWave = 16;     // one MFMA N tile
-constexpr int kRows = kWaves * kSamplesPerWave;
+constexpr int kNT = 1;                 // MFMA N tiles (16 samples each) per wave
+constexpr int kWaves = 8 / kNT;         // waves per work-group
+constexpr int kRows = kWaves * 16 * kNT;  // samples per work-group
 constexpr int kThreads = kWaves * 64;
 constexpr int kPadF = 8;                // LDS row padding (bf16 elements): rows 16 bytes apart mod 256
 
@@ -67,190 +71,234 @@ __device__ __forceinline__ f32x4f mfma3(const bf16x8f& ah, const bf16x8f& al, co
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
 }
 
-// The tile sequence of one layer: the HID/16 row tiles of W1^T (K = H), the HID/16 row tiles
-// of W2^T (K = HID), then for q = 0..H/16-1 the rows 16q.. (scales) and H + 16q.. (shifts)
+// The stage sequence of one layer; a stage is two 16-row weight tiles (hi and lo), staged in
+// LDS together behind one barrier: the HID/32 row pairs of W1^T (K = H), the HID/32 row pairs
+// of W2^T (K = HID), then for q = 0..H/16-1 the rows 16q.. (scales) with H + 16q.. (shifts)
 // of W3^T (K = HID).
 template <int H, int HID>
-struct Tiles {
-  static constexpr int n1 = HID / 16, n2 = HID / 16, n3 = 2 * (H / 16);
+struct Stages {
+  static constexpr int n1 = HID / 32, n2 = HID / 32, n3 = H / 16;
   static constexpr int total = n1 + n2 + n3;
   static constexpr int kmax = H > HID ? H : HID;
   static constexpr int ld = kmax + kPadF;      // LDS row stride (elements)
   static constexpr int tile = 16 * ld;         // elements of one hi (or lo) tile
-  static constexpr int units = 2 * 16 * kmax / 8 / kThreads;  // 16-byte loads per thread per tile
-  __device__ static constexpr int K(int t) { return t < n1 ? H : HID; }
+  static constexpr int stage = 4 * tile;       // [tile 0 hi, tile 0 lo, tile 1 hi, tile 1 lo]
+  static constexpr int units = 2 * 2 * 16 * kmax / 8 / kThreads;  // 16-byte loads per thread
+  __device__ static constexpr int K(int s) { return s < n1 ? H : HID; }
 };
 
+// source of stage s: weight arrays and the first rows of its two tiles
 template <int H, int HID>
-__device__ __forceinline__ void tile_source(const FusedLayerArgs& a, int t, const unsigned short*& wh,
-                                            const unsigned short*& wl, int& row0) {
-  using T = Tiles<H, HID>;
-  if (t < T::n1) {
+__device__ __forceinline__ void stage_source(const FusedLayerArgs& a, int s, const unsigned short*& wh,
+                                             const unsigned short*& wl, int& rowA, int& rowB) {
+  using S = Stages<H, HID>;
+  if (s < S::n1) {
     wh = a.W1h;
     wl = a.W1l;
-    row0 = 16 * t;
-  } else if (t < T::n1 + T::n2) {
+    rowA = 32 * s;
+    rowB = rowA + 16;
+  } else if (s < S::n1 + S::n2) {
     wh = a.W2h;
     wl = a.W2l;
-    row0 = 16 * (t - T::n1);
+    rowA = 32 * (s - S::n1);
+    rowB = rowA + 16;
   } else {
-    const int u = t - T::n1 - T::n2;
+    const int q = s - S::n1 - S::n2;
     wh = a.W3h;
     wl = a.W3l;
-    row0 = (u & 1) ? H + 16 * (u >> 1) : 16 * (u >> 1);
+    rowA = 16 * q;
+    rowB = H + 16 * q;
   }
 }
 
-// Fetch tile t into registers: unit i of this thread is 16-byte unit tid + kThreads * i of the
-// tile (hi units first, then lo); units beyond the tile's K are not loaded.
+// Unit i of this thread is 16-byte unit tid + kThreads * i of the stage, ordered
+// (tile, hi/lo, row, 8-element column group); units beyond the stage's K are skipped.
 template <int H, int HID>
-__device__ __forceinline__ void fetch(const FusedLayerArgs& a, int t, int tid, uvec4f (&pre)[Tiles<H, HID>::units]) {
-  using T = Tiles<H, HID>;
+__device__ __forceinline__ void fetch(const FusedLayerArgs& a, int s, int tid, uvec4f (&pre)[Stages<H, HID>::units]) {
+  using S = Stages<H, HID>;
   const unsigned short *wh, *wl;
-  int row0;
-  tile_source<H, HID>(a, t, wh, wl, row0);
-  const int K = T::K(t), per = 16 * K / 8;  // units per array
+  int rowA, rowB;
+  stage_source<H, HID>(a, s, wh, wl, rowA, rowB);
+  const int K = S::K(s), per = 16 * K / 8;  // units per (tile, hi/lo)
 #pragma unroll
-  for (int i = 0; i < T::units; ++i) {
+  for (int i = 0; i < S::units; ++i) {
     const int u = tid + kThreads * i;
-    if (u < 2 * per) {
-      const int arr = u >= per, r = u - arr * per;
+    if (u < 4 * per) {
+      const int blk = u / per, r = u - blk * per;  // blk = 2 * tile + (lo ? 1 : 0)
       const int row = r / (K / 8), c8 = r - row * (K / 8);
-      const unsigned short* src = (arr ? wl : wh) + (int64_t)(row0 + row) * K + 8 * c8;
+      const unsigned short* src = ((blk & 1) ? wl : wh) + (int64_t)(((blk >> 1) ? rowB : rowA) + row) * K + 8 * c8;
       pre[i] = *(const uvec4f*)src;
     }
   }
 }
 
 template <int H, int HID>
-__device__ __forceinline__ void store(int t, int tid, const uvec4f (&pre)[Tiles<H, HID>::units], unsigned short* buf) {
-  using T = Tiles<H, HID>;
-  const int K = T::K(t), per = 16 * K / 8;
+__device__ __forceinline__ void store(int s, int tid, const uvec4f (&pre)[Stages<H, HID>::units], unsigned short* buf) {
+  using S = Stages<H, HID>;
+  const int K = S::K(s), per = 16 * K / 8;
 #pragma unroll
-  for (int i = 0; i < T::units; ++i) {
+  for (int i = 0; i < S::units; ++i) {
     const int u = tid + kThreads * i;
-    if (u < 2 * per) {
-      const int arr = u >= per, r = u - arr * per;
+    if (u < 4 * per) {
+      const int blk = u / per, r = u - blk * per;
       const int row = r / (K / 8), c8 = r - row * (K / 8);
-      *(uvec4f*)(buf + arr * T::tile + row * T::ld + 8 * c8) = pre[i];
+      *(uvec4f*)(buf + blk * S::tile + row * S::ld + 8 * c8) = pre[i];
     }
   }
 }
 
-// acc += W^T tile (LDS, 16 rows x K) * X^T (operands in registers, K/32 blocks)
+// W^T tile (LDS, 16 rows x K, lo part tile_elems after hi) times X^T for each of the wave's
+// kNT sample tiles (operands in registers): one LDS read of a weight block feeds 3 kNT MFMAs
 template <int NB, int LD>
-__device__ __forceinline__ f32x4f tile_product(const unsigned short* buf, const Op8 (&bh)[NB],
-                                               const Op8 (&bl)[NB], int li, int g, int tile_elems) {
-  f32x4f acc = {0.f, 0.f, 0.f, 0.f};
+__device__ __forceinline__ void tile_product(const unsigned short* buf, const Op8 (&bh)[kNT][NB],
+                                             const Op8 (&bl)[kNT][NB], int li, int g, int tile_elems,
+                                             f32x4f (&acc)[kNT]) {
+#pragma unroll
+  for (int j = 0; j < kNT; ++j) acc[j] = f32x4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int c = 0; c < NB; ++c) {
     const unsigned short* p = buf + li * LD + 32 * c + 8 * g;
     const bf16x8f ah = __builtin_bit_cast(bf16x8f, *(const uvec4f*)p);
     const bf16x8f al = __builtin_bit_cast(bf16x8f, *(const uvec4f*)(p + tile_elems));
-    acc = mfma3(ah, al, as_bf(bh[c]), as_bf(bl[c]), acc);
+#pragma unroll
+    for (int j = 0; j < kNT; ++j) acc[j] = mfma3(ah, al, as_bf(bh[j][c]), as_bf(bl[j][c]), acc[j]);
   }
-  return acc;
 }
 
 template <int H, int HID>
 __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
-  using T = Tiles<H, HID>;
+  using S = Stages<H, HID>;
   constexpr int NBX = H / 32, NBH = HID / 32;
-  extern __shared__ __align__(16) unsigned short lds[];  // [2 buffers][hi, lo][16][ld]
+  extern __shared__ __align__(16) unsigned short lds[];  // [2 buffers][stage]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, g = lane >> 4;
   const int D = a.D;
-  const int64_t n = (int64_t)blockIdx.x * kRows + wave * kSamplesPerWave + li;  // this lane's sample
-  float* zrow = a.z + n * D;
+  // this lane's samples: one per sample tile j
+  const int64_t n0 = (int64_t)blockIdx.x * kRows + wave * 16 * kNT + li;
+  float* zrow[kNT];
+#pragma unroll
+  for (int j = 0; j < kNT; ++j) zrow[j] = a.z + (n0 + 16 * j) * D;
 
   // conditioning half as the first product's B operand: block c, elements 32c + 8g + e
-  Op8 xh[NBX], xl[NBX];
+  Op8 xh[kNT][NBX], xl[kNT][NBX];
 #pragma unroll
-  for (int c = 0; c < NBX; ++c) {
-    const float4* src = (const float4*)(zrow + a.a_off + 32 * c + 8 * g);
-    const float4 v0 = src[0], v1 = src[1];
-    const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+  for (int j = 0; j < kNT; ++j)
 #pragma unroll
-    for (int e = 0; e < 8; ++e) split4(v[e], e, xh[c], xl[c]);
-  }
+    for (int c = 0; c < NBX; ++c) {
+      const float4* src = (const float4*)(zrow[j] + a.a_off + 32 * c + 8 * g);
+      const float4 v0 = src[0], v1 = src[1];
+      const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+      for (int e = 0; e < 8; ++e) split4(v[e], e, xh[j][c], xl[j][c]);
+    }
 
-  Op8 h1h[NBH], h1l[NBH], h2h[NBH], h2l[NBH];
-  double part = 0.0;
+  Op8 h1h[kNT][NBH], h1l[kNT][NBH], h2h[kNT][NBH], h2l[kNT][NBH];
+  double part[kNT];
+#pragma unroll
+  for (int j = 0; j < kNT; ++j) part[j] = 0.0;
 
-  uvec4f pre[T::units];
+  uvec4f pre[S::units];
   fetch<H, HID>(a, 0, tid, pre);
   store<H, HID>(0, tid, pre, lds);
-  // pipeline step around tile t: prefetch t+1 into registers, wait for tile t in LDS, then
-  // (after the caller's product) write t+1 into the other buffer
-#define PIB_TILE_BEGIN(t)                                         \
-  if ((t) + 1 < T::total) fetch<H, HID>(a, (t) + 1, tid, pre);    \
+  // pipeline step around stage s: prefetch s+1 into registers, wait for stage s in LDS, then
+  // (after the caller's products) write s+1 into the other buffer
+#define PIB_STAGE_BEGIN(s)                                        \
+  if ((s) + 1 < S::total) fetch<H, HID>(a, (s) + 1, tid, pre);    \
   __syncthreads();                                                \
-  const unsigned short* buf = lds + ((t) & 1) * 2 * T::tile;
-#define PIB_TILE_END(t) \
-  if ((t) + 1 < T::total) store<H, HID>((t) + 1, tid, pre, lds + (((t) + 1) & 1) * 2 * T::tile);
+  const unsigned short* buf = lds + ((s) & 1) * S::stage;
+#define PIB_STAGE_END(s) \
+  if ((s) + 1 < S::total) store<H, HID>((s) + 1, tid, pre, lds + (((s) + 1) & 1) * S::stage);
 
+  // first product: stage c gives output tiles 2c, 2c+1 = operand block c of the second
 #pragma clang loop unroll(full)
-  for (int t = 0; t < T::n1; ++t) {
-    PIB_TILE_BEGIN(t)
-    const f32x4f acc = tile_product<NBX, T::ld>(buf, xh, xl, li, g, T::tile);
+  for (int c = 0; c < S::n1; ++c) {
+    PIB_STAGE_BEGIN(c)
+    f32x4f acc0[kNT], acc1[kNT];
+    tile_product<NBX, S::ld>(buf, xh, xl, li, g, S::tile, acc0);
+    tile_product<NBX, S::ld>(buf + 2 * S::tile, xh, xl, li, g, S::tile, acc1);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float v = acc[r] + a.b1[16 * t + 4 * g + r];
-      split4(v > 0.f ? v : 0.f, 4 * (t & 1) + r, h1h[t >> 1], h1l[t >> 1]);
+      const float b0 = a.b1[32 * c + 4 * g + r], b1 = a.b1[32 * c + 16 + 4 * g + r];
+#pragma unroll
+      for (int j = 0; j < kNT; ++j) {
+        const float v0 = acc0[j][r] + b0, v1 = acc1[j][r] + b1;
+        split4(v0 > 0.f ? v0 : 0.f, r, h1h[j][c], h1l[j][c]);
+        split4(v1 > 0.f ? v1 : 0.f, 4 + r, h1h[j][c], h1l[j][c]);
+      }
     }
-    PIB_TILE_END(t)
+    PIB_STAGE_END(c)
   }
 #pragma clang loop unroll(full)
-  for (int u = 0; u < T::n2; ++u) {
-    const int t = T::n1 + u;
-    PIB_TILE_BEGIN(t)
-    const f32x4f acc = tile_product<NBH, T::ld>(buf, h1h, h1l, li, g, T::tile);
+  for (int c = 0; c < S::n2; ++c) {
+    const int s = S::n1 + c;
+    PIB_STAGE_BEGIN(s)
+    f32x4f acc0[kNT], acc1[kNT];
+    tile_product<NBH, S::ld>(buf, h1h, h1l, li, g, S::tile, acc0);
+    tile_product<NBH, S::ld>(buf + 2 * S::tile, h1h, h1l, li, g, S::tile, acc1);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float v = acc[r] + a.b2[16 * u + 4 * g + r];
-      split4(v > 0.f ? v : 0.f, 4 * (u & 1) + r, h2h[u >> 1], h2l[u >> 1]);
+      const float b0 = a.b2[32 * c + 4 * g + r], b1 = a.b2[32 * c + 16 + 4 * g + r];
+#pragma unroll
+      for (int j = 0; j < kNT; ++j) {
+        const float v0 = acc0[j][r] + b0, v1 = acc1[j][r] + b1;
+        split4(v0 > 0.f ? v0 : 0.f, r, h2h[j][c], h2l[j][c]);
+        split4(v1 > 0.f ? v1 : 0.f, 4 + r, h2h[j][c], h2l[j][c]);
+      }
     }
-    PIB_TILE_END(t)
+    PIB_STAGE_END(s)
   }
-  for (int q = 0; q < H / 16; ++q) {  // the operands h2 are indexed by constants only here
-    f32x4f sraw;
-    {
-      const int t = T::n1 + T::n2 + 2 * q;
-      PIB_TILE_BEGIN(t)
-      sraw = tile_product<NBH, T::ld>(buf, h2h, h2l, li, g, T::tile);
-      PIB_TILE_END(t)
+  // third product and coupling; the transformed half's z (four features per lane per q and
+  // sample tile) is loaded two q-steps ahead
+  float4 zq0[kNT], zq1[kNT];
+#pragma unroll
+  for (int j = 0; j < kNT; ++j) {
+    zq0[j] = *(const float4*)(zrow[j] + a.b_off + 4 * g);
+    zq1[j] = *(const float4*)(zrow[j] + a.b_off + 16 + 4 * g);
+  }
+  for (int q = 0; q < S::n3; ++q) {  // the operands h2 are indexed by constants only here
+    const int s = S::n1 + S::n2 + q;
+    float4 zv[kNT];
+#pragma unroll
+    for (int j = 0; j < kNT; ++j) {
+      zv[j] = zq0[j];
+      zq0[j] = zq1[j];
+      if (q + 2 < S::n3) zq1[j] = *(const float4*)(zrow[j] + a.b_off + 16 * (q + 2) + 4 * g);
     }
-    {
-      const int t = T::n1 + T::n2 + 2 * q + 1;
-      PIB_TILE_BEGIN(t)
-      const f32x4f acc = tile_product<NBH, T::ld>(buf, h2h, h2l, li, g, T::tile);
-      const int f0 = 16 * q + 4 * g;  // this lane's four features of the transformed half
-      float4* zp = (float4*)(zrow + a.b_off + f0);
-      const float4 zv = *zp;
-      float zz[4] = {zv.x, zv.y, zv.z, zv.w};
+    PIB_STAGE_BEGIN(s)
+    f32x4f sraw[kNT], acc[kNT];
+    tile_product<NBH, S::ld>(buf, h2h, h2l, li, g, S::tile, sraw);
+    tile_product<NBH, S::ld>(buf + 2 * S::tile, h2h, h2l, li, g, S::tile, acc);
+    const int f0 = 16 * q + 4 * g;  // this lane's four features of the transformed half
+#pragma unroll
+    for (int j = 0; j < kNT; ++j) {
+      float zz[4] = {zv[j].x, zv[j].y, zv[j].z, zv[j].w};
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float s = a.smax * tanhf((sraw[r] + a.b3[f0 + r]) / a.smax);
-        const float sh = acc[r] + a.b3[H + f0 + r];
-        zz[r] = a.inverse ? (zz[r] - sh) * expf(-s) : fmaf(zz[r], expf(s), sh);
-        part += (double)s;
+        const float sc = a.smax * tanhf((sraw[j][r] + a.b3[f0 + r]) / a.smax);
+        const float sh = acc[j][r] + a.b3[H + f0 + r];
+        zz[r] = a.inverse ? (zz[r] - sh) * expf(-sc) : fmaf(zz[r], expf(sc), sh);
+        part[j] += (double)sc;
       }
-      *zp = float4{zz[0], zz[1], zz[2], zz[3]};
-      PIB_TILE_END(t)
+      *(float4*)(zrow[j] + a.b_off + f0) = float4{zz[0], zz[1], zz[2], zz[3]};
     }
+    PIB_STAGE_END(s)
   }
-#undef PIB_TILE_BEGIN
-#undef PIB_TILE_END
+#undef PIB_STAGE_BEGIN
+#undef PIB_STAGE_END
   // sum s over the lane's features, then over the four lanes that hold the same sample
-  part += shfl_xor_d(part, 16);
-  part += shfl_xor_d(part, 32);
-  if (g == 0) a.logdet[n] += a.inverse ? -part : part;
+#pragma unroll
+  for (int j = 0; j < kNT; ++j) {
+    double t = part[j];
+    t += shfl_xor_d(t, 16);
+    t += shfl_xor_d(t, 32);
+    if (g == 0) a.logdet[n0 + 16 * j] += a.inverse ? -t : t;
+  }
 }
 
 template <int H, int HID>
 hipError_t launch_hh(const FusedLayerArgs& a, hipStream_t s) {
-  using T = Tiles<H, HID>;
-  const size_t lds = (size_t)2 * 2 * T::tile * sizeof(unsigned short);
+  using S = Stages<H, HID>;
+  const size_t lds = (size_t)2 * S::stage * sizeof(unsigned short);
   static bool attr_set = false;  // more than the default 64 KB of dynamic LDS when H = 512
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute((const void*)k_flow_layer<H, HID>,
@@ -264,17 +312,18 @@ hipError_t launch_hh(const FusedLayerArgs& a, hipStream_t s) {
 
 }  // namespace
 
+// D/2 = 256 only: at D/2 = 512 the first product's operands (128 registers) push the kernel
+// past 256 VGPRs, and the compiler then assigns some chained MFMAs a destination that partly
+// overlaps their accumulator input, which the matrix cores do not compute correctly.
 bool fused_layer_supported(int64_t B, int D, int hidden) {
-  const int H = D / 2;
-  return B > 0 && B % kRows == 0 && hidden == 256 && D == 2 * H && (H == 256 || H == 512);
+  return B > 0 && B % kRows == 0 && hidden == 256 && D == 512;
 }
 
 hipError_t launch_flow_layer(const FusedLayerArgs& a, hipStream_t s) {
   if (!fused_layer_supported(a.B, a.D, a.hidden)) return hipErrorInvalidValue;
   if ((a.a_off != 0 && a.a_off != a.D / 2) || (a.b_off != 0 && a.b_off != a.D / 2) || a.a_off == a.b_off)
     return hipErrorInvalidValue;
-  if (a.D / 2 == 256) return launch_hh<256, 256>(a, s);
-  return launch_hh<512, 256>(a, s);
+  return launch_hh<256, 256>(a, s);
 }
 
 }  // namespace pib

@@ -192,7 +192,7 @@ class Flow:
                                  bias=self.p(layer, "b3"), stream=stream)
 
     def _fused_ok(self, B):
-        return self.use_fused and self.hidden == 256 and self.H in (256, 512) and B % 128 == 0
+        return self.use_fused and self.hidden == 256 and self.H == 256 and B % 128 == 0
 
     def _layer_fused(self, inverse, layer, zp, B, logdet, stream):
         """The whole coupling layer in one launch (pib_flow_layer_fused)."""
