@@ -249,6 +249,24 @@ int pib_flow_layer_fused(pib_ctx* ctx, int inverse, int64_t B, int D, int hidden
                          const uint16_t* W3_hi_dev, const uint16_t* W3_lo_dev, const float* b1_dev,
                          const float* b2_dev, const float* b3_dev, void* stream);
 
+/* Base distribution of the flow drawn directly in its whitened, permuted coordinates: the
+ * harmonic rho is the mixture sum_a w_a N(mu_a, I) there (mu_a (D, double) = the surface's
+ * displacement d_a - m rotated by the circulant basis and divided by sigma; w_a the sampler's
+ * surface weights, pib_set_sampler). c_a = P log F_a - |mu_a|^2 / 2 (F_a the O-matrix
+ * prefactor of the sampling model), so that
+ *   log rho(R0) = logsumexp_a (c_a + z . mu_a) - |z|^2 / 2
+ * for R0 the unwhitened z: the same function as pib_flow_logrho (the O-matrix exponent summed
+ * over the ring is (x - d)^T Lambda (x - d)). No counterpart in the reference, which draws R0
+ * and evaluates rho at it; this replaces pib_sample_block + pib_flow_whiten +
+ * pib_flow_logrho for flow draws. */
+int pib_flow_set_base(pib_ctx* ctx, int A_rho, int D, const double* mu_host, const double* c_host);
+
+/* B draws z (float, B x D, the flow's permuted coordinates) and log rho(R0) (double, B); the
+ * surface of sample s is drawn as in pib_sample_block (Philox counter (offset + s, 0, 1)), the
+ * normals from Philox counters (offset + s, m, 2), m the normal pair. */
+int pib_flow_draw_base(pib_ctx* ctx, uint64_t seed, int64_t offset, int64_t B, float* z_dev,
+                       double* logrho_dev, void* stream);
+
 /* log rho(R) of the sampling model (unscaled, double, B) and, if dR_dev is not NULL,
  * dR = -c * d log rho / dR (float, B*N*P; c_dev NULL means c = 1). Uses the PIB_SET_RHO
  * constants and P. Same function as build_o_matrix + build_denominator

@@ -53,6 +53,8 @@ SIGNATURES = {
                                  _c_i64, _vp, _c_i64, _vp, _c_int, _vp, _c_i64, _c_int, _vp]),
     "pib_gemm_bf16x3_presplit": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp, _vp,
                                           _c_i64, _vp, _c_i64, _vp, _vp, _c_i64, _vp, _c_int, _vp]),
+    "pib_flow_set_base": (_c_int, [_vp, _c_int, _c_int, _vp, _vp]),
+    "pib_flow_draw_base": (_c_int, [_vp, _c_u64, _c_i64, _c_i64, _vp, _vp, _vp]),
     "pib_split_bf16": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_int, _vp, _vp, _c_i64, _vp]),
     "pib_flow_layer_fused": (_c_int, [_vp, _c_int, _c_i64, _c_int, _c_int, _c_int, _c_int, ctypes.c_float,
                                       _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -275,6 +277,18 @@ class Context:
         self._check(self.lib.pib_flow_coupling_inverse_backward(
             self.h, int(B), int(D), int(a_off), int(b_off), float(smax), _devptr(y), _devptr(o),
             _devptr(dz), _devptr(dout), _devptr(c), _stream(stream)))
+
+    def flow_set_base(self, mu, c):
+        """mu (A_rho, D) and c (A_rho,) of the flow's base distribution (float64 host arrays)."""
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        self._check(self.lib.pib_flow_set_base(self.h, int(mu.shape[0]), int(mu.shape[1]),
+                                               mu.ctypes.data_as(ctypes.c_void_p),
+                                               c.ctypes.data_as(ctypes.c_void_p)))
+
+    def flow_draw_base(self, seed, offset, B, z, logrho, stream=None):
+        self._check(self.lib.pib_flow_draw_base(self.h, int(seed) & (2**64 - 1), int(offset), int(B),
+                                                _devptr(z), _devptr(logrho), _stream(stream)))
 
     def flow_logrho(self, B, R, c=None, dR=None, logrho=None, stream=None):
         self._check(self.lib.pib_flow_logrho(self.h, int(B), _devptr(R), _devptr(c), _devptr(dR),

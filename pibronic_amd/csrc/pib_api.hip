@@ -201,6 +201,8 @@ int pib_ctx_destroy(pib_ctx* c) {
     release(c->k2_f[s]);
   }
   release(c->mode_tab);
+  release(c->fb_mu);
+  release(c->fb_c);
   release(c->mode_tab_f);
   release(c->s_cdf);
   release(c->s_chol);
@@ -721,6 +723,47 @@ int pib_flow_coupling_inverse_backward(pib_ctx* c, int64_t B, int D, int a_off, 
   PIB_HIP(c, hipSetDevice(c->device));
   hipError_t e = launch_coupling_inverse_backward(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "flow_coupling_inverse_backward launch");
+  return PIB_OK;
+}
+
+int pib_flow_set_base(pib_ctx* c, int A_rho, int D, const double* mu, const double* cc) {
+  if (!c) return PIB_ERR_ARG;
+  if (A_rho < 1 || A_rho > 16 || D < 2 || (D & 1) || !mu || !cc)
+    return fail(c, PIB_ERR_ARG, "flow_set_base: need 1 <= A_rho <= 16, D even and non-null arrays");
+  if (!finite_all(mu, (size_t)A_rho * D) || !finite_all(cc, A_rho))
+    return fail(c, PIB_ERR_ARG, "flow_set_base: non-finite value");
+  int r;
+  if ((r = upload(c, c->fb_mu, mu, (size_t)A_rho * D * sizeof(double)))) return r;
+  if ((r = upload(c, c->fb_c, cc, (size_t)A_rho * sizeof(double)))) return r;
+  c->fb_A = A_rho;
+  c->fb_D = D;
+  return PIB_OK;
+}
+
+int pib_flow_draw_base(pib_ctx* c, uint64_t seed, int64_t offset, int64_t B, float* z, double* logrho,
+                       void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (!c->fb_A) return fail(c, PIB_ERR_STATE, "flow_draw_base: base distribution not set");
+  if (!c->sampler_set || c->s_A != c->fb_A)
+    return fail(c, PIB_ERR_STATE, "flow_draw_base: sampler surface weights not set for this A_rho");
+  if (B < 0 || offset < 0) return fail(c, PIB_ERR_ARG, "flow_draw_base: negative B or offset");
+  if (B == 0) return PIB_OK;
+  if (!z || !logrho) return fail(c, PIB_ERR_ARG, "flow_draw_base: null output");
+  BaseArgs a{};
+  a.B = B;
+  a.offset = offset;
+  a.D = c->fb_D;
+  a.A_rho = c->fb_A;
+  a.key0 = (uint32_t)seed;
+  a.key1 = (uint32_t)(seed >> 32);
+  a.cdf = (const double*)c->s_cdf.p;
+  a.mu = (const double*)c->fb_mu.p;
+  a.c = (const double*)c->fb_c.p;
+  a.z = z;
+  a.logrho = logrho;
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_flow_base(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "flow_draw_base launch");
   return PIB_OK;
 }
 

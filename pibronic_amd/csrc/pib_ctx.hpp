@@ -57,6 +57,8 @@ struct pib_ctx {
   // flow basis
   int f_N = 0, f_P = 0;
   pib::DevBuf f_U, f_sigma, f_isigma, f_m;
+  pib::DevBuf fb_mu, fb_c;  // flow base distribution (pib_flow_set_base)
+  int fb_A = 0, fb_D = 0;
 };
 
 namespace pib {

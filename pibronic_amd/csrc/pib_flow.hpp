@@ -67,6 +67,17 @@ hipError_t launch_flow_pre(int mode, const BasisArgs& b, const void* in, float* 
 hipError_t launch_flow_post(int mode, const BasisArgs& b, const float* in, void* out, hipStream_t s);
 // the whole transform (pre, product with U or U^T, post) in one kernel, for P in {64, 128}
 bool whiten_fused_supported(int P);
+struct BaseArgs {
+  int64_t B, offset;
+  int D, A_rho;
+  uint32_t key0, key1;
+  const double* cdf;  // A_rho cumulative surface weights
+  const double* mu;   // A_rho * D
+  const double* c;    // A_rho
+  float* z;           // B * D
+  double* logrho;     // B
+};
+hipError_t launch_flow_base(const BaseArgs& a, hipStream_t s);
 hipError_t launch_whiten_fused(int mode, const BasisArgs& b, const void* in, void* out, const float* U,
                                int num_cu, hipStream_t s);
 

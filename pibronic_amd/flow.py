@@ -1,8 +1,8 @@
 """Learned proposal: an affine-coupling flow that replaces the harmonic rho as the sampling
 distribution of the PIMC estimator.
 
-Construction. Let R0 ~ rho (the reference's harmonic sampling distribution, drawn by the
-device sampler). Whiten it into the ring's normal-mode coordinates,
+Construction. Let R0 ~ rho (the reference's harmonic sampling distribution). Whiten it into
+the ring's normal-mode coordinates,
 Z = (R0 - m) U / sigma (U the circulant eigenvectors, sigma the harmonic standard deviations,
 m the weighted mean of the surfaces' displacements), apply L affine coupling layers
 (RealNVP, Dinh, Sohl-Dickstein & Bengio 2017) y = F(Z), and map back, R = m + (y sigma) U^T.
@@ -17,6 +17,10 @@ last layer of every coupling network starts at zero, so an untrained flow is exa
 Layers alternate the even-k and odd-k normal modes as the transformed half; each coupling
 network is D/2 -> hidden -> hidden -> D (ReLU), giving raw scales and shifts for the other
 half, with scales bounded by smax * tanh(raw / smax).
+
+Draws start in the whitened coordinates: there rho is a mixture of unit Gaussians whose means
+differ between surfaces (pib_flow_draw_base), so Z and log rho(R0) come from one kernel and R0
+itself is never formed.
 
 Training minimises KL(p || q) for p proportional to |g|: the loss of a batch with fixed
 weights c_b (summing to one) is -sum_b c_b log q(R_b), differentiated through the inverse
@@ -54,6 +58,7 @@ class Flow:
         sigma = rho.standard_deviation[0]                 # (N, P), identical for all surfaces
         m = rho.state_weight @ rho.state_shift            # (N,)
         self.ctx.flow_set_basis(data.circulant_eigvects, sigma, m)
+        self._set_base(rho, sigma, m)
         # parameter layout per layer: W1 (H x h), b1, W2 (h x h), b2, W3 (h x D), b3 (D)
         H, h, D = self.H, hidden, self.D
         self.shapes = [("W1", (H, h)), ("b1", (h,)), ("W2", (h, h)), ("b2", (h,)),
@@ -77,6 +82,24 @@ class Flow:
         self.use_fused = True      # bf16x3 draws and densities: one launch per coupling layer where supported
         self._split_version = -1   # parameter version of the pre-split bf16 weights
         self._fsplit_version = -1  # parameter version of the fused layer's permuted weights
+
+    def _set_base(self, rho, sigma, m):
+        """rho in the flow's coordinates Z = (R0 - m) U / sigma: the mixture over surfaces a
+        (weights w_a) of N(mu_a, I), mu_a,jk = (d_aj - m_j) (sum_p U[p, k]) / sigma_jk, with
+        log rho(R0) = logsumexp_a (c_a + Z . mu_a) - |Z|^2 / 2, c_a = P log F_a - |mu_a|^2 / 2
+        (pimc.py:1087-1136: the O-matrix exponent summed over the ring is
+        (x - d)^T (2 coth I - csch C) (x - d), whose eigenbasis is U with 1 / sigma^2 as
+        eigenvalues)."""
+        U = np.asarray(self.data.circulant_eigvects, dtype=np.float64)
+        shift = np.asarray(rho.state_shift, dtype=np.float64)           # (A_rho, N)
+        pref = np.asarray(rho.const.omatrix_prefactor, dtype=np.float64)  # (A_rho,)
+        if np.any(pref <= 0.0):
+            raise ValueError("flow base distribution needs positive O-matrix prefactors")
+        colsum = U.sum(axis=0)                                            # (P,)
+        mu = (shift - m[None, :])[:, :, None] * colsum[None, None, :] / np.asarray(sigma)[None]
+        mu_p = np.concatenate([mu[:, :, 0::2].reshape(len(pref), -1), mu[:, :, 1::2].reshape(len(pref), -1)], axis=1)
+        c = self.P * np.log(pref) - 0.5 * np.sum(mu_p * mu_p, axis=1)
+        self.ctx.flow_set_base(mu_p, c)
 
     # ------------------------------------------------------------------ layout helpers
     def _offset(self, layer, name):
@@ -208,13 +231,21 @@ class Flow:
         import torch
         B = R0.shape[0]
         self._check_batch(B)
-        D, h = self.D, self.hidden
+        D = self.D
         zp = self._buf("zp", (B, D))
         work = self._buf("work", (2 * B * D,))
-        h1, h2, o = self._buf("h1", (B, h)), self._buf("h2", (B, h)), self._buf("o", (B, D))
         logdet = torch.zeros(B, dtype=torch.float64, device=self.dev)
         R = torch.empty_like(R0)
         self.ctx.flow_whiten(_native.FLOW_WHITEN, B, R0, work, zp, stream)
+        self._layers_forward(zp, B, logdet, stream)
+        self.ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, R, stream)
+        return R, logdet
+
+    def _layers_forward(self, zp, B, logdet, stream):
+        """The coupling layers in the sampling direction on zp (B, D) in place; adds
+        log|det| to logdet."""
+        D, h = self.D, self.hidden
+        h1, h2, o = self._buf("h1", (B, h)), self._buf("h2", (B, h)), self._buf("o", (B, D))
         fast = self.gemm == "bf16x3"
         if fast:
             self._split_weights(stream)
@@ -228,8 +259,6 @@ class Flow:
             else:
                 self._mlp(l, zp, B, h1, h2, o, stream)
             self.ctx.flow_coupling(False, B, D, a_off, b_off, self.smax, zp, o, logdet, stream)
-        self.ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, R, stream)
-        return R, logdet
 
     def log_q(self, R, stream=None):
         """Density direction: log q(R) = log rho(R0) + log|det dF^-1/dy| with R0 = T^-1(R)."""
@@ -321,13 +350,18 @@ class Flow:
 
     # ------------------------------------------------------------------ proposal draws
     def draw(self, B, seed, offset, stream=None):
-        """B proposal samples: R (B, N, P) double and log q(R) (B,)."""
+        """B proposal samples: R (B, N, P) double and log q(R) (B,). The base sample is drawn
+        directly in the flow's coordinates with its log rho (pib_flow_draw_base)."""
         import torch
-        R0 = torch.empty((B, self.N, self.P), dtype=torch.float64, device=self.dev)
-        self.ctx.sample_block(seed, offset, B, R0, None, stream)
+        self._check_batch(B)
+        zp = self._buf("zp", (B, self.D))
         logrho = torch.empty(B, dtype=torch.float64, device=self.dev)
-        self.ctx.flow_logrho(B, R0, logrho=logrho, stream=stream)
-        R, logdet = self.forward(R0, stream)
+        self.ctx.flow_draw_base(seed, offset, B, zp, logrho, stream)
+        logdet = torch.zeros(B, dtype=torch.float64, device=self.dev)
+        self._layers_forward(zp, B, logdet, stream)
+        R = torch.empty((B, self.N, self.P), dtype=torch.float64, device=self.dev)
+        work = self._buf("work", (2 * B * self.D,))
+        self.ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, R, stream)
         return R, logrho - logdet
 
 

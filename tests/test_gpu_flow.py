@@ -408,6 +408,61 @@ def test_fused_layer_matches_layer_by_layer(tmp_path, N, P):
     assert dR <= 5e-5 * scale and dld <= 1e-2
 
 
+def test_base_draw_density_matches_oracle(model):
+    """pib_flow_draw_base: log rho at the drawn point equals the reference's log rho(R0) for
+    R0 = m + (Z sigma) U^T formed in float64 from the float Z (oracle: the O-matrix product of
+    pimc.py:1087-1136 in log form), to FP64 rounding."""
+    _announce("flow base draw: log rho")
+    data = model
+    fl, ref, _ = _flow_and_ref(data)
+    B = 512
+    zp = torch.empty((B, fl.D), device=DEV)
+    lr = torch.empty(B, dtype=torch.float64, device=DEV)
+    fl.ctx.flow_draw_base(9, 100, B, zp, lr)
+    torch.cuda.synchronize()
+    Z = ref.unperm(zp.cpu().double())
+    R0 = ref.m[None, :, None] + (Z * ref.sigma) @ ref.U.T
+    want = ref.log_rho(R0)
+    err = (lr.cpu() - want).abs().max()
+    print("max |d log rho|", float(err), "|log rho| ~", float(want.abs().mean()))
+    assert float(err) <= 1e-9 * (1.0 + float(want.abs().max()))
+
+
+def test_base_draw_moments(model):
+    """Every coordinate of the base draw has the mixture's mean sum_a w_a mu_a and variance
+    1 + sum_a w_a mu_a^2 - (sum_a w_a mu_a)^2 (within 6 standard errors, B = 65536); draws
+    with another offset are different numbers."""
+    _announce("flow base draw: moments")
+    data = model
+    fl, ref, _ = _flow_and_ref(data)
+    B = 65536
+    zp = torch.empty((B, fl.D), device=DEV)
+    lr = torch.empty(B, dtype=torch.float64, device=DEV)
+    fl.ctx.flow_draw_base(3, 0, B, zp, lr)
+    z2 = torch.empty((64, fl.D), device=DEV)
+    fl.ctx.flow_draw_base(3, 64, 64, z2, lr[:64])
+    torch.cuda.synchronize()
+    z = zp.cpu().double()
+    assert not torch.equal(z[:64], z2.cpu().double())
+    assert torch.equal(z[64:128], z2.cpu().double())
+    rho = data.rho
+    m = np.asarray(rho.state_weight, dtype=np.float64) @ np.asarray(rho.state_shift)  # the flow's centre
+    w = np.asarray(rho.state_weight, dtype=np.float64)
+    w = w / w.sum()                                                                    # surface probabilities
+    sigma = np.asarray(rho.standard_deviation[0])
+    U = np.asarray(data.circulant_eigvects)
+    mu = (np.asarray(rho.state_shift) - m[None, :])[:, :, None] * U.sum(axis=0)[None, None, :] / sigma[None]
+    mu = ref.perm(torch.as_tensor(mu)).numpy()                     # (A_rho, D)
+    mean = w @ mu
+    var = 1.0 + w @ (mu * mu) - mean * mean
+    se_m = np.sqrt(var / B)
+    se_v = var * np.sqrt(2.0 / B) * 1.5
+    zm, zv = z.mean(dim=0).numpy(), z.var(dim=0).numpy()
+    print("max |mean err|/se", float(np.max(np.abs(zm - mean) / se_m)), "max |var err|/se", float(np.max(np.abs(zv - var) / se_v)))
+    assert np.all(np.abs(zm - mean) < 6 * se_m)
+    assert np.all(np.abs(zv - var) < 6 * se_v)
+
+
 def test_identity_flow_reproduces_harmonic_weights(model):
     _announce("zero-initialised flow = rho")
     data = model
