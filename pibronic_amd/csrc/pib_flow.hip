@@ -219,17 +219,25 @@ hipError_t launch_whiten_fused(int mode, const BasisArgs& b, const void* in, voi
 }
 
 // ------------------------------------------------------------------ flow base draws
-// One wave per sample: the surface a from the cumulative weights (the sampler's counter
-// (s, 0, 1)), normal pairs from Philox counters (s, m, 2) by Box-Muller in FP64, z = eps + mu_a
-// rounded to float (the value the flow receives; its density is evaluated at that value), and
-// log rho = logsumexp_b (c_b + z . mu_b) - |z|^2 / 2 with fixed-order wave sums.
+// Sixteen lanes per sample, four samples per wave: the surface a from the cumulative weights
+// (the sampler's counter (s, 0, 1)), normal pairs from Philox counters (s, m, 2) by Box-Muller
+// in FP64, z = eps + mu_a rounded to float (the value the flow receives; its density is
+// evaluated at that value), and log rho = logsumexp_b (c_b + z . mu_b) - |z|^2 / 2 with
+// fixed-order sums over the sample's sixteen lanes.
 constexpr int kBaseMaxA = 16;
+constexpr int kBaseLanes = 16;
+
+__device__ __forceinline__ double group16_sum(double v) {
+#pragma unroll
+  for (int o = 1; o < kBaseLanes; o <<= 1) v += shfl_xor_d(v, o);
+  return v;
+}
 
 __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
-  const int lane = threadIdx.x & 63;
-  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (s >= a.B) return;  // whole wave
-  const int64_t gs = a.offset + s;
+  const int sl = threadIdx.x & (kBaseLanes - 1);
+  const int64_t s = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kBaseLanes;
+  const bool live = s < a.B;  // uniform over the sixteen lanes of a sample
+  const int64_t gs = a.offset + (live ? s : 0);
   int src = 0;
   if (a.A_rho > 1) {
     const u32x4 r = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), 0u, 1u}, a.key0, a.key1);
@@ -237,39 +245,37 @@ __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
     while (src < a.A_rho - 1 && u >= a.cdf[src]) ++src;
   }
   const double* __restrict__ mus = a.mu + (int64_t)src * a.D;
-  float* zr = a.z + s * a.D;
+  float* zr = a.z + (live ? s : 0) * a.D;
   double zz = 0.0, dot[kBaseMaxA];
 #pragma unroll
   for (int b = 0; b < kBaseMaxA; ++b) dot[b] = 0.0;
-  for (int m = lane; 2 * m < a.D; m += 64) {
-    const u32x4 r = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), (uint32_t)m, 2u}, a.key0, a.key1);
-    const double u1 = 1.0 - u53(r.x, r.y);  // (0, 1]
-    const double u2 = u53(r.z, r.w);
-    const double rad = sqrt(-2.0 * log(u1));
-    double sn, cs;
-    sincospi(2.0 * u2, &sn, &cs);
-    const float z0 = (float)(rad * cs + mus[2 * m]), z1 = (float)(rad * sn + mus[2 * m + 1]);
-    *(float2*)(zr + 2 * m) = float2{z0, z1};
-    const double d0 = z0, d1 = z1;
-    zz = fma(d0, d0, fma(d1, d1, zz));
+  if (live) {
+    for (int m = sl; 2 * m < a.D; m += kBaseLanes) {
+      const u32x4 r = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), (uint32_t)m, 2u}, a.key0, a.key1);
+      const double u1 = 1.0 - u53(r.x, r.y);  // (0, 1]
+      const double u2 = u53(r.z, r.w);
+      const double rad = sqrt(-2.0 * log(u1));
+      double sn, cs;
+      sincospi(2.0 * u2, &sn, &cs);
+      const float z0 = (float)(rad * cs + mus[2 * m]), z1 = (float)(rad * sn + mus[2 * m + 1]);
+      *(float2*)(zr + 2 * m) = float2{z0, z1};
+      const double d0 = z0, d1 = z1;
+      zz = fma(d0, d0, fma(d1, d1, zz));
 #pragma unroll
-    for (int b = 0; b < kBaseMaxA; ++b)
-      if (b < a.A_rho) dot[b] = fma(d0, a.mu[(int64_t)b * a.D + 2 * m], fma(d1, a.mu[(int64_t)b * a.D + 2 * m + 1], dot[b]));
+      for (int b = 0; b < kBaseMaxA; ++b)
+        if (b < a.A_rho) dot[b] = fma(d0, a.mu[(int64_t)b * a.D + 2 * m], fma(d1, a.mu[(int64_t)b * a.D + 2 * m + 1], dot[b]));
+    }
   }
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) zz += shfl_xor_d(zz, o);
+  zz = group16_sum(zz);
   double mx = -INFINITY, t[kBaseMaxA];
 #pragma unroll
   for (int b = 0; b < kBaseMaxA; ++b) {
     if (b < a.A_rho) {
-      double v = dot[b];
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) v += shfl_xor_d(v, o);
-      t[b] = a.c[b] + v;
+      t[b] = a.c[b] + group16_sum(dot[b]);
       mx = fmax(mx, t[b]);
     }
   }
-  if (lane == 0) {
+  if (live && sl == 0) {
     double acc = 0.0;
 #pragma unroll
     for (int b = 0; b < kBaseMaxA; ++b)
@@ -280,7 +286,7 @@ __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
 
 hipError_t launch_flow_base(const BaseArgs& a, hipStream_t s) {
   if (a.B <= 0 || a.D <= 0 || (a.D & 1) || a.A_rho < 1 || a.A_rho > kBaseMaxA) return hipErrorInvalidValue;
-  const int64_t grid = (a.B + 3) / 4;
+  const int64_t grid = (a.B + 256 / kBaseLanes - 1) / (256 / kBaseLanes);
   if (grid > 0x7fffffff) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_flow_base, dim3((unsigned)grid), dim3(256), 0, s, a);
   return hipGetLastError();
