@@ -167,6 +167,49 @@ __device__ __forceinline__ void hestenes_step(double (&v)[kW], double& al, int j
   al = low ? fma(-t, gam, al) : fma(t, gam, al);
 }
 
+// Round r of a 12-column round-robin (circle method) sweep, for A <= 12: rows and columns
+// 12..15 of G hold only padding (zero rows under the real columns, sigma e_j columns whose
+// rotations are no-ops), so each round pairs the twelve real-or-padding columns below 12 --
+// lane 11 with lane r, lane j with lane (2r - j) mod 11 otherwise -- and touches twelve rows.
+// 11 rounds of 6 disjoint pairs replace the 15 xor matchings of the 16-column sweep. Lanes
+// 12..15 pair with themselves (no-op).
+__device__ __forceinline__ void hestenes_round12(double (&v)[kW], double& al, int j, int r, int group,
+                                                 double tol2, bool& busy) {
+  constexpr int kR = 12;
+  const int pj = (j >= kR) ? j : (j == kR - 1) ? r : (j == r) ? kR - 1 : (2 * r - j + 2 * (kR - 1)) % (kR - 1);
+  const int src = group * kW + pj;
+  double pc[kR];
+#pragma unroll
+  for (int i = 0; i < kR; ++i) pc[i] = __shfl(v[i], src, kWave);
+  const double be = __shfl(al, src, kWave);
+  double gam = 0.0;
+#pragma unroll
+  for (int i = 0; i < kR; ++i) gam = fma(v[i], pc[i], gam);
+  const bool self = pj == j;
+  const bool low = j < pj;  // this lane holds the lower column index p
+  const double app = low ? al : be, aqq = low ? be : al;
+  const double d = aqq - app, h = self ? 0.0 : 2.0 * gam;
+  const double x = fma(d, d, h * h);
+  const double rr = x * rsq_nr_w<1>(x > 0.0 ? x : 1.0);
+  const double den = fabs(d) + rr;
+  double t = h * rcp_nr_w<1>(den > 0.0 ? den : 1.0);
+  t = d < 0.0 ? -t : t;
+  const double c = rsq_nr_w<2>(fma(t, t, 1.0));
+  const double sn = t * c;
+  busy |= !self && (gam * gam > tol2 * (app * aqq));
+  const double sg = low ? -sn : sn;  // p: c g_p - s g_q ; q: s g_p + c g_q
+#pragma unroll
+  for (int i = 0; i < kR; ++i) v[i] = fma(c, v[i], sg * pc[i]);
+  al = self ? al : (low ? fma(-t, gam, al) : fma(t, gam, al));
+}
+
+__device__ __forceinline__ double column_norm2_12(const double (&v)[kW]) {
+  double a2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) a2 = fma(v[i], v[i], a2);
+  return a2;
+}
+
 __device__ __forceinline__ double column_norm2(const double (&v)[kW]) {
   double a2 = 0.0;
 #pragma unroll
@@ -176,7 +219,7 @@ __device__ __forceinline__ double column_norm2(const double (&v)[kW]) {
 
 // One sample per wave; L is the wave's private LDS tile set. BF: ring products (and M) as
 // BF16x3 on the bf16 matrix cores with FP32 accumulation, renormalised every bead.
-template <bool PM, bool DBG, bool BF = false>
+template <bool PM, bool DBG, bool BF = false, bool C12 = false>
 __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t sample, WideLds& L) {
   const int lane = threadIdx.x & (kWave - 1);
   const int g = lane / kW, j = lane % kW;
@@ -294,7 +337,14 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
 #pragma unroll
     for (int i = 0; i < kW; ++i) v[i] += (i == j) ? sig : 0.0;
     double al = 0.0;
-    for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
+    if (C12) {  // A <= 12
+      for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
+        al = column_norm2_12(v);
+        bool busy = false;
+        for (int r = 0; r < 11; ++r) hestenes_round12(v, al, j, r, g, tol2, busy);
+        if (!__any(busy)) break;
+      }
+    } else for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
       al = column_norm2(v);
       bool busy = false;
       hestenes_step<1>(v, al, j, tol2, busy);
@@ -485,13 +535,13 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
   }
 }
 
-template <bool PM, bool DBG, bool BF>
+template <bool PM, bool DBG, bool BF, bool C12>
 __global__ void __launch_bounds__(kWave * kWideWaves) k_eval_wide(const EvalArgs args) {
   __shared__ WideLds lds_all[kWideWaves];
   const int wv = threadIdx.x / kWave;
   const int64_t sample = (int64_t)blockIdx.x * kWideWaves + wv;
   if (sample >= args.B) return;  // whole wave; no work-group barrier is used
-  eval_wide_sample<PM, DBG, BF>(args, sample, lds_all[wv]);
+  eval_wide_sample<PM, DBG, BF, C12>(args, sample, lds_all[wv]);
 }
 
 // Persistent batched launch (see k_eval_batched): a tile is a range of samples ("waves") of
@@ -511,7 +561,12 @@ k_eval_wide_batched(const EvalArgs* __restrict__ probs, const BatchTile* __restr
     const BatchTile tile = tiles[t];
     const EvalArgs& a = probs[tile.prob];
     for (int64_t smp = tile.wave0; smp < tile.wave0 + tile.nwaves; ++smp) {
-      if (smp < a.B) eval_wide_sample<PM, false>(a, smp, L);
+      if (smp < a.B) {
+        if (a.A <= 12)
+          eval_wide_sample<PM, false, false, true>(a, smp, L);
+        else
+          eval_wide_sample<PM, false, false, false>(a, smp, L);
+      }
       wave_sync();
     }
   }
@@ -533,9 +588,11 @@ hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s, bool bf16x
   const dim3 grid((unsigned)((a.B + kWideWaves - 1) / kWideWaves));
   const dim3 block(kWave * kWideWaves);
   const bool dbg = a.dbg != nullptr;
+  const bool c12 = a.A <= 12;
 #define PIB_WIDE(pm_, dbg_, bf_) \
   if ((pm != 0) == pm_ && dbg == dbg_ && bf16x3 == bf_) { \
-    hipLaunchKernelGGL((k_eval_wide<pm_, dbg_, bf_>), grid, block, 0, s, a); \
+    if (c12) hipLaunchKernelGGL((k_eval_wide<pm_, dbg_, bf_, true>), grid, block, 0, s, a); \
+    else hipLaunchKernelGGL((k_eval_wide<pm_, dbg_, bf_, false>), grid, block, 0, s, a); \
     return hipGetLastError(); \
   }
   PIB_WIDE(false, false, false) PIB_WIDE(true, false, false) PIB_WIDE(false, true, false)
