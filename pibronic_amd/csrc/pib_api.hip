@@ -90,10 +90,11 @@ int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* w
   if (bf && !fits_wide)
     return fail(c, PIB_ERR_UNSUPPORTED, "eval_block: BF16x3 ring products are in the wide kernel (A, A_rho <= 16)");
   const bool wide = !fp32 && fits_wide && (bf || (flags & PIB_EVAL_WIDE) || !eval_supported(c->A, AR));
-  if (fp32)
+  if (!wide)
     for (int s = 0; s < need; ++s)
       if (!c->pref_positive[s])
-        return fail(c, PIB_ERR_UNSUPPORTED, "eval_block: the FP32 kernel works with log F_a and needs every O-matrix prefactor > 0");
+        return fail(c, PIB_ERR_UNSUPPORTED, std::string("eval_block: the ") + (fp32 ? "FP32" : "FP64") +
+                                                " lanes kernel works with log F_a and needs every O-matrix prefactor > 0");
   if (!wide && !eval_supported(c->A, AR))
     return fail(c, PIB_ERR_UNSUPPORTED, std::string("eval_block: no ") + (fp32 ? "FP32" : "FP64") +
                                             " kernel for A=" + std::to_string(c->A) +
@@ -116,6 +117,7 @@ int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* w
     a.set[s].th = (const double*)c->th[s].p;
     a.set[s].lin = (const double*)c->lin[s].p;
     a.set[s].k2 = (const double*)c->k2[s].p;
+    a.set[s].lpref = (const double*)c->lpref[s].p;
     if (s >= need) a.set[s] = a.set[SET_VIB];
   }
   a.energy = (const double*)c->energy.p;
@@ -198,6 +200,7 @@ int pib_ctx_destroy(pib_ctx* c) {
     release(c->th[s]);
     release(c->lin[s]);
     release(c->k2[s]);
+    release(c->lpref[s]);
     release(c->k2_f[s]);
   }
   release(c->mode_tab);
@@ -331,6 +334,9 @@ int pib_set_omatrix_set(pib_ctx* c, int slot, int A_set, int N, const double* co
     if ((r = upload(c, c->th[slot], th.data(), N * sizeof(double)))) return r;
     if ((r = upload(c, c->lin[slot], lin.data(), (size_t)A_set * N * sizeof(double)))) return r;
     if ((r = upload(c, c->k2[slot], k2.data(), A_set * sizeof(double)))) return r;
+    std::vector<double> lp(A_set);
+    for (int q = 0; q < A_set; ++q) lp[q] = pref[q] > 0.0 ? std::log(pref[q]) : 0.0;
+    if ((r = upload(c, c->lpref[slot], lp.data(), A_set * sizeof(double)))) return r;
     std::vector<float> k2f(k2.begin(), k2.end());
     if ((r = upload(c, c->k2_f[slot], k2f.data(), A_set * sizeof(float)))) return r;
     c->h_th[slot] = th;

@@ -33,7 +33,7 @@ struct pib_ctx {
   int set_A[pib::NUM_SETS] = {0, 0, 0, 0};
   int set_N[pib::NUM_SETS] = {0, 0, 0, 0};
   pib::DevBuf coth[pib::NUM_SETS], csch[pib::NUM_SETS], pref[pib::NUM_SETS], shift[pib::NUM_SETS];
-  pib::DevBuf th[pib::NUM_SETS], lin[pib::NUM_SETS], k2[pib::NUM_SETS];
+  pib::DevBuf th[pib::NUM_SETS], lin[pib::NUM_SETS], k2[pib::NUM_SETS], lpref[pib::NUM_SETS];
   // host copies feeding the per-mode constant records of the lanes kernel
   std::vector<double> h_th[pib::NUM_SETS], h_cs[pib::NUM_SETS], h_lin[pib::NUM_SETS];
   std::vector<double> h_linear, h_qdiag;

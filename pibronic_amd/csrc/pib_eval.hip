@@ -229,12 +229,12 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
   // read-only constants: restrict-qualified so the compiler can keep them in scalar loads
   struct RO {
     const double* __restrict__ k2;
-    const double* __restrict__ pref;
+    const double* __restrict__ lpref;
   };
-  const RO sr{args.set[SET_RHO].k2, args.set[SET_RHO].pref};
-  const RO sv{args.set[SET_VIB].k2, args.set[SET_VIB].pref};
-  const RO sp{args.set[SET_VIB_PLUS].k2, args.set[SET_VIB_PLUS].pref};
-  const RO sm{args.set[SET_VIB_MINUS].k2, args.set[SET_VIB_MINUS].pref};
+  const RO sr{args.set[SET_RHO].k2, args.set[SET_RHO].lpref};
+  const RO sv{args.set[SET_VIB].k2, args.set[SET_VIB].lpref};
+  const RO sp{args.set[SET_VIB_PLUS].k2, args.set[SET_VIB_PLUS].lpref};
+  const RO sm{args.set[SET_VIB_MINUS].k2, args.set[SET_VIB_MINUS].lpref};
   const double* __restrict__ Ev = args.energy;
   const double* __restrict__ Qd = args.qdiag;
   const int* __restrict__ pairs = args.pair;
@@ -248,12 +248,11 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
       Ng[i][k] = (i == k) ? 1.0 : 0.0;
       if (PM) { Np[i][k] = Ng[i][k]; Nm[i][k] = Ng[i][k]; }
     }
-  double rp[AR];
+  // log-domain products: lr[a] = sum_p log(O_rho,a(p) / S_p), ls = sum_p log S_p
+  double lr[AR];
 #pragma unroll
-  for (int a = 0; a < AR; ++a) rp[a] = 1.0;
-  // prod_p S_p kept as mantissa * 2^exponent; one log per lane at the end
-  double smant = 1.0;
-  int sexp = 0;
+  for (int a = 0; a < AR; ++a) lr[a] = 0.0;
+  double ls = 0.0;
 
   const int pbeg = gl * K;
   const int pend = live ? min(pbeg + K, P) : pbeg;
@@ -288,24 +287,26 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
         for (int c = 0; c <= b; ++c) v[b][c] = fma(xk * Qh[b * A + c], xj, v[b][c]);
     }
 
-    // O-factors, per-bead scaling S_p = max over both O's (pimc.py:1076-1084)
-    double Or[AR], Ov[A], Op[A], Om[A];
-    double S = 0.0;
+    // O-factors and the per-bead scaling S_p = max over both O's (pimc.py:1076-1084), in the
+    // log domain: log O_a = log F_a - e_a / 2, log S_p = max; rho's factors never leave it,
+    // so per bead only the vib O-factors (and M's eigenvalues) need an exponential
+    double lOr[AR], lOv[A], lOp[A], lOm[A];
+    double lS = -INFINITY;
 #pragma unroll
     for (int a = 0; a < AR; ++a) {
-      Or[a] = exp(-0.5 * ((tr + er[a]) + sr.k2[a])) * sr.pref[a];
-      S = fmax(S, Or[a]);
+      lOr[a] = fma(-0.5, (tr + er[a]) + sr.k2[a], sr.lpref[a]);
+      lS = fmax(lS, lOr[a]);
     }
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-      Ov[a] = exp(-0.5 * ((tv + ev[a]) + sv.k2[a])) * sv.pref[a];
-      S = fmax(S, Ov[a]);
+      lOv[a] = fma(-0.5, (tv + ev[a]) + sv.k2[a], sv.lpref[a]);
+      lS = fmax(lS, lOv[a]);
       if (PM) {
-        Op[a] = exp(-0.5 * ((tp + ep[a]) + sp.k2[a])) * sp.pref[a];
-        Om[a] = exp(-0.5 * ((tm + em[a]) + sm.k2[a])) * sm.pref[a];
+        lOp[a] = fma(-0.5, (tp + ep[a]) + sp.k2[a], sp.lpref[a]);
+        lOm[a] = fma(-0.5, (tm + em[a]) + sm.k2[a], sm.lpref[a]);
       }
     }
-    if (args.noscale) S = 1.0;
+    if (args.noscale) lS = 0.0;
 
     // coupling matrix -> Jacobi -> M = U exp(-tau lambda) U^T (pimc.py:1171-1187)
     double a_[A][A], u[A][A];
@@ -333,10 +334,10 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
       const int rec = eval_debug_record(A, AR);
       double* dr = args.dbg + ((int64_t)sample * P + p) * rec;
 #pragma unroll
-      for (int a = 0; a < AR; ++a) dr[a] = Or[a];
+      for (int a = 0; a < AR; ++a) dr[a] = exp(lOr[a]);
 #pragma unroll
-      for (int a = 0; a < A; ++a) dr[AR + a] = Ov[a];
-      dr[AR + A] = S;
+      for (int a = 0; a < A; ++a) dr[AR + a] = exp(lOv[a]);
+      dr[AR + A] = exp(lS);
 #pragma unroll
       for (int c = 0; c < A; ++c)
 #pragma unroll
@@ -344,19 +345,18 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
     }
 
     // scale (pimc.py:1062-1066, 1445, 1448), then accumulate rho and the ring products
-    const double inv = 1.0 / S;
+    double Ov[A], Op[A], Om[A];
 #pragma unroll
-    for (int a = 0; a < AR; ++a) rp[a] *= Or[a] * inv;
+    for (int a = 0; a < AR; ++a) lr[a] += lOr[a] - lS;
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-      Ov[a] *= inv;
-      if (PM) { Op[a] *= inv; Om[a] *= inv; }
+      Ov[a] = exp(lOv[a] - lS);
+      if (PM) {
+        Op[a] = exp(lOp[a] - lS);
+        Om[a] = exp(lOm[a] - lS);
+      }
     }
-    {
-      int e;
-      smant = frexp(smant * S, &e);
-      sexp += e;
-    }
+    ls += lS;
 #pragma unroll
     for (int i = 0; i < A; ++i) {
       double rg[A], rpl[A], rmi[A];
@@ -385,7 +385,6 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
     }
   }
 
-  double ls = log(smant) + (double)sexp * 0.69314718055994530942;
 
   // ordered tree over the G lanes of a sample: lane g (g % 2s == 0) absorbs lane g + s
   for (int s = 1; s < G; s <<= 1) {
@@ -437,8 +436,8 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
     }
 #pragma unroll
     for (int a = 0; a < AR; ++a) {
-      const double t = shfl_down_d(rp[a], s);
-      if (leader) rp[a] *= t;
+      const double t = shfl_down_d(lr[a], s);
+      if (leader) lr[a] += t;
     }
     const double tl = shfl_down_d(ls, s);
     if (leader) ls += tl;
@@ -446,9 +445,9 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
 
   if (live && gl == 0) {
     const int64_t st = args.out_stride;
-    double rho = rp[0];
+    double rho = exp(lr[0]);
 #pragma unroll
-    for (int a = 1; a < AR; ++a) rho += rp[a];
+    for (int a = 1; a < AR; ++a) rho += exp(lr[a]);
     double g = Ng[0][0];
 #pragma unroll
     for (int i = 1; i < A; ++i) g += Ng[i][i];

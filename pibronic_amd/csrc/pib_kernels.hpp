@@ -21,6 +21,7 @@ struct OSet {
   const double* th;     // N
   const double* lin;    // A_set * N
   const double* k2;     // A_set
+  const double* lpref;  // A_set, log F_a (lanes kernel: O-factors in the log domain)
 };
 
 enum : int { SET_RHO = 0, SET_VIB = 1, SET_VIB_PLUS = 2, SET_VIB_MINUS = 3, NUM_SETS = 4 };
