@@ -296,6 +296,8 @@ def main():
     ap.add_argument("--ring-bf16x3", action="store_true",
                     help="wide kernel with M and ring products as BF16x3 on the bf16 matrix cores "
                          "(PIB_EVAL_BF16X3); the error against FP64 on the same samples is reported")
+    ap.add_argument("--wide", action="store_true",
+                    help="harmonic leg: the one-wave-per-sample FP64 kernel (PIB_EVAL_WIDE) at any A <= 16")
     ap.add_argument("--fp32", action="store_true",
                     help="reduced-precision kernel (PIB_EVAL_FP32); the error against FP64 on the "
                          "same samples is reported")
@@ -365,7 +367,7 @@ def main():
     K, W = args.steps, args.warmup
     sums = torch.zeros((K + W, _native.NUM_SUMS), dtype=torch.float64, device="cuda")
     flags = ((_native.EVAL_PM if pm else 0) | (_native.EVAL_FP32 if args.fp32 else 0)
-             | (_native.EVAL_BF16X3 if args.ring_bf16x3 else 0))
+             | (_native.EVAL_BF16X3 if args.ring_bf16x3 else 0) | (_native.EVAL_WIDE if args.wide else 0))
     ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
     ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
 
@@ -443,7 +445,7 @@ def main():
         "reduced_precision_error_vs_fp64": fp32_err,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": ("k_eval_fp32" if args.fp32 else "k_eval_wide" if (A > 8 or args.ring_bf16x3)
+                     "kernel": ("k_eval_fp32" if args.fp32 else "k_eval_wide" if (A > 8 or args.ring_bf16x3 or args.wide)
                                 else "k_eval_fp64"), "flops_per_sample": fl["eval"],
                      "algorithmic_bytes": B * N * P * 8 + B * (5 if pm else 3) * 8,
                      "traffic_detail": tdetail},
