@@ -59,4 +59,18 @@ __device__ __forceinline__ double shfl_xor_d(double v, int mask) {
   return __shfl_xor(v, mask, kWave);
 }
 
+// Allow `bytes` of dynamic LDS for kernel `fn` on the current device (needed above 64 KB).
+// The attribute is per device; it is set on the first launch on each device (a bit per device
+// in `done`, which the caller keeps per kernel; setting it twice is harmless).
+inline hipError_t ensure_dynamic_lds(const void* fn, size_t bytes, unsigned long long& done) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const unsigned long long bit = (dev >= 0 && dev < 64) ? (1ull << dev) : 0ull;
+  if (bit && (__atomic_load_n(&done, __ATOMIC_ACQUIRE) & bit)) return hipSuccess;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess && bit) __atomic_fetch_or(&done, bit, __ATOMIC_RELEASE);
+  return e;
+}
+
 }  // namespace pib

@@ -190,12 +190,8 @@ template <int P, int MODE>
 static hipError_t launch_whiten_pm(const BasisArgs& b, const void* in, void* out, const float* U, int num_cu,
                                    hipStream_t s) {
   const size_t lds = WhitenLds<P>::bytes;
-  static bool attr_set = false;  // more than the default 64 KB of dynamic LDS when P = 128
-  if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_whiten<P, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    attr_set = true;
-  }
+  static unsigned long long lds_set = 0;  // more than the default 64 KB of dynamic LDS when P = 128
+  if (hipError_t e = ensure_dynamic_lds((const void*)k_whiten<P, MODE>, lds, lds_set)) return e;
   const int64_t blocks = (b.B * (int64_t)b.N + 63) / 64;  // 4 waves x 16 rows
   const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)4 * num_cu));
   hipLaunchKernelGGL((k_whiten<P, MODE>), dim3((unsigned)grid), dim3(256), lds, s, b, in, out, U);

@@ -299,13 +299,8 @@ template <int H, int HID>
 hipError_t launch_hh(const FusedLayerArgs& a, hipStream_t s) {
   using S = Stages<H, HID>;
   const size_t lds = (size_t)2 * S::stage * sizeof(unsigned short);
-  static bool attr_set = false;  // more than the default 64 KB of dynamic LDS when H = 512
-  if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_flow_layer<H, HID>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    attr_set = true;
-  }
+  static unsigned long long lds_set = 0;  // 67.6 KB of dynamic LDS: above the default 64 KB
+  if (hipError_t e = ensure_dynamic_lds((const void*)k_flow_layer<H, HID>, lds, lds_set)) return e;
   hipLaunchKernelGGL((k_flow_layer<H, HID>), dim3((unsigned)(a.B / kRows)), dim3(kThreads), lds, s, a);
   return hipGetLastError();
 }
