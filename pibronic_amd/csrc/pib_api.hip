@@ -625,7 +625,7 @@ int pib_flow_layer_fused(pib_ctx* c, int inverse, int64_t B, int D, int hidden, 
   if (!z || !logdet || !W1_hi || !W1_lo || !W2_hi || !W2_lo || !W3_hi || !W3_lo || !b1 || !b2 || !b3)
     return fail(c, PIB_ERR_ARG, "flow_layer_fused: null pointer");
   if (!fused_layer_supported(B, D, hidden))
-    return fail(c, PIB_ERR_UNSUPPORTED, "flow_layer_fused: needs hidden 256, D/2 in {256, 512}, B a multiple of 32");
+    return fail(c, PIB_ERR_UNSUPPORTED, "flow_layer_fused: needs hidden 256, D = 512 and B a multiple of 128");
   const int H = D / 2;
   if (!((a_off == 0 && b_off == H) || (a_off == H && b_off == 0)))
     return fail(c, PIB_ERR_ARG, "flow_layer_fused: the halves must be (0, D/2) or (D/2, 0)");

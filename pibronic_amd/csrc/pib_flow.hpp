@@ -112,7 +112,9 @@ struct FusedLayerArgs {
   float smax;
   float* z;          // (B, D) flow state, transformed half updated in place
   double* logdet;    // (B,)
-  const unsigned short *W1h, *W1l, *W2h, *W2l, *W3h, *W3l;  // transposed (N x K) bf16 hi/lo
+  // transposed (N x K) bf16 hi/lo; W2 and W3 with their inputs permuted within blocks of 32
+  // (pib_split_bf16 with PIB_SPLIT_PERM32)
+  const unsigned short *W1h, *W1l, *W2h, *W2l, *W3h, *W3l;
   const float *b1, *b2, *b3;
 };
 bool fused_layer_supported(int64_t B, int D, int hidden);
