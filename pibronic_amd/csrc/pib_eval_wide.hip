@@ -4,27 +4,33 @@
 // Same estimator as k_eval_fp64 (pibronic/pimc/pimc.py:1087-1213 fused; block loops
 // :1341-1462), mapped for matrices that do not fit one lane's registers:
 //
-//  * Beads are processed four at a time. Lane (g, j) = (lane / 16, lane % 16) owns column j
-//    of the coupling matrix of bead 4c + g: it assembles V[:, j] from the linear and
-//    quadratic couplings, forms the O-factor of surface j for every temperature set, and
-//    takes part in the per-bead scaling maximum over its 16-lane group.
-//  * Eigen-decomposition: one-sided (Hestenes) Jacobi on G = V + sigma I, sigma = 1.5 |V|_F,
-//    which is positive definite with condition number <= 5. Its singular vectors are the
-//    eigenvectors of V and lambda_j = |G u_j| - sigma. A sweep is the 15 perfect matchings
-//    {j, j xor k}, k = 1..15 (every pair once); partners swap columns with cross-lane moves,
-//    so no rotation parameters have to be broadcast. The eigenvectors are the normalised
-//    columns of the rotated G (Demmel & Veselic, "Jacobi's method is more accurate than QR",
-//    SIAM J. Matrix Anal. Appl. 13 (1992), for the one-sided scheme).
-//  * M = U diag(exp(-tau lambda)) U^T and the ring product run on the matrix cores: 16x16x4
-//    FP64 MFMA tiles. The FP64 accumulator layout is: register r of lane l holds
-//    D[l/16 + 4r][l%16] (measured on gfx950). Permuting the sum index of each tile step
-//    (k = lane/16 + 4*step) makes that layout directly the operand layout of the next
-//    product, so the chain never leaves registers.
-//    The chain is carried transposed, R^T <- diag(O_p / S_p) M_p R^T, whose trace equals the
-//    reference's tr(prod_p M_p O_p) because M_p is symmetric and O_p diagonal.
+//  * Beads are processed four at a time. For the O-factors lane (g, j) = (lane / 16, lane % 16)
+//    takes surface j of bead 4c + g: it forms the O-factor of surface j for every temperature
+//    set and takes part in the per-bead scaling maximum over its 16-lane group.
+//  * The coupling matrices V of the four beads are assembled directly in the FP64 matrix-core
+//    accumulator layout: register r of lane l holds X[l/16 + 4r][l%16] (measured on gfx950).
+//    For a symmetric X that layout is also the operand layout of a product X Y: with the sum
+//    index of tile step k taken as lane/16 + 4k, lane l supplies X[l%16][l/16 + 4k] =
+//    X[l/16 + 4k][l%16] (its own register k) as the A operand and Y[l/16 + 4k][l%16] (register
+//    k of Y) as the B operand. Every product below chains in registers that way.
+//  * M = exp(-tau V) directly, without eigenvectors: the reference forms
+//    M = U diag(exp(-tau lambda)) U^T from eigh(V) (pimc.py:1167-1187), and M is the only
+//    consumer of the decomposition. For X = -tau V, alpha = max(|X^3|_F^(1/3), |X^4|_F^(1/4))
+//    bounds the spectral radius (Al-Mohy and Higham, SIAM J. Matrix Anal. Appl. 31 (2009) 970);
+//    the powers are scaled by 2^-s so that alpha 2^-s <= theta_12 = 0.2996, where the degree-12
+//    Taylor polynomial T_12 has a backward error below the FP64 unit roundoff (Sastre et al.,
+//    SIAM J. Sci. Comput. 37 (2015) A439); T_12 is evaluated by Paterson-Stockmeyer in the
+//    fourth power (five products in all) and squared s times. The four beads of a chunk go
+//    through it in lockstep (one s, the largest they need), so four independent product
+//    chains fill the matrix-core pipeline. V is symmetric, so every power and polynomial of
+//    it is, and the accumulator layout of each product feeds the next one.
+//  * The ring product runs on the same FP64 matrix cores, carried transposed,
+//    R^T <- diag(O_p / S_p) M_p R^T, whose trace equals the reference's tr(prod_p M_p O_p)
+//    because M_p is symmetric and O_p diagonal.
 //
-// Padding: columns j >= A hold sigma e_j, are never rotated (their inner products with real
-// columns are exactly zero) and carry O = 0, so they drop out of every trace.
+// Padding: rows and columns j >= A of V are zero, the identity of the Taylor polynomial covers
+// only the leading A x A block, so M is zero outside it; padded surfaces carry O = 0 and drop
+// out of every trace.
 #include "pib_common.hpp"
 #include "pib_kernels.hpp"
 
@@ -35,41 +41,12 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int kW = 16;                 // padded matrix dimension = lanes per bead
 constexpr int kBeadsPerChunk = kWave / kW;
 constexpr int kWideWaves = 2;          // waves (samples) per work-group
-constexpr int kURow = kW + 1;          // LDS row stride of the eigenvector tiles (bank spread)
 
 struct WideLds {
-  double U[kBeadsPerChunk][kW * kURow];  // U[b][i * kURow + m] = eigenvector m, component i
-  double D[kBeadsPerChunk][kW];          // exp(-tau lambda_m)
-  double O[3][kBeadsPerChunk][kW];       // scaled O-factors (vib, vib+, vib-)
+  double X[kBeadsPerChunk][4][kWave];  // -tau V of the chunk's beads, register r of lane l
+  double O[3][kBeadsPerChunk][kW];     // scaled O-factors (vib, vib+, vib-)
+  float Mf[kW][kW + 1];                // M in FP32 for the BF16x3 ring (layout change)
 };
-
-// hardware 1/x and 1/sqrt(x) (~2^-29) with Newton steps (see rcp_nr in the per-lane kernel):
-// two steps where orthogonality or an output depends on it, one inside the rotation angle
-template <int STEPS = 2>
-__device__ __forceinline__ double rcp_nr_w(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-#pragma unroll
-  for (int it = 0; it < STEPS; ++it) {
-    const double e = fma(-x, y, 1.0);
-    y = fma(y, e, y);
-  }
-  return y;
-}
-
-template <int STEPS = 2>
-__device__ __forceinline__ double rsq_nr_w(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-#pragma unroll
-  for (int it = 0; it < STEPS; ++it) {
-    const double e = fma(-(x * y), y, 1.0);
-    y = fma(0.5 * y, e, y);
-  }
-  return y;
-}
-
-__device__ __forceinline__ double oterm_w(double coth, double csch2, double x1, double x2) {
-  return coth * (x1 * x1 + x2 * x2) - (csch2 * x1) * x2;
-}
 
 // Wave-local ordering of LDS traffic: every access of the wave before this point is complete
 // and visible to every lane of the same wave after it.
@@ -79,8 +56,8 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// sum / max over the 16 lanes of a group; the xor butterfly gives every lane the same bits
-// (a + b == b + a exactly)
+// sum / max over the 16 lanes of a group, and sum over the wave; the xor butterfly gives every
+// lane the same bits (a + b == b + a exactly)
 __device__ __forceinline__ double group_sum(double x) {
 #pragma unroll
   for (int m = 1; m < kW; m <<= 1) x += shfl_xor_d(x, m);
@@ -90,6 +67,97 @@ __device__ __forceinline__ double group_max(double x) {
 #pragma unroll
   for (int m = 1; m < kW; m <<= 1) x = fmax(x, shfl_xor_d(x, m));
   return x;
+}
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) x += shfl_xor_d(x, m);
+  return x;
+}
+
+// Sum over the wave with the same bits on every lane, without LDS: within each row of 16
+// lanes the quad permutations [1,0,3,2] and [2,3,0,1], the half-row mirror and the row mirror
+// give every lane its row's sum (the two lanes of each exchange add the same two values); the
+// four row sums are read from lanes 0, 16, 32 and 48 and added in a fixed order.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move_d(double x) {
+  const long long u = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const long long u = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)u, l);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double x) {
+  x += dpp_move_d<0xB1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_move_d<0x4E>(x);   // quad_perm [2,3,0,1]
+  x += dpp_move_d<0x141>(x);  // row_half_mirror
+  x += dpp_move_d<0x140>(x);  // row_mirror
+  return (readlane_d(x, 0) + readlane_d(x, 16)) + (readlane_d(x, 32) + readlane_d(x, 48));
+}
+
+// Z + X Y for a symmetric X, all in the FP64 accumulator layout
+__device__ __forceinline__ d4 sym_mul(const d4& X, const d4& Y, d4 Z) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) Z = __builtin_amdgcn_mfma_f64_16x16x4f64(X[k], Y[k], Z, 0, 0, 0);
+  return Z;
+}
+__device__ __forceinline__ d4 sym_mul(const d4& X, const d4& Y) {
+  return sym_mul(X, Y, d4{0.0, 0.0, 0.0, 0.0});
+}
+
+constexpr double kLog2Theta12 = -1.738798;  // log2(0.2996158)
+
+// log2(x) to ~1e-7 (frexp and the hardware log2 of the mantissa); 0 gives -inf, NaN NaN
+__device__ __forceinline__ double log2_coarse(double x) {
+  int e = 0;
+  const double m = frexp(x, &e);
+  return (double)e + (double)__log2f((float)m);
+}
+
+// exp(X) for a symmetric X in the FP64 accumulator layout, zero outside its leading n x n
+// block (scaling and squaring of the degree-12 Taylor polynomial; see the file header). The
+// result is zero outside the leading block.
+__device__ __forceinline__ d4 expm_sym(const d4& X, int n, int g, int j) {
+  const bool dl = ((j & 3) == g) && (j < n);  // register j / 4 of this lane holds X[j][j]
+  const int dr = j >> 2;
+  const d4 X2 = sym_mul(X, X);
+  const d4 X3 = sym_mul(X2, X);
+  d4 X4 = sym_mul(X2, X2);
+  double n3 = 0.0, n4 = 0.0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    n3 = fma(X3[r], X3[r], n3);
+    n4 = fma(X4[r], X4[r], n4);
+  }
+  n3 = wave_sum_dpp(n3);
+  n4 = wave_sum_dpp(n4);
+  // smallest s >= 0 with |X^3|^(1/3) 2^-s and |X^4|^(1/4) 2^-s <= theta_12 (log2 |X^k|_F =
+  // log2(n_k) / 2); a zero matrix gives -inf, a NaN s = 0, the count is capped at 64
+  const double sd = fmax(log2_coarse(n3) * (1.0 / 6.0), log2_coarse(n4) * 0.125) - kLog2Theta12;
+  int s = 0;
+  if (sd > 0.0) s = sd < 64.0 ? (int)ceil(sd) : 64;
+  s = __builtin_amdgcn_readfirstlane(s);  // identical on every lane
+  constexpr double c3 = 1.0 / 6, c4 = 1.0 / 24, c5 = 1.0 / 120, c6 = 1.0 / 720, c7 = 1.0 / 5040,
+                   c8 = 1.0 / 40320, c9 = 1.0 / 362880, c10 = 1.0 / 3628800,
+                   c11 = 1.0 / 39916800, c12 = 1.0 / 479001600;
+  d4 B0, B1, B2;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const double x1 = ldexp(X[r], -s), x2 = ldexp(X2[r], -2 * s), x3 = ldexp(X3[r], -3 * s);
+    X4[r] = ldexp(X4[r], -4 * s);
+    const double I = (dl && r == dr) ? 1.0 : 0.0;
+    // T_12 = B0 + X^4 (B1 + X^4 B2), B_i cubic (B2 also takes the X^4 / 12! term)
+    B2[r] = fma(c12, X4[r], fma(c11, x3, fma(c10, x2, fma(c9, x1, c8 * I))));
+    B1[r] = fma(c7, x3, fma(c6, x2, fma(c5, x1, c4 * I)));
+    B0[r] = fma(c3, x3, fma(0.5, x2, x1 + I));
+  }
+  d4 T = sym_mul(X4, sym_mul(X4, B2, B1), B0);
+  for (int k = 0; k < s; ++k) T = sym_mul(T, T);
+  return T;
 }
 
 // BF16x3 ring products: FP32 values split into bf16 hi + lo, products on the bf16 matrix cores
@@ -108,11 +176,15 @@ __device__ __forceinline__ void split_bf(float x, short& h, short& l) {
   h = (short)hb;
   l = (short)f2bf_w(x - __uint_as_float((unsigned)hb << 16));
 }
-__device__ __forceinline__ f4w mfma_bf16x3(const s4w& ah, const s4w& al, const s4w& bh, const s4w& bl,
-                                           f4w c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, c, 0, 0, 0);
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, c, 0, 0, 0);
+// the three partial products go to separate zero accumulators and are added on the VALU: a
+// chain through the accumulator lets the register allocator give a product a destination that
+// partly overlaps its accumulator input, which the matrix cores do not compute correctly
+__device__ __forceinline__ f4w mfma_bf16x3(const s4w& ah, const s4w& al, const s4w& bh, const s4w& bl) {
+  const f4w z = {0.f, 0.f, 0.f, 0.f};
+  const f4w lo1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, z, 0, 0, 0);
+  const f4w lo2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, z, 0, 0, 0);
+  const f4w hi = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, z, 0, 0, 0);
+  return (lo1 + lo2) + hi;
 }
 // scale a 16x16 FP32 matrix (one wave, 4 values per lane) by a power of two so that its largest
 // magnitude lies in [0.5, 1); the exponent (the same on every lane) goes to e
@@ -130,111 +202,29 @@ __device__ __forceinline__ void renorm_wave(f4w& R, int& e) {
   }
 }
 
-template <int K>
-struct HiBit {
-  static constexpr int v = (K >= 8) ? 8 : (K >= 4) ? 4 : (K >= 2) ? 2 : 1;
-};
-
-// One step of the sweep: lane j and lane j ^ K orthogonalise their columns.
-template <int K>
-__device__ __forceinline__ void hestenes_step(double (&v)[kW], double& al, int j, double tol2,
-                                              bool& busy) {
-  double pc[kW];
-#pragma unroll
-  for (int i = 0; i < kW; ++i) pc[i] = shfl_xor_d(v[i], K);
-  const double be = shfl_xor_d(al, K);
-  double gam = 0.0;
-#pragma unroll
-  for (int i = 0; i < kW; ++i) gam = fma(v[i], pc[i], gam);
-  const bool low = (j & HiBit<K>::v) == 0;  // this lane holds the lower column index p
-  const double app = low ? al : be, aqq = low ? be : al;
-  // Gram block [[app, gam], [gam, aqq]]: t = tan of the rotation that zeroes gam (smaller
-  // root, |t| <= 1); gam == 0 gives t = 0, an exact no-op (see jacobi_rotate)
-  const double d = aqq - app, h = 2.0 * gam;
-  const double x = fma(d, d, h * h);
-  const double r = x * rsq_nr_w<1>(x > 0.0 ? x : 1.0);
-  const double den = fabs(d) + r;
-  double t = h * rcp_nr_w<1>(den > 0.0 ? den : 1.0);
-  t = d < 0.0 ? -t : t;
-  const double c = rsq_nr_w<2>(fma(t, t, 1.0));
-  const double s = t * c;
-  busy |= (gam * gam > tol2 * (app * aqq));
-  const double sg = low ? -s : s;  // p: c g_p - s g_q ; q: s g_p + c g_q
-#pragma unroll
-  for (int i = 0; i < kW; ++i) v[i] = fma(c, v[i], sg * pc[i]);
-  // the rotation maps the Gram block to diag(app - t gam, aqq + t gam) (as a_pp - t a_pq in
-  // the two-sided method); the norms are recomputed exactly at every sweep start
-  al = low ? fma(-t, gam, al) : fma(t, gam, al);
-}
-
-// Round r of a 12-column round-robin (circle method) sweep, for A <= 12: rows and columns
-// 12..15 of G hold only padding (zero rows under the real columns, sigma e_j columns whose
-// rotations are no-ops), so each round pairs the twelve real-or-padding columns below 12 --
-// lane 11 with lane r, lane j with lane (2r - j) mod 11 otherwise -- and touches twelve rows.
-// 11 rounds of 6 disjoint pairs replace the 15 xor matchings of the 16-column sweep. Lanes
-// 12..15 pair with themselves (no-op).
-__device__ __forceinline__ void hestenes_round12(double (&v)[kW], double& al, int j, int r, int group,
-                                                 double tol2, bool& busy) {
-  constexpr int kR = 12;
-  const int pj = (j >= kR) ? j : (j == kR - 1) ? r : (j == r) ? kR - 1 : (2 * r - j + 2 * (kR - 1)) % (kR - 1);
-  const int src = group * kW + pj;
-  double pc[kR];
-#pragma unroll
-  for (int i = 0; i < kR; ++i) pc[i] = __shfl(v[i], src, kWave);
-  const double be = __shfl(al, src, kWave);
-  double gam = 0.0;
-#pragma unroll
-  for (int i = 0; i < kR; ++i) gam = fma(v[i], pc[i], gam);
-  const bool self = pj == j;
-  const bool low = j < pj;  // this lane holds the lower column index p
-  const double app = low ? al : be, aqq = low ? be : al;
-  const double d = aqq - app, h = self ? 0.0 : 2.0 * gam;
-  const double x = fma(d, d, h * h);
-  const double rr = x * rsq_nr_w<1>(x > 0.0 ? x : 1.0);
-  const double den = fabs(d) + rr;
-  double t = h * rcp_nr_w<1>(den > 0.0 ? den : 1.0);
-  t = d < 0.0 ? -t : t;
-  const double c = rsq_nr_w<2>(fma(t, t, 1.0));
-  const double sn = t * c;
-  busy |= !self && (gam * gam > tol2 * (app * aqq));
-  const double sg = low ? -sn : sn;  // p: c g_p - s g_q ; q: s g_p + c g_q
-#pragma unroll
-  for (int i = 0; i < kR; ++i) v[i] = fma(c, v[i], sg * pc[i]);
-  al = self ? al : (low ? fma(-t, gam, al) : fma(t, gam, al));
-}
-
-__device__ __forceinline__ double column_norm2_12(const double (&v)[kW]) {
-  double a2 = 0.0;
-#pragma unroll
-  for (int i = 0; i < 12; ++i) a2 = fma(v[i], v[i], a2);
-  return a2;
-}
-
-__device__ __forceinline__ double column_norm2(const double (&v)[kW]) {
-  double a2 = 0.0;
-#pragma unroll
-  for (int i = 0; i < kW; ++i) a2 = fma(v[i], v[i], a2);
-  return a2;
-}
-
-// One sample per wave; L is the wave's private LDS tile set. BF: ring products (and M) as
-// BF16x3 on the bf16 matrix cores with FP32 accumulation, renormalised every bead.
-template <bool PM, bool DBG, bool BF = false, bool C12 = false>
+// One sample per wave; L is the wave's private LDS tile set. BF: ring products as BF16x3 on
+// the bf16 matrix cores with FP32 accumulation, renormalised every bead (M stays FP64).
+// One sample per wave; L is the wave's private LDS tile set. BF: ring products as BF16x3 on
+// the bf16 matrix cores with FP32 accumulation, renormalised every bead (M stays FP64). NR:
+// rows g + 4r, r < NR, of V are assembled (NR = ceil(A / 4); rows >= A are zero padding).
+template <bool PM, bool DBG, bool BF, int NR>
 __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t sample, WideLds& L) {
   const int lane = threadIdx.x & (kWave - 1);
   const int g = lane / kW, j = lane % kW;
   const int A = args.A, AR = args.AR, N = args.N, P = args.P;
-  const bool sep = AR != A;
   const bool jr = j < AR, jv = j < A;
+  const int jrc = jr ? j : AR - 1, jvc = jv ? j : A - 1;  // in-bounds surface of this lane
   const double* __restrict__ qs = args.q + sample * (int64_t)N * P;
   const OSet sr = args.set[SET_RHO], sv = args.set[SET_VIB];
   const OSet sp = args.set[SET_VIB_PLUS], sm = args.set[SET_VIB_MINUS];
+  const ModeRec R(A, AR);
+  const int rl = R.len();
+  const double* __restrict__ tab = args.mode_tab;
   const double2* __restrict__ LQ = args.wide_lq;
   const double* __restrict__ Ew = args.wide_e;
   const int* __restrict__ pairs = args.pair;
   const double* __restrict__ Qp = args.wide_qpair;
-  const double tol2 = 1e-26;  // |g_p . g_q| <= 1e-13 |g_p| |g_q| for every pair ends the sweeps
-  constexpr int kMaxSweeps = 16;
+  const double ntau = -args.tau;
 
   // transposed ring products in accumulator layout: reg r of lane l = R^T[l/16 + 4r][l%16]
   d4 Rg, Rp, Rm;
@@ -258,64 +248,92 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
   int sexp = 0;
 
   for (int p0 = 0; p0 < P; p0 += kBeadsPerChunk) {
+    const int nb = min(kBeadsPerChunk, P - p0);
     const int p = p0 + g;
     const bool bead = p < P;  // uniform over the 16-lane group
     const int pc_ = bead ? p : P - 1;
     const int pn = (pc_ + 1 == P) ? 0 : pc_ + 1;
+    int pb[kBeadsPerChunk];  // the chunk's beads (clamped; results past P are not used)
+#pragma unroll
+    for (int b = 0; b < kBeadsPerChunk; ++b) pb[b] = min(p0 + b, P - 1);
 
+    // O-exponents in the expanded form of the lanes kernel (see OSet): surface-independent
+    // parts t (identical on the lanes of a group) and this lane's surface's linear part e
+    double tr = 0.0, tv = 0.0, tp = 0.0, tm = 0.0;
     double er = 0.0, ev = 0.0, ep = 0.0, em = 0.0;
-    double v[kW];
+    d4 v[kBeadsPerChunk];  // V of the chunk's beads, accumulator layout
 #pragma unroll
-    for (int i = 0; i < kW; ++i) v[i] = 0.0;
+    for (int b = 0; b < kBeadsPerChunk; ++b) v[b] = d4{0.0, 0.0, 0.0, 0.0};
+    // the next mode's per-lane loads are issued while this mode's arithmetic runs
+    double x0 = qs[pc_], xn = qs[pn];
+    double2 c[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) c[r] = LQ[(g + 4 * r) * kW + j];
     for (int d = 0; d < N; ++d) {
-      const double x0 = qs[d * P + pc_], xn = qs[d * P + pn];
-      double q1 = x0, q1n = xn;
-      if (jr) {  // sampling model: q -= d_rho; O_rho; q += d_rho (pimc.py:1091-1127)
-        const double dd = sr.shift[j * N + d];
-        const double x1 = x0 - dd, x2 = xn - dd;
-        er += oterm_w(sr.coth[d], 2.0 * sr.csch[d], x1, x2);
-        if (!sep) { q1 = x1 + dd; q1n = x2 + dd; }
-      }
-      if (jv) {  // coupled model at tau (and tau +- dbeta/P)
-        const double dd = sv.shift[j * N + d];
-        const double y1 = q1 - dd, y2 = q1n - dd;
-        ev += oterm_w(sv.coth[d], 2.0 * sv.csch[d], y1, y2);
-        if (PM) {
-          const double z1 = (y1 + dd) - dd, z2 = (y2 + dd) - dd;
-          ep += oterm_w(sp.coth[d], 2.0 * sp.csch[d], z1, z2);
-          const double w1 = (z1 + dd) - dd, w2 = (z2 + dd) - dd;
-          em += oterm_w(sm.coth[d], 2.0 * sm.csch[d], w1, w2);
-        }
-      }
-      // column j of V (lower triangle mirrored): sum_d x_d (L + Qh x_d)  (pimc.py:1147-1160)
-      const double2* lq = LQ + (size_t)d * kW * kW + j;
+      const double* __restrict__ t = tab + d * rl;
+      const double* qd = qs + d * P;
+      const int dn = (d + 1 < N) ? d + 1 : d;
+      const double x0n = qs[dn * P + pc_], xnn = qs[dn * P + pn];
+      double2 cn[NR];
 #pragma unroll
-      for (int i = 0; i < kW; ++i) {
-        if (i < A) {
-          const double2 c = lq[i * kW];
-          v[i] = fma(x0, fma(c.y, x0, c.x), v[i]);
-        }
+      for (int r = 0; r < NR; ++r) cn[r] = LQ[((size_t)dn * kW + g + 4 * r) * kW + j];
+
+      const double sx = x0 + xn, dx = x0 - xn;
+      const double q2s = fma(x0, x0, xn * xn), d2 = dx * dx;
+      tr = fma(t[1], d2, fma(t[0], q2s, tr));
+      tv = fma(t[3], d2, fma(t[2], q2s, tv));
+      er = fma(t[R.lin_rho() + jrc], sx, er);
+      ev = fma(t[R.lin_vib() + jvc], sx, ev);
+      if (PM) {
+        tp = fma(t[R.pm() + 1], d2, fma(t[R.pm() + 0], q2s, tp));
+        tm = fma(t[R.pm() + 3], d2, fma(t[R.pm() + 2], q2s, tm));
+        ep = fma(t[R.lin_plus() + jvc], sx, ep);
+        em = fma(t[R.lin_minus() + jvc], sx, em);
       }
+      // V[i][j] += x_d (L + Qh x_d), i = g + 4r (pimc.py:1147-1160)
+      double xb[kBeadsPerChunk];
+#pragma unroll
+      for (int b = 0; b < kBeadsPerChunk; ++b) xb[b] = qd[pb[b]];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+#pragma unroll
+        for (int b = 0; b < kBeadsPerChunk; ++b) v[b][r] = fma(xb[b], fma(c[r].y, xb[b], c[r].x), v[b][r]);
+      }
+      x0 = x0n;
+      xn = xnn;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) c[r] = cn[r];
     }
     for (int t = 0; t < args.npair; ++t) {  // off-diagonal quadratic mode pairs
       const int jj = pairs[2 * t], kk = pairs[2 * t + 1];
-      const double xj = qs[jj * P + pc_], xk = qs[kk * P + pc_];
-      const double* qp = Qp + (size_t)t * kW * kW + j;
+      double xj[kBeadsPerChunk], xk[kBeadsPerChunk];
 #pragma unroll
-      for (int i = 0; i < kW; ++i)
-        if (i < A) v[i] = fma(xk * qp[i * kW], xj, v[i]);
+      for (int b = 0; b < kBeadsPerChunk; ++b) {
+        xj[b] = qs[jj * P + pb[b]];
+        xk[b] = qs[kk * P + pb[b]];
+      }
+      const double* qp = Qp + (size_t)t * kW * kW + g * kW + j;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const double cq = qp[4 * r * kW];
+#pragma unroll
+        for (int b = 0; b < kBeadsPerChunk; ++b) v[b][r] = fma(xk[b] * cq, xj[b], v[b][r]);
+      }
     }
 #pragma unroll
-    for (int i = 0; i < kW; ++i)
-      if (i < A) v[i] += Ew[i * kW + j];
+    for (int r = 0; r < NR; ++r) {
+      const double e = Ew[(g + 4 * r) * kW + j];
+#pragma unroll
+      for (int b = 0; b < kBeadsPerChunk; ++b) v[b][r] += e;
+    }
 
     // O-factors and the per-bead scaling S_p = max over both O's (pimc.py:1076-1084)
-    const double Or = jr ? exp(-0.5 * er) * sr.pref[j] : 0.0;
-    const double Ov = jv ? exp(-0.5 * ev) * sv.pref[j] : 0.0;
+    const double Or = jr ? exp(-0.5 * ((tr + er) + sr.k2[jrc])) * sr.pref[jrc] : 0.0;
+    const double Ov = jv ? exp(-0.5 * ((tv + ev) + sv.k2[jvc])) * sv.pref[jvc] : 0.0;
     double Opl = 0.0, Omi = 0.0;
     if (PM && jv) {
-      Opl = exp(-0.5 * ep) * sp.pref[j];
-      Omi = exp(-0.5 * em) * sm.pref[j];
+      Opl = exp(-0.5 * ((tp + ep) + sp.k2[jvc])) * sp.pref[jvc];
+      Omi = exp(-0.5 * ((tm + em) + sm.k2[jvc])) * sm.pref[jvc];
     }
     double S = group_max(fmax(Or, Ov));
     if (args.noscale) S = 1.0;
@@ -328,51 +346,13 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
       if (j == 0) dr[AR + A] = S;
     }
 
-    // G = V + sigma I, one-sided Jacobi
-    double f = 0.0;
-#pragma unroll
-    for (int i = 0; i < kW; ++i) f = fma(v[i], v[i], f);
-    f = group_sum(f);
-    const double sig = 1.5 * (f * rsq_nr_w(f > 0.0 ? f : 1.0)) + 1e-100;
-#pragma unroll
-    for (int i = 0; i < kW; ++i) v[i] += (i == j) ? sig : 0.0;
-    double al = 0.0;
-    if (C12) {  // A <= 12
-      for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
-        al = column_norm2_12(v);
-        bool busy = false;
-        for (int r = 0; r < 11; ++r) hestenes_round12(v, al, j, r, g, tol2, busy);
-        if (!__any(busy)) break;
-      }
-    } else for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
-      al = column_norm2(v);
-      bool busy = false;
-      hestenes_step<1>(v, al, j, tol2, busy);
-      hestenes_step<2>(v, al, j, tol2, busy);
-      hestenes_step<3>(v, al, j, tol2, busy);
-      hestenes_step<4>(v, al, j, tol2, busy);
-      hestenes_step<5>(v, al, j, tol2, busy);
-      hestenes_step<6>(v, al, j, tol2, busy);
-      hestenes_step<7>(v, al, j, tol2, busy);
-      hestenes_step<8>(v, al, j, tol2, busy);
-      hestenes_step<9>(v, al, j, tol2, busy);
-      hestenes_step<10>(v, al, j, tol2, busy);
-      hestenes_step<11>(v, al, j, tol2, busy);
-      hestenes_step<12>(v, al, j, tol2, busy);
-      hestenes_step<13>(v, al, j, tol2, busy);
-      hestenes_step<14>(v, al, j, tol2, busy);
-      hestenes_step<15>(v, al, j, tol2, busy);
-      if (!__any(busy)) break;
-    }
-    al = column_norm2(v);
-    const double rs = rsq_nr_w(al);
-    const double lam = al * rs - sig;
     const double inv = 1.0 / S;
-
-    wave_sync();  // the previous chunk's ring steps have read the tiles
+    wave_sync();  // the previous chunk's ring steps have read the X and O tiles
 #pragma unroll
-    for (int i = 0; i < kW; ++i) L.U[g][i * kURow + j] = v[i] * rs;
-    L.D[g][j] = exp(-args.tau * lam);
+    for (int b = 0; b < kBeadsPerChunk; ++b) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) L.X[b][r][lane] = ntau * v[b][r];
+    }
     L.O[0][g][j] = Ov * inv;
     if (PM) {
       L.O[1][g][j] = Opl * inv;
@@ -386,96 +366,68 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
     }
     wave_sync();
 
-    // ring steps, beads in order (pimc.py:1198-1206)
-    const int nb = min(kBeadsPerChunk, P - p0);
+    // M_p = exp(-tau V_p) and the ring steps, beads in order (pimc.py:1181-1206); one bead's
+    // exponential at a time keeps the register count (and so the waves per SIMD) in check
+#pragma clang loop unroll(disable)
     for (int b = 0; b < nb; ++b) {
-      if constexpr (BF) {
-        // M = (U diag(d)) U^T: A[i][k] = U[i][k] d_k, B[k][i'] = U[i'][k], k = 4(l/16) + e
-        s4w ah, al, bh, bl;
+      {
+        d4 X;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int k = 4 * g + e;
-          const double u = L.U[b][j * kURow + k];
-          short h, l;
-          split_bf((float)(u * L.D[b][k]), h, l);
-          ah[e] = h;
-          al[e] = l;
-          split_bf((float)u, h, l);
-          bh[e] = h;
-          bl[e] = l;
-        }
-        const f4w M = mfma_bf16x3(ah, al, bh, bl, f4w{0.f, 0.f, 0.f, 0.f});
+        for (int r = 0; r < 4; ++r) X[r] = L.X[b][r][lane];
+        const d4 M = expm_sym(X, A, g, j);
         if (DBG) {
           const int rec = eval_debug_record(A, AR);
           double* dr = args.dbg + ((int64_t)sample * P + p0 + b) * rec + AR + A + 1;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int row = 4 * g + r;
-            if (row < A && j < A) dr[row * A + j] = (double)M[r];
+            const int row = g + 4 * r;
+            if (row < A && j < A) dr[row * A + j] = M[r];
           }
         }
-        // Z = M R^T: the A operand M[j][4(l/16)+e] is M's own register e (M symmetric), the
-        // B operand is the ring state's register e
-        s4w mh, ml;
+        if constexpr (BF) {
+          // M into the FP32 layout (reg e = M[4(l/16)+e][l%16] = M[l%16][4(l/16)+e]), split
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          short h, l;
-          split_bf(M[e], h, l);
-          mh[e] = h;
-          ml[e] = l;
-        }
-#pragma unroll
-        for (int w = 0; w < (PM ? 3 : 1); ++w) {
-          f4w& F = (w == 0) ? Fg : (w == 1) ? Fp : Fm;
-          int& fe = (w == 0) ? xg : (w == 1) ? xp : xm;
-          s4w rh, rl;
+          for (int r = 0; r < 4; ++r) L.Mf[g + 4 * r][j] = (float)M[r];
+          wave_sync();
+          s4w mh, ml;
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             short h, l;
-            split_bf(F[e], h, l);
-            rh[e] = h;
-            rl[e] = l;
+            split_bf(L.Mf[4 * g + e][j], h, l);
+            mh[e] = h;
+            ml[e] = l;
           }
-          const f4w Z = mfma_bf16x3(mh, ml, rh, rl, f4w{0.f, 0.f, 0.f, 0.f});
+          wave_sync();  // every lane has read Mf before the next bead writes it
 #pragma unroll
-          for (int r = 0; r < 4; ++r) F[r] = Z[r] * (float)L.O[w][b][4 * g + r];
-          renorm_wave(F, fe);
+          for (int w = 0; w < (PM ? 3 : 1); ++w) {
+            f4w& F = (w == 0) ? Fg : (w == 1) ? Fp : Fm;
+            int& fe = (w == 0) ? xg : (w == 1) ? xp : xm;
+            s4w rh, rl;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              short h, l;
+              split_bf(F[e], h, l);
+              rh[e] = h;
+              rl[e] = l;
+            }
+            const f4w Z = mfma_bf16x3(mh, ml, rh, rl);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) F[r] = Z[r] * (float)L.O[w][b][4 * g + r];
+            renorm_wave(F, fe);
+          }
+        } else {
+          const d4 Z = sym_mul(M, Rg);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) Rg[r] = Z[r] * L.O[0][b][g + 4 * r];
+          if (PM) {
+            const d4 Zp = sym_mul(M, Rp), Zm = sym_mul(M, Rm);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              Rp[r] = Zp[r] * L.O[1][b][g + 4 * r];
+              Rm[r] = Zm[r] * L.O[2][b][g + 4 * r];
+            }
+          }
         }
-      } else {
-      d4 M = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int kb = 0; kb < 4; ++kb) {
-        const int m = g + 4 * kb;
-        const double u = L.U[b][j * kURow + m];
-        M = __builtin_amdgcn_mfma_f64_16x16x4f64(u * L.D[b][m], u, M, 0, 0, 0);
-      }
-      if (DBG) {
-        const int rec = eval_debug_record(A, AR);
-        double* dr = args.dbg + ((int64_t)sample * P + p0 + b) * rec + AR + A + 1;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = g + 4 * r;
-          if (row < A && j < A) dr[row * A + j] = M[r];
-        }
-      }
-      d4 Z = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int kb = 0; kb < 4; ++kb) Z = __builtin_amdgcn_mfma_f64_16x16x4f64(M[kb], Rg[kb], Z, 0, 0, 0);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) Rg[r] = Z[r] * L.O[0][b][g + 4 * r];
-      if (PM) {
-        d4 Zp = {0.0, 0.0, 0.0, 0.0}, Zm = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-          Zp = __builtin_amdgcn_mfma_f64_16x16x4f64(M[kb], Rp[kb], Zp, 0, 0, 0);
-          Zm = __builtin_amdgcn_mfma_f64_16x16x4f64(M[kb], Rm[kb], Zm, 0, 0, 0);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          Rp[r] = Zp[r] * L.O[1][b][g + 4 * r];
-          Rm[r] = Zm[r] * L.O[2][b][g + 4 * r];
-        }
-      }
       }
     }
   }
@@ -510,13 +462,10 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
   double rho = group_sum(jr ? rp : 0.0);
   ls += shfl_xor_d(ls, 16);
   ls += shfl_xor_d(ls, 32);
-#pragma unroll
-  for (int m = 1; m < kWave; m <<= 1) {
-    tg += shfl_xor_d(tg, m);
-    if (PM) {
-      tp += shfl_xor_d(tp, m);
-      tm += shfl_xor_d(tm, m);
-    }
+  tg = wave_sum(tg);
+  if (PM) {
+    tp = wave_sum(tp);
+    tm = wave_sum(tm);
   }
   if (BF) {  // exponents are uniform over the wave
     tg = ldexp(tg, xg);
@@ -535,19 +484,24 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
   }
 }
 
-template <bool PM, bool DBG, bool BF, bool C12>
-__global__ void __launch_bounds__(kWave * kWideWaves) k_eval_wide(const EvalArgs args) {
+// Occupancy: the plain kernel for A <= 12 fits 168 VGPRs (three waves per SIMD; 7.3 against
+// 9.3 ms at the compiler's default register budget, A=12, N=24, P=128, measured on MI355X);
+// the +-dbeta ring states and the other variants keep two waves, where they do not spill.
+template <bool PM, bool DBG, bool BF, int NR>
+__global__ void __launch_bounds__(kWave * kWideWaves)
+__attribute__((amdgpu_waves_per_eu((PM || DBG || BF || NR == 4) ? 2 : 3)))
+k_eval_wide(const EvalArgs args) {
   __shared__ WideLds lds_all[kWideWaves];
-  const int wv = threadIdx.x / kWave;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int64_t sample = (int64_t)blockIdx.x * kWideWaves + wv;
   if (sample >= args.B) return;  // whole wave; no work-group barrier is used
-  eval_wide_sample<PM, DBG, BF, C12>(args, sample, lds_all[wv]);
+  eval_wide_sample<PM, DBG, BF, NR>(args, sample, lds_all[wv]);
 }
 
 // Persistent batched launch (see k_eval_batched): a tile is a range of samples ("waves") of
 // one problem; problems may differ in A, A_rho, N and P (all runtime here).
 template <bool PM>
-__global__ void __launch_bounds__(kWave * kWideWaves)
+__global__ void __launch_bounds__(kWave * kWideWaves) __attribute__((amdgpu_waves_per_eu(2)))
 k_eval_wide_batched(const EvalArgs* __restrict__ probs, const BatchTile* __restrict__ tiles, int ntiles,
                     int* counter) {
   __shared__ WideLds lds_all[kWideWaves];
@@ -561,12 +515,7 @@ k_eval_wide_batched(const EvalArgs* __restrict__ probs, const BatchTile* __restr
     const BatchTile tile = tiles[t];
     const EvalArgs& a = probs[tile.prob];
     for (int64_t smp = tile.wave0; smp < tile.wave0 + tile.nwaves; ++smp) {
-      if (smp < a.B) {
-        if (a.A <= 12)
-          eval_wide_sample<PM, false, false, true>(a, smp, L);
-        else
-          eval_wide_sample<PM, false, false, false>(a, smp, L);
-      }
+      if (smp < a.B) eval_wide_sample<PM, false, false, 4>(a, smp, L);
       wave_sync();
     }
   }
@@ -588,15 +537,25 @@ hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s, bool bf16x
   const dim3 grid((unsigned)((a.B + kWideWaves - 1) / kWideWaves));
   const dim3 block(kWave * kWideWaves);
   const bool dbg = a.dbg != nullptr;
-  const bool c12 = a.A <= 12;
-#define PIB_WIDE(pm_, dbg_, bf_) \
-  if ((pm != 0) == pm_ && dbg == dbg_ && bf16x3 == bf_) { \
-    if (c12) hipLaunchKernelGGL((k_eval_wide<pm_, dbg_, bf_, true>), grid, block, 0, s, a); \
-    else hipLaunchKernelGGL((k_eval_wide<pm_, dbg_, bf_, false>), grid, block, 0, s, a); \
+  // rows of V assembled: ceil(A / 4) in the plain and +-dbeta kernels, all 16 otherwise
+  const int nr = (a.A + 3) / 4;
+  if (!dbg && !bf16x3) {
+#define PIB_WIDE_NR(pm_, nr_) \
+  if ((pm != 0) == pm_ && nr == nr_) { \
+    hipLaunchKernelGGL((k_eval_wide<pm_, false, false, nr_>), grid, block, 0, s, a); \
     return hipGetLastError(); \
   }
-  PIB_WIDE(false, false, false) PIB_WIDE(true, false, false) PIB_WIDE(false, true, false)
-  PIB_WIDE(true, true, false) PIB_WIDE(false, false, true) PIB_WIDE(true, false, true)
+    PIB_WIDE_NR(false, 1) PIB_WIDE_NR(false, 2) PIB_WIDE_NR(false, 3) PIB_WIDE_NR(false, 4)
+    PIB_WIDE_NR(true, 1) PIB_WIDE_NR(true, 2) PIB_WIDE_NR(true, 3) PIB_WIDE_NR(true, 4)
+#undef PIB_WIDE_NR
+    return hipErrorInvalidValue;
+  }
+#define PIB_WIDE(pm_, dbg_, bf_) \
+  if ((pm != 0) == pm_ && dbg == dbg_ && bf16x3 == bf_) { \
+    hipLaunchKernelGGL((k_eval_wide<pm_, dbg_, bf_, 4>), grid, block, 0, s, a); \
+    return hipGetLastError(); \
+  }
+  PIB_WIDE(false, true, false) PIB_WIDE(true, true, false) PIB_WIDE(false, false, true) PIB_WIDE(true, false, true)
   PIB_WIDE(false, true, true) PIB_WIDE(true, true, true)
 #undef PIB_WIDE
   return hipErrorInvalidValue;
