@@ -99,14 +99,18 @@ __device__ __forceinline__ double wave_sum_dpp(double x) {
   return (readlane_d(x, 0) + readlane_d(x, 16)) + (readlane_d(x, 32) + readlane_d(x, 48));
 }
 
-// Z + X Y for a symmetric X, all in the FP64 accumulator layout
+// Z + X Y for a symmetric X, all in the FP64 accumulator layout. Tile step k sums over the
+// indices 4k..4k+3; with NR < 4 the indices from 4 NR on are zero padding of both operands and
+// their steps are skipped.
+template <int NR = 4>
 __device__ __forceinline__ d4 sym_mul(const d4& X, const d4& Y, d4 Z) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) Z = __builtin_amdgcn_mfma_f64_16x16x4f64(X[k], Y[k], Z, 0, 0, 0);
+  for (int k = 0; k < NR; ++k) Z = __builtin_amdgcn_mfma_f64_16x16x4f64(X[k], Y[k], Z, 0, 0, 0);
   return Z;
 }
+template <int NR = 4>
 __device__ __forceinline__ d4 sym_mul(const d4& X, const d4& Y) {
-  return sym_mul(X, Y, d4{0.0, 0.0, 0.0, 0.0});
+  return sym_mul<NR>(X, Y, d4{0.0, 0.0, 0.0, 0.0});
 }
 
 constexpr double kLog2Theta12 = -1.738798;  // log2(0.2996158)
@@ -119,14 +123,15 @@ __device__ __forceinline__ double log2_coarse(double x) {
 }
 
 // exp(X) for a symmetric X in the FP64 accumulator layout, zero outside its leading n x n
-// block (scaling and squaring of the degree-12 Taylor polynomial; see the file header). The
-// result is zero outside the leading block.
+// block, n <= 4 NR (scaling and squaring of the degree-12 Taylor polynomial; see the file
+// header). The result is zero outside the leading block.
+template <int NR>
 __device__ __forceinline__ d4 expm_sym(const d4& X, int n, int g, int j) {
   const bool dl = ((j & 3) == g) && (j < n);  // register j / 4 of this lane holds X[j][j]
   const int dr = j >> 2;
-  const d4 X2 = sym_mul(X, X);
-  const d4 X3 = sym_mul(X2, X);
-  d4 X4 = sym_mul(X2, X2);
+  const d4 X2 = sym_mul<NR>(X, X);
+  const d4 X3 = sym_mul<NR>(X2, X);
+  d4 X4 = sym_mul<NR>(X2, X2);
   double n3 = 0.0, n4 = 0.0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -155,8 +160,8 @@ __device__ __forceinline__ d4 expm_sym(const d4& X, int n, int g, int j) {
     B1[r] = fma(c7, x3, fma(c6, x2, fma(c5, x1, c4 * I)));
     B0[r] = fma(c3, x3, fma(0.5, x2, x1 + I));
   }
-  d4 T = sym_mul(X4, sym_mul(X4, B2, B1), B0);
-  for (int k = 0; k < s; ++k) T = sym_mul(T, T);
+  d4 T = sym_mul<NR>(X4, sym_mul<NR>(X4, B2, B1), B0);
+  for (int k = 0; k < s; ++k) T = sym_mul<NR>(T, T);
   return T;
 }
 
@@ -264,45 +269,64 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
     d4 v[kBeadsPerChunk];  // V of the chunk's beads, accumulator layout
 #pragma unroll
     for (int b = 0; b < kBeadsPerChunk; ++b) v[b] = d4{0.0, 0.0, 0.0, 0.0};
-    // the next mode's per-lane loads are issued while this mode's arithmetic runs
-    double x0 = qs[pc_], xn = qs[pn];
-    double2 c[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) c[r] = LQ[(g + 4 * r) * kW + j];
-    for (int d = 0; d < N; ++d) {
+    // everything mode d needs, loaded while mode d - 1's arithmetic runs
+    struct ModeIn {
+      double x0, xn;                  // this group's bead and its ring neighbour
+      double xb[kBeadsPerChunk];      // the chunk's beads (uniform)
+      double c0, c1, c2, c3;          // th, csch of the rho and vib sets (uniform)
+      double p0, p1, p2, p3;          // th, csch of the vib+ and vib- sets (uniform)
+      double lr, lv, lp, lm;          // this lane's surface: lin of rho, vib, vib+, vib-
+      double2 c[NR];                  // {L, Qh} at rows g + 4r, column j
+    };
+    auto load_mode = [&](int d) {
+      ModeIn m;
       const double* __restrict__ t = tab + d * rl;
       const double* qd = qs + d * P;
-      const int dn = (d + 1 < N) ? d + 1 : d;
-      const double x0n = qs[dn * P + pc_], xnn = qs[dn * P + pn];
-      double2 cn[NR];
+      m.x0 = qd[pc_];
+      m.xn = qd[pn];
 #pragma unroll
-      for (int r = 0; r < NR; ++r) cn[r] = LQ[((size_t)dn * kW + g + 4 * r) * kW + j];
-
-      const double sx = x0 + xn, dx = x0 - xn;
-      const double q2s = fma(x0, x0, xn * xn), d2 = dx * dx;
-      tr = fma(t[1], d2, fma(t[0], q2s, tr));
-      tv = fma(t[3], d2, fma(t[2], q2s, tv));
-      er = fma(t[R.lin_rho() + jrc], sx, er);
-      ev = fma(t[R.lin_vib() + jvc], sx, ev);
+      for (int b = 0; b < kBeadsPerChunk; ++b) m.xb[b] = qd[pb[b]];
+      m.c0 = t[0];
+      m.c1 = t[1];
+      m.c2 = t[2];
+      m.c3 = t[3];
+      m.lr = t[R.lin_rho() + jrc];
+      m.lv = t[R.lin_vib() + jvc];
       if (PM) {
-        tp = fma(t[R.pm() + 1], d2, fma(t[R.pm() + 0], q2s, tp));
-        tm = fma(t[R.pm() + 3], d2, fma(t[R.pm() + 2], q2s, tm));
-        ep = fma(t[R.lin_plus() + jvc], sx, ep);
-        em = fma(t[R.lin_minus() + jvc], sx, em);
+        m.p0 = t[R.pm() + 0];
+        m.p1 = t[R.pm() + 1];
+        m.p2 = t[R.pm() + 2];
+        m.p3 = t[R.pm() + 3];
+        m.lp = t[R.lin_plus() + jvc];
+        m.lm = t[R.lin_minus() + jvc];
+      }
+#pragma unroll
+      for (int r = 0; r < NR; ++r) m.c[r] = LQ[((size_t)d * kW + g + 4 * r) * kW + j];
+      return m;
+    };
+    ModeIn cur = load_mode(0);
+    for (int d = 0; d < N; ++d) {
+      const ModeIn nxt = load_mode(d + 1 < N ? d + 1 : d);
+      const double sx = cur.x0 + cur.xn, dx = cur.x0 - cur.xn;
+      const double q2s = fma(cur.x0, cur.x0, cur.xn * cur.xn), d2 = dx * dx;
+      tr = fma(cur.c1, d2, fma(cur.c0, q2s, tr));
+      tv = fma(cur.c3, d2, fma(cur.c2, q2s, tv));
+      er = fma(cur.lr, sx, er);
+      ev = fma(cur.lv, sx, ev);
+      if (PM) {
+        tp = fma(cur.p1, d2, fma(cur.p0, q2s, tp));
+        tm = fma(cur.p3, d2, fma(cur.p2, q2s, tm));
+        ep = fma(cur.lp, sx, ep);
+        em = fma(cur.lm, sx, em);
       }
       // V[i][j] += x_d (L + Qh x_d), i = g + 4r (pimc.py:1147-1160)
-      double xb[kBeadsPerChunk];
-#pragma unroll
-      for (int b = 0; b < kBeadsPerChunk; ++b) xb[b] = qd[pb[b]];
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
 #pragma unroll
-        for (int b = 0; b < kBeadsPerChunk; ++b) v[b][r] = fma(xb[b], fma(c[r].y, xb[b], c[r].x), v[b][r]);
+        for (int b = 0; b < kBeadsPerChunk; ++b)
+          v[b][r] = fma(cur.xb[b], fma(cur.c[r].y, cur.xb[b], cur.c[r].x), v[b][r]);
       }
-      x0 = x0n;
-      xn = xnn;
-#pragma unroll
-      for (int r = 0; r < NR; ++r) c[r] = cn[r];
+      cur = nxt;
     }
     for (int t = 0; t < args.npair; ++t) {  // off-diagonal quadratic mode pairs
       const int jj = pairs[2 * t], kk = pairs[2 * t + 1];
@@ -374,7 +398,7 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
         d4 X;
 #pragma unroll
         for (int r = 0; r < 4; ++r) X[r] = L.X[b][r][lane];
-        const d4 M = expm_sym(X, A, g, j);
+        const d4 M = expm_sym<NR>(X, A, g, j);
         if (DBG) {
           const int rec = eval_debug_record(A, AR);
           double* dr = args.dbg + ((int64_t)sample * P + p0 + b) * rec + AR + A + 1;
@@ -416,11 +440,12 @@ __device__ __forceinline__ void eval_wide_sample(const EvalArgs& args, int64_t s
             renorm_wave(F, fe);
           }
         } else {
-          const d4 Z = sym_mul(M, Rg);
+          // sum index >= 4 NR: M's padding columns are zero
+          const d4 Z = sym_mul<NR>(M, Rg);
 #pragma unroll
           for (int r = 0; r < 4; ++r) Rg[r] = Z[r] * L.O[0][b][g + 4 * r];
           if (PM) {
-            const d4 Zp = sym_mul(M, Rp), Zm = sym_mul(M, Rm);
+            const d4 Zp = sym_mul<NR>(M, Rp), Zm = sym_mul<NR>(M, Rm);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               Rp[r] = Zp[r] * L.O[1][b][g + 4 * r];
