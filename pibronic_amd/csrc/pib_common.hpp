@@ -59,6 +59,14 @@ __device__ __forceinline__ double shfl_xor_d(double v, int mask) {
   return __shfl_xor(v, mask, kWave);
 }
 
+// log2(x) to ~1e-7 (frexp and the hardware log2 of the mantissa); 0 gives -inf, NaN NaN. Used
+// to pick squaring counts, where only the integer part matters.
+__device__ __forceinline__ double log2_coarse(double x) {
+  int e = 0;
+  const double m = frexp(x, &e);
+  return (double)e + (double)__log2f((float)m);
+}
+
 // Allow `bytes` of dynamic LDS for kernel `fn` on the current device (needed above 64 KB).
 // The attribute is per device; it is set on the first launch on each device (a bit per device
 // in `done`, which the caller keeps per kernel; setting it twice is harmless).

@@ -7,9 +7,10 @@
 //
 // Mapping (gfx950, wave64): G lanes cooperate on one sample, 64/G samples per wave. Lane g
 // owns the contiguous bead chunk [g*K, min((g+1)*K, P)). Per bead it forms the diagonal
-// O-factors of every temperature set, the residual coupling V in registers, diagonalises V
-// with Jacobi sweeps (per lane, A x A in registers), builds M = exp(-tau V) and
-// multiplies M * diag(O_vib / S_p) into its running ring product. The G partial products are
+// O-factors of every temperature set, the residual coupling V in registers, M = exp(-tau V)
+// directly (a scaled and squared Taylor polynomial, per lane, A x A in registers; the
+// reference's eigendecomposition serves only to form M) and multiplies M * diag(O_vib / S_p)
+// into its running ring product. The G partial products are
 // then combined by a log2(G)-level ordered tree of A x A products over cross-lane moves
 // (chunk 0 * chunk 1 * ... keeps the reference's left-to-right order).
 //
@@ -22,145 +23,89 @@
 // constants, and the O-factors differ from the reference's by the reference's own rounding
 // (<= 1e-11 relative at A=12, N=24, P=128; ~1e-13 at the small workloads). V is built from
 // the unshifted coordinates (the reference's shift/unshift leaves an ulp of drift in q,
-// which moves V by ~1e-16). Linear algebra (V, Jacobi, M, ring) uses explicit fma().
+// which moves V by ~1e-16). Linear algebra (V, M, ring) uses explicit fma().
 #include "pib_common.hpp"
 #include "pib_kernels.hpp"
 
 namespace pib {
 
 // ---------------------------------------------------------------------------------------
-// Cyclic Jacobi eigen-decomposition of a symmetric A x A matrix held in registers.
-// Only the upper triangle a[i][j] (i <= j) is read and written. On exit the diagonal holds
-// the eigenvalues and the columns of u the eigenvectors. Rotation per Rutishauser's form
-// (Golub & Van Loan, Matrix Computations, 4th ed., Alg. 8.5.2 / 8.5.3).
+// M = exp(X) for a symmetric A x A matrix held in one lane's registers, X = -tau V.
+//
+// The reference forms M = U diag(exp(-tau lambda)) U^T from eigh(V) (pimc.py:1167-1187); M is
+// the only consumer of the decomposition, so it is computed directly: scaling and squaring of
+// the degree-8 Taylor polynomial T_8, evaluated by Horner's rule. X is scaled by 2^-s so that
+// |X|_F 2^-s <= theta_8 = 0.049, below the root of |y|^8 / 9! = u (the truncation term
+// relative to y, i.e. a backward error under the unit roundoff); the Frobenius norm bounds
+// the spectral radius, and for a symmetric X the 2-norm error of a polynomial in X is its
+// largest error over the eigenvalues. T_8 is then squared s times. Every intermediate is a
+// polynomial in X, hence symmetric, so products form the upper triangle only. s differs
+// between lanes; the squaring loop runs the wave's largest count. Degree 8 minimises
+// products plus squarings for Horner evaluation (one product per degree against one squaring
+// per halving of theta).
 // ---------------------------------------------------------------------------------------
+constexpr double kLog2Theta8 = -4.3511;  // log2(0.049)
+
+// Z = X Y + d I for commuting symmetric X, Y (Z symmetric; Z must not alias X or Y)
 template <int A>
-__device__ __forceinline__ double& UT(double (&a)[A][A], int i, int j) {
-  return i <= j ? a[i][j] : a[j][i];
-}
-
-// 1/x and 1/sqrt(x) from the hardware approximations (relative error ~2^-29) refined by
-// Newton steps, each of which squares the relative error: one step gives ~2^-50, two ~1 ulp.
-// Used only inside Jacobi rotations: the rotation's orthogonality depends on c^2 + s^2 = 1,
-// so c = 1/sqrt(1 + t^2) takes two steps; t (and the square root inside it) only has to be
-// close to the annihilating angle -- an error of 2^-50 leaves a_pq ~2^-50 |a_pq| behind --
-// and takes one.
-template <int STEPS>
-__device__ __forceinline__ double rcp_nr(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-#pragma unroll
-  for (int it = 0; it < STEPS; ++it) {
-    const double e = fma(-x, y, 1.0);
-    y = fma(y, e, y);
-  }
-  return y;
-}
-
-template <int STEPS>
-__device__ __forceinline__ double rsq_nr(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-#pragma unroll
-  for (int it = 0; it < STEPS; ++it) {
-    const double e = fma(-(x * y), y, 1.0);  // 1 - x y^2
-    y = fma(0.5 * y, e, y);
-  }
-  return y;
-}
-
-template <int A>
-__device__ __forceinline__ void jacobi_rotate(double (&a)[A][A], double (&u)[A][A], int p, int q) {
-  // t = tan(phi) of the rotation annihilating a_pq: the root of smaller magnitude of
-  // t^2 + 2 t (a_qq - a_pp)/(2 a_pq) - 1 = 0, i.e. t = sgn(d) h / (|d| + sqrt(d^2 + h^2)),
-  // d = a_qq - a_pp, h = 2 a_pq; |t| <= 1. Branch-free: a_pq == 0 gives t = 0, c = 1, s = 0,
-  // an exact no-op.
-  const double apq = a[p][q];
-  const double d = a[q][q] - a[p][p];
-  const double h = 2.0 * apq;
-  // safe arguments instead of branches: x == 0 gives r = 0 * rsq(1) = 0, and den == 0
-  // implies h == 0, so t = 0 * rcp(1) = 0
-  const double x = fma(d, d, h * h);
-  const double r = x * rsq_nr<1>(x > 0.0 ? x : 1.0);
-  const double den = fabs(d) + r;
-  double t = h * rcp_nr<1>(den > 0.0 ? den : 1.0);
-  t = d < 0.0 ? -t : t;
-  const double c = rsq_nr<2>(fma(t, t, 1.0));
-  const double s = t * c;
-  a[p][p] = fma(-t, apq, a[p][p]);
-  a[q][q] = fma(t, apq, a[q][q]);
-  a[p][q] = 0.0;
-#pragma unroll
-  for (int r_ = 0; r_ < A; ++r_) {
-    if (r_ == p || r_ == q) continue;
-    const double g = UT<A>(a, r_, p), hh = UT<A>(a, r_, q);
-    UT<A>(a, r_, p) = fma(c, g, -s * hh);
-    UT<A>(a, r_, q) = fma(s, g, c * hh);
-  }
-#pragma unroll
-  for (int r_ = 0; r_ < A; ++r_) {
-    const double g = u[r_][p], hh = u[r_][q];
-    u[r_][p] = fma(c, g, -s * hh);
-    u[r_][q] = fma(s, g, c * hh);
-  }
-}
-
-// Symmetric eigen-decomposition by Jacobi sweeps. On entry a holds V (upper
-// triangle); on exit its diagonal holds the eigenvalues and V = u diag(a) u^T. (A warm start
-// from the previous bead's eigenvectors saves about one sweep on average, but it keeps a
-// second A x A basis live across the mode loop; on gfx950 the lost occupancy costs more than
-// the sweep, so every bead starts from the identity.)
-template <int A>
-__device__ __forceinline__ void jacobi_eigh(double (&a)[A][A], double (&u)[A][A]) {
+__device__ __forceinline__ void sym_prod_lane(const double (&X)[A][A], const double (&Y)[A][A], double d,
+                                              double (&Z)[A][A]) {
 #pragma unroll
   for (int i = 0; i < A; ++i)
 #pragma unroll
-    for (int k = 0; k < A; ++k) u[i][k] = (i == k) ? 1.0 : 0.0;
-  if (A == 1) return;
-  double fro2 = 0.0;
+    for (int k = i; k < A; ++k) {
+      double s = (i == k) ? fma(X[i][0], Y[0][k], d) : X[i][0] * Y[0][k];
+#pragma unroll
+      for (int c = 1; c < A; ++c) s = fma(X[i][c], Y[c][k], s);
+      Z[i][k] = s;
+      Z[k][i] = s;
+    }
+}
+
+template <int A>
+__device__ __forceinline__ void expm_lane(double (&X)[A][A], double (&T)[A][A]) {
+  double nf = 0.0;
 #pragma unroll
   for (int i = 0; i < A; ++i) {
-    fro2 = fma(a[i][i], a[i][i], fro2);
+    nf = fma(X[i][i], X[i][i], nf);
 #pragma unroll
-    for (int j = i + 1; j < A; ++j) fro2 = fma(2.0 * a[i][j], a[i][j], fro2);
+    for (int k = i + 1; k < A; ++k) nf = fma(2.0 * X[i][k], X[i][k], nf);
   }
-  // stop at a relative off-diagonal norm of 1e-12: the residual perturbs M = exp(-tau V) by
-  // about tau*|V|*1e-12 per bead, two orders below the per-sample parity bound; quadratic
-  // convergence takes the typical matrix from 1e-4 to below 1e-12 in one sweep
-  const double tol2 = 1e-24 * fro2;
-  constexpr int kMaxSweeps = (A <= 4) ? 12 : 20;
-  // round-robin (tournament) order: each round holds disjoint pairs, whose c/s chains do
-  // not depend on one another, so the scheduler overlaps them (the cyclic row order makes
-  // every rotation wait for the previous one); on the workload matrices it also converges in
-  // about one sweep less. Convergence is tested before every round, uniformly over the wave:
-  // converged lanes keep rotating (harmlessly) until every lane of the wave has converged.
-  constexpr int n = (A % 2 == 0) ? A : A + 1;  // odd A: one idle index per round
-  bool conv = false;
-  for (int sweep = 0; sweep < kMaxSweeps && !conv; ++sweep) {
+  // smallest s >= 0 with |X|_F 2^-s <= theta_8; a zero matrix gives -inf, NaN gives s = 0
+  const double sd = 0.5 * log2_coarse(nf) - kLog2Theta8;
+  int s = 0;
+  if (sd > 0.0) s = sd < 64.0 ? (int)ceil(sd) : 64;
 #pragma unroll
-    for (int rnd = 0; rnd < n - 1; ++rnd) {
-      if (!conv) {
-        double off = 0.0;
+  for (int i = 0; i < A; ++i)
 #pragma unroll
-        for (int i = 0; i < A; ++i)
-#pragma unroll
-          for (int j = i + 1; j < A; ++j) off = fma(a[i][j], a[i][j], off);
-        conv = __all(!(off > tol2));
-      }
-      if (!conv) {
-#pragma unroll
-        for (int i = 0; i < n / 2; ++i) {
-          const int x = (i == 0) ? n - 1 : (rnd + i) % (n - 1);
-          const int y = (i == 0) ? rnd : (rnd - i + n - 1) % (n - 1);
-          if (x < A && y < A) jacobi_rotate<A>(a, u, x < y ? x : y, x < y ? y : x);
-        }
-      }
+    for (int k = i; k < A; ++k) {
+      X[i][k] = ldexp(X[i][k], -s);
+      X[k][i] = X[i][k];
     }
+  constexpr double c[9] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040, 1.0 / 40320};
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = 0; k < A; ++k) T[i][k] = (i == k) ? fma(c[8], X[i][k], c[7]) : c[8] * X[i][k];
+#pragma unroll
+  for (int n = 6; n >= 0; --n) {
+    double Z[A][A];
+    sym_prod_lane<A>(X, T, c[n], Z);
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int k = 0; k < A; ++k) T[i][k] = Z[i][k];
+  }
+  for (int k = 0; k < s; ++k) {
+    double Z[A][A];
+    sym_prod_lane<A>(T, T, 0.0, Z);
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int j = 0; j < A; ++j) T[i][j] = Z[i][j];
   }
 }
 
-// Sum over modes for one bead p (next bead pn): the surface-independent parts t* of the
-// O-matrix exponents, the per-surface linear parts e*, and the lower triangle of the residual
-// coupling V[b][c] = sum_j (q_j 0.5 Q[j,j,b,c] + L[j,b,c]) q_j (pimc.py:1147-1160). Mode j's
-// constants are the record tab + j * ModeRec(A, AR).len().
 template <int A, int AR, bool PM, bool QD>
 __device__ __forceinline__ void mode_terms(const double* __restrict__ tab, const double* __restrict__ qs, int N,
                                            int P, int p, int pn, double& tr, double& tv, double& tp,
@@ -205,13 +150,13 @@ __device__ __forceinline__ void mode_terms(const double* __restrict__ tab, const
   }
 }
 
-// Occupancy: for A <= 4 the VGPR budget is capped at 168 (three waves per SIMD). The kernel
-// is latency-bound (long dependent FP64 chains); at A = 4 with +-dbeta this is 16% faster
-// than the compiler's default of 256 VGPRs / two waves (measured on MI355X). Larger A keeps
-// the default: its live matrices do not fit 168 VGPRs without heavy spilling.
+// Occupancy: for A <= 4 the VGPR budget is capped at 256 (two waves per SIMD): the plain
+// kernel needs 128 (four waves), the +-dbeta kernel's three ring states need ~200, and a cap
+// of 168 made it spill (0.78 against 0.59 ms at A = 4, N = 8, P = 64, 1e5 samples, measured on
+// MI355X). Larger A keeps the default budget (512 with the accumulation registers).
 template <int A>
 struct EvalOccupancy {
-  static constexpr int waves = (A <= 4) ? 3 : 1;
+  static constexpr int waves = (A <= 4) ? 2 : 1;
 };
 
 // The samples of one wave: wave w evaluates samples [w * 64/G, (w + 1) * 64/G) of `args`.
@@ -308,27 +253,19 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
     }
     if (args.noscale) lS = 0.0;
 
-    // coupling matrix -> Jacobi -> M = U exp(-tau lambda) U^T (pimc.py:1171-1187)
-    double a_[A][A], u[A][A];
-#pragma unroll
-    for (int b = 0; b < A; ++b)
-#pragma unroll
-      for (int c = 0; c <= b; ++c) a_[c][b] = v[b][c] + Ev[b * A + c];
-    jacobi_eigh<A>(a_, u);
-    double ex[A];
-#pragma unroll
-    for (int d = 0; d < A; ++d) ex[d] = exp(-args.tau * a_[d][d]);
+    // coupling matrix -> M = exp(-tau V) (pimc.py:1167-1187)
     double M[A][A];
+    {
+      double X[A][A];
 #pragma unroll
-    for (int c = 0; c < A; ++c)
+      for (int b = 0; b < A; ++b)
 #pragma unroll
-      for (int e = 0; e <= c; ++e) {
-        double s = (u[c][0] * ex[0]) * u[e][0];
-#pragma unroll
-        for (int d = 1; d < A; ++d) s = fma(u[c][d] * ex[d], u[e][d], s);
-        M[c][e] = s;
-        M[e][c] = s;
-      }
+        for (int c = 0; c <= b; ++c) {
+          X[b][c] = -args.tau * (v[b][c] + Ev[b * A + c]);
+          X[c][b] = X[b][c];
+        }
+      expm_lane<A>(X, M);
+    }
 
     if (DBG) {
       const int rec = eval_debug_record(A, AR);

@@ -1,10 +1,9 @@
 // Reduced-precision variant of k_eval_fp64 (same mapping, same outputs, A <= 8).
 //
-// What runs in FP32: the O-matrix exponents, the coupling matrix V, the Jacobi sweeps, M and
-// the ring products. What stays FP64: the bead coordinates as read (rounded to float once per
-// mode), the rotation parameters of near-degenerate Jacobi pairs (|a_qq - a_pp| < 1e-6 |V|_F),
-// the per-sample log-scale sum_p log S_p and the per-surface log-products of rho, and the
-// final assembly of s_g, s_rho.
+// What runs in FP32: the O-matrix exponents, the coupling matrix V, M = exp(-tau V) and the
+// ring products. What stays FP64: the bead coordinates as read (rounded to float once per
+// mode), the per-sample log-scale sum_p log S_p and the per-surface log-products of rho, and
+// the final assembly of s_g, s_rho.
 //
 // Range: the O-factors are formed in the log domain, log O_a = log F_a - e_a / 2, so that
 // S_p = max_a O_a and O_a / S_p = exp(log O_a - log S_p) never under/overflow before scaling.
@@ -24,103 +23,72 @@ namespace pib {
 
 namespace {
 
-template <int A>
-__device__ __forceinline__ float& UTf(float (&a)[A][A], int i, int j) {
-  return i <= j ? a[i][j] : a[j][i];
-}
+// M = exp(X) for a symmetric A x A matrix in one lane's registers, X = -tau V, in FP32: the
+// FP64 kernel's scaled and squared Taylor polynomial (see expm_lane there) at degree 5, with
+// |X|_F 2^-s <= theta_5 = 0.13, below the root of |y|^5 / 6! = 2^-24 (degree 5 minimises
+// products plus squarings for FP32 Horner evaluation).
+constexpr float kLog2Theta5f = -2.9434f;  // log2(0.13)
 
-// tan of the Jacobi angle (see jacobi_rotate in the FP64 kernel): FP32 with the hardware
-// reciprocal and reciprocal square root (1 ulp), or FP64 when the diagonal gap is below
-// `gap` (near-degenerate pair).
-__device__ __forceinline__ float jacobi_t(float app, float aqq, float apq, float gap) {
-  const float d = aqq - app;
-  if (fabsf(d) < gap) {
-    const double dd = (double)aqq - (double)app, h = 2.0 * (double)apq;
-    const double r = sqrt(fma(dd, dd, h * h));
-    const double den = fabs(dd) + r;
-    double t = den > 0.0 ? h / den : 0.0;
-    return (float)(dd < 0.0 ? -t : t);
-  }
-  const float h = 2.0f * apq;
-  const float x = fmaf(d, d, h * h);
-  const float r = x > 0.0f ? x * __builtin_amdgcn_rsqf(x) : 0.0f;
-  const float den = fabsf(d) + r;
-  const float t = den > 0.0f ? h * __builtin_amdgcn_rcpf(den) : 0.0f;
-  return d < 0.0f ? -t : t;
-}
-
+// Z = X Y + d I for commuting symmetric X, Y (Z symmetric; Z must not alias X or Y)
 template <int A>
-__device__ __forceinline__ void jacobi_rotate_f(float (&a)[A][A], float (&u)[A][A], int p, int q,
-                                                float gap) {
-  const float apq = a[p][q];
-  const float t = jacobi_t(a[p][p], a[q][q], apq, gap);
-  const float c = __builtin_amdgcn_rsqf(fmaf(t, t, 1.0f));
-  const float s = t * c;
-  a[p][p] = fmaf(-t, apq, a[p][p]);
-  a[q][q] = fmaf(t, apq, a[q][q]);
-  a[p][q] = 0.0f;
-#pragma unroll
-  for (int r = 0; r < A; ++r) {
-    if (r == p || r == q) continue;
-    const float g = UTf<A>(a, r, p), h = UTf<A>(a, r, q);
-    UTf<A>(a, r, p) = fmaf(c, g, -s * h);
-    UTf<A>(a, r, q) = fmaf(s, g, c * h);
-  }
-#pragma unroll
-  for (int r = 0; r < A; ++r) {
-    const float g = u[r][p], h = u[r][q];
-    u[r][p] = fmaf(c, g, -s * h);
-    u[r][q] = fmaf(s, g, c * h);
-  }
-}
-
-// Jacobi sweeps in round-robin order with a convergence test before every round (as the
-// FP64 kernel's jacobi_eigh), to a relative off-diagonal norm of 1e-6.
-template <int A>
-__device__ __forceinline__ void jacobi_eigh_f(float (&a)[A][A], float (&u)[A][A]) {
+__device__ __forceinline__ void sym_prod_lane_f(const float (&X)[A][A], const float (&Y)[A][A], float d,
+                                                float (&Z)[A][A]) {
 #pragma unroll
   for (int i = 0; i < A; ++i)
 #pragma unroll
-    for (int k = 0; k < A; ++k) u[i][k] = (i == k) ? 1.0f : 0.0f;
-  if (A == 1) return;
-  float fro2 = 0.0f;
+    for (int k = i; k < A; ++k) {
+      float s = (i == k) ? fmaf(X[i][0], Y[0][k], d) : X[i][0] * Y[0][k];
+#pragma unroll
+      for (int c = 1; c < A; ++c) s = fmaf(X[i][c], Y[c][k], s);
+      Z[i][k] = s;
+      Z[k][i] = s;
+    }
+}
+
+template <int A>
+__device__ __forceinline__ void expm_lane_f(float (&X)[A][A], float (&T)[A][A]) {
+  float nf = 0.0f;
 #pragma unroll
   for (int i = 0; i < A; ++i) {
-    fro2 = fmaf(a[i][i], a[i][i], fro2);
+    nf = fmaf(X[i][i], X[i][i], nf);
 #pragma unroll
-    for (int j = i + 1; j < A; ++j) fro2 = fmaf(2.0f * a[i][j], a[i][j], fro2);
+    for (int k = i + 1; k < A; ++k) nf = fmaf(2.0f * X[i][k], X[i][k], nf);
   }
-  const float gap = 1e-6f * sqrtf(fro2);
-  const float tol2 = 1e-12f * fro2;
-  constexpr int kMaxSweeps = (A <= 4) ? 12 : 20;
-  constexpr int n = (A % 2 == 0) ? A : A + 1;
-  bool conv = false;
-  for (int sweep = 0; sweep < kMaxSweeps && !conv; ++sweep) {
+  // smallest s >= 0 with |X|_F 2^-s <= theta_5; a zero matrix gives -inf, NaN gives s = 0
+  const float sd = 0.5f * __log2f(nf) - kLog2Theta5f;
+  int s = 0;
+  if (sd > 0.0f) s = sd < 64.0f ? (int)ceilf(sd) : 64;
 #pragma unroll
-    for (int rnd = 0; rnd < n - 1; ++rnd) {
-      if (!conv) {
-        float off = 0.0f;
+  for (int i = 0; i < A; ++i)
 #pragma unroll
-        for (int i = 0; i < A; ++i)
-#pragma unroll
-          for (int j = i + 1; j < A; ++j) off = fmaf(a[i][j], a[i][j], off);
-        conv = __all(!(off > tol2));
-      }
-      if (!conv) {
-#pragma unroll
-        for (int i = 0; i < n / 2; ++i) {
-          const int x = (i == 0) ? n - 1 : (rnd + i) % (n - 1);
-          const int y = (i == 0) ? rnd : (rnd - i + n - 1) % (n - 1);
-          if (x < A && y < A) jacobi_rotate_f<A>(a, u, x < y ? x : y, x < y ? y : x, gap);
-        }
-      }
+    for (int k = i; k < A; ++k) {
+      X[i][k] = ldexpf(X[i][k], -s);
+      X[k][i] = X[i][k];
     }
+  constexpr float c[6] = {1.0f, 1.0f, 1.0f / 2, 1.0f / 6, 1.0f / 24, 1.0f / 120};
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = 0; k < A; ++k) T[i][k] = (i == k) ? fmaf(c[5], X[i][k], c[4]) : c[5] * X[i][k];
+#pragma unroll
+  for (int n = 3; n >= 0; --n) {
+    float Z[A][A];
+    sym_prod_lane_f<A>(X, T, c[n], Z);
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int k = 0; k < A; ++k) T[i][k] = Z[i][k];
+  }
+  for (int k = 0; k < s; ++k) {
+    float Z[A][A];
+    sym_prod_lane_f<A>(T, T, 0.0f, Z);
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int j = 0; j < A; ++j) T[i][j] = Z[i][j];
   }
 }
 
-// Sum over modes for one bead (the FP64 kernel's mode_terms in FP32, coordinates rounded to
-// float once per mode): surface-independent exponent parts t*, per-surface linear parts e*,
-// and the lower triangle of V.
 template <int A, int AR, bool PM, bool QD>
 __device__ __forceinline__ void mode_terms_f(const float* __restrict__ tab, const double* __restrict__ qs, int N,
                                              int P, int p, int pn, float& tr, float& tv, float& tp,
@@ -287,26 +255,18 @@ __global__ void __launch_bounds__(256) k_eval_fp32(const EvalArgs args) {
 #pragma unroll
     for (int a = 0; a < AR; ++a) lr[a] += (double)(lOr[a] - lS);
 
-    float a_[A][A], u[A][A];
-#pragma unroll
-    for (int b = 0; b < A; ++b)
-#pragma unroll
-      for (int c = 0; c <= b; ++c) a_[c][b] = v[b][c] + Ev[b * A + c];
-    jacobi_eigh_f<A>(a_, u);
-    float ex[A];
-#pragma unroll
-    for (int d = 0; d < A; ++d) ex[d] = expf(-tau * a_[d][d]);
     float M[A][A];
+    {
+      float X[A][A];
 #pragma unroll
-    for (int c = 0; c < A; ++c)
+      for (int b = 0; b < A; ++b)
 #pragma unroll
-      for (int e = 0; e <= c; ++e) {
-        float s = (u[c][0] * ex[0]) * u[e][0];
-#pragma unroll
-        for (int d = 1; d < A; ++d) s = fmaf(u[c][d] * ex[d], u[e][d], s);
-        M[c][e] = s;
-        M[e][c] = s;
-      }
+        for (int c = 0; c <= b; ++c) {
+          X[b][c] = -tau * (v[b][c] + Ev[b * A + c]);
+          X[c][b] = X[b][c];
+        }
+      expm_lane_f<A>(X, M);
+    }
     float Ov[A], Opl[A], Omi[A];
 #pragma unroll
     for (int a = 0; a < A; ++a) {

@@ -115,13 +115,6 @@ __device__ __forceinline__ d4 sym_mul(const d4& X, const d4& Y) {
 
 constexpr double kLog2Theta12 = -1.738798;  // log2(0.2996158)
 
-// log2(x) to ~1e-7 (frexp and the hardware log2 of the mantissa); 0 gives -inf, NaN NaN
-__device__ __forceinline__ double log2_coarse(double x) {
-  int e = 0;
-  const double m = frexp(x, &e);
-  return (double)e + (double)__log2f((float)m);
-}
-
 // exp(X) for a symmetric X in the FP64 accumulator layout, zero outside its leading n x n
 // block, n <= 4 NR (scaling and squaring of the degree-12 Taylor polynomial; see the file
 // header). The result is zero outside the leading block.
