@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix peaks coincide)
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 matrix cores
 HBM_PEAK_GBS = 8000.0
 
 
@@ -151,11 +152,14 @@ def learned_leg(data, args, log):
 
     e0 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
     e1 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    l0 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    l1 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
 
     def step(i, timed):
         if timed:
             e0[i - W].record(stream)
-        R, logq = fl.draw(B, args.seed + 2, (1 << 40) + i * B, stream)
+        R, logq = fl.draw(B, args.seed + 2, (1 << 40) + i * B, stream,
+                          layer_events=(l0[i - W], l1[i - W]) if timed else None)
         if timed:
             e1[i - W].record(stream)
         data.evaluate_device(R, B, False, out, B, stream=stream)
@@ -170,14 +174,27 @@ def learned_leg(data, args, log):
     torch.cuda.synchronize()
     el = time.perf_counter() - t1
     draw_ms = sum(a.elapsed_time(b) for a, b in zip(e0, e1)) / K
+    layers_ms = sum(a.elapsed_time(b) for a, b in zip(l0, l1)) / K
     H, hdn, D = fl.H, fl.hidden, fl.D
     mlp_flops = 2.0 * B * fl.layers * (H * hdn + hdn * hdn + hdn * D)
+    fused = fl.use_fused and fl._fused_ok(B)
+    # the coupling layers (the learned step's largest cost) against the bf16 matrix cores:
+    # algorithmic flops are the dense products' 2 m n k; BF16x3 issues three bf16 products
+    # for each, so the matrix cores execute 3x that
+    layer_tflops = mlp_flops / (layers_ms * 1e-3) / 1e12
+    layer_roofline = {"kernel": ("k_flow_layer (one launch per coupling layer)" if fused
+                                 else "k_gemm_bf16x3_pre + k_coupling per layer"),
+                      "bound": "mfma", "achieved": layer_tflops, "peak": BF16_PEAK_TFLOPS,
+                      "unit": "TFLOP/s", "frac": layer_tflops / BF16_PEAK_TFLOPS,
+                      "matrix_core_frac": 3 * layer_tflops / BF16_PEAK_TFLOPS,
+                      "layers_ms": layers_ms, "flops_per_draw": mlp_flops,
+                      "note": "HIP events around the coupling layers of each timed draw"}
     res = weight_summary(sums[W:].sum(dim=0).cpu().numpy(), el)
     res.update(train_seconds=train_s, train_steps=args.train_steps, train_batch=args.train_batch,
                samples_per_step=B, layers=args.flow_layers, hidden=args.flow_hidden,
                gemm=args.flow_gemm, flow_draw_ms=draw_ms,
                flow_mlp_tflops=mlp_flops / (draw_ms * 1e-3) / 1e12,
-               flow_mlp_flops_per_draw=mlp_flops,
+               flow_mlp_flops_per_draw=mlp_flops, roofline=layer_roofline,
                train_final_batch_ess=hist[-1]["ess_frac"] if hist else None)
     return res
 

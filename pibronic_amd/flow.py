@@ -349,16 +349,22 @@ class Flow:
                           beta1, beta2, eps, self.step_count, stream)
 
     # ------------------------------------------------------------------ proposal draws
-    def draw(self, B, seed, offset, stream=None):
+    def draw(self, B, seed, offset, stream=None, layer_events=None):
         """B proposal samples: R (B, N, P) double and log q(R) (B,). The base sample is drawn
-        directly in the flow's coordinates with its log rho (pib_flow_draw_base)."""
+        directly in the flow's coordinates with its log rho (pib_flow_draw_base).
+        layer_events: optional pair of torch.cuda.Event recorded on `stream` around the
+        coupling layers (to time them apart from the base draw and the unwhitening)."""
         import torch
         self._check_batch(B)
         zp = self._buf("zp", (B, self.D))
         logrho = torch.empty(B, dtype=torch.float64, device=self.dev)
         self.ctx.flow_draw_base(seed, offset, B, zp, logrho, stream)
         logdet = torch.zeros(B, dtype=torch.float64, device=self.dev)
+        if layer_events is not None:
+            layer_events[0].record(stream)
         self._layers_forward(zp, B, logdet, stream)
+        if layer_events is not None:
+            layer_events[1].record(stream)
         R = torch.empty((B, self.N, self.P), dtype=torch.float64, device=self.dev)
         work = self._buf("work", (2 * B * self.D,))
         self.ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, R, stream)

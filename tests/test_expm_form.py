@@ -1,0 +1,157 @@
+"""M = exp(-tau V) without an eigendecomposition: the schemes of the evaluation kernels,
+restated in NumPy and checked against the reference's route M = U diag(exp(-tau lambda)) U^T
+from eigh (pibronic/pimc/pimc.py:1167-1187).
+
+The kernels use a scaled and squared Taylor polynomial T_m: X = -tau V is scaled by 2^-s so
+that a bound alpha on its spectral radius satisfies alpha 2^-s <= theta_m, T_m is evaluated
+and squared s times. For a symmetric X every intermediate is a polynomial in X, so the
+2-norm error is the largest scalar error over the eigenvalues. Schemes:
+  * per-lane FP64 kernel (A <= 8): m = 8 by Horner's rule, alpha = |X|_F, theta_8 = 0.049;
+  * per-lane FP32 kernel: m = 5 by Horner's rule in float32, alpha = |X|_F, theta_5 = 0.13;
+  * one-wave FP64 kernel (A <= 16): m = 12 by Paterson-Stockmeyer in X^4,
+    alpha = max(|X^3|_F^(1/3), |X^4|_F^(1/4)), theta_12 = 0.2996.
+The bound checked is normwise: |M - M_eigh|_max <= 16 (2^s + m) max(1, |X|_2) u |M|_max,
+with u the format's unit roundoff: each squaring doubles the relative error of the power it
+squares, and eigh's own M carries errors of order |X|_2 u. FP32 matrices are kept below
+exp(88) (float range), as the FP32 kernel's M must be.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pimc_oracle as po
+from pibronic_amd import synthetic
+from pibronic_amd.vibronic_model_keys import VibronicModelKeys as VMK
+
+U64 = 2.0 ** -53
+U32 = 2.0 ** -24
+
+
+def _squarings(alpha, theta):
+    with np.errstate(divide="ignore"):
+        sd = np.log2(alpha) - math.log2(theta)
+    return np.where(sd > 0, np.ceil(np.nan_to_num(sd, neginf=0.0)), 0).astype(int)
+
+
+def expm_horner(X, m, theta, dtype):
+    """Per-lane scheme: Frobenius bound, Horner evaluation of T_m, s squarings (per matrix).
+    Returns M and s."""
+    X = np.asarray(X, dtype=dtype)
+    n = X.shape[-1]
+    s = _squarings(np.linalg.norm(X, axis=(-2, -1)), theta)
+    Y = X * np.ldexp(np.ones_like(s, dtype=dtype), -s)[..., None, None]
+    I = np.eye(n, dtype=dtype)
+    c = [dtype(1.0 / math.factorial(k)) for k in range(m + 1)]
+    T = c[m] * Y + c[m - 1] * I
+    for k in range(m - 2, -1, -1):
+        T = Y @ T + c[k] * I
+    for k in range(int(s.max()) if s.size else 0):
+        T = np.where((s > k)[..., None, None], T @ T, T)
+    return T, s
+
+
+def expm_ps12(X):
+    """One-wave scheme: alpha from |X^3|, |X^4|, T_12 = B0 + X^4 (B1 + X^4 B2)."""
+    X = np.asarray(X, dtype=np.float64)
+    n = X.shape[-1]
+    X2 = X @ X
+    X3 = X2 @ X
+    X4 = X2 @ X2
+    alpha = np.maximum(np.linalg.norm(X3, axis=(-2, -1)) ** (1 / 3),
+                       np.linalg.norm(X4, axis=(-2, -1)) ** 0.25)
+    s = _squarings(alpha, 0.2996158)
+    f = np.ldexp(1.0, -s)[..., None, None]
+    x1, x2, x3, x4 = X * f, X2 * f ** 2, X3 * f ** 3, X4 * f ** 4
+    I = np.eye(n)
+    c = [1.0 / math.factorial(k) for k in range(13)]
+    B2 = c[8] * I + c[9] * x1 + c[10] * x2 + c[11] * x3 + c[12] * x4
+    B1 = c[4] * I + c[5] * x1 + c[6] * x2 + c[7] * x3
+    B0 = I + x1 + c[2] * x2 + c[3] * x3
+    T = x4 @ (x4 @ B2 + B1) + B0
+    for k in range(int(s.max()) if s.size else 0):
+        T = np.where((s > k)[..., None, None], T @ T, T)
+    return T, s
+
+
+def expm_eigh(X):
+    lam, U = np.linalg.eigh(np.asarray(X, dtype=np.float64))
+    return np.einsum("...ik,...k,...jk->...ij", U, np.exp(lam), U)
+
+
+SCHEMES = {  # (function, unit roundoff, degree)
+    "lanes_fp64": (lambda X: expm_horner(X, 8, 0.049, np.float64), U64, 8),
+    "lanes_fp32": (lambda X: expm_horner(X, 5, 0.13, np.float32), U32, 5),
+    "wide_fp64": (expm_ps12, U64, 12),
+}
+
+
+def _check(X, scheme):
+    f, u, m = SCHEMES[scheme]
+    M, s = f(X)
+    M = M.astype(np.float64)
+    R = expm_eigh(X)
+    err = np.abs(M - R).max(axis=(-2, -1))
+    nrm = np.abs(R).max(axis=(-2, -1))
+    rho = np.abs(np.linalg.eigvalsh(np.asarray(X, dtype=np.float64))).max(axis=-1)
+    bound = 16.0 * (2.0 ** s + m) * np.maximum(1.0, rho) * u * nrm
+    worst = float(np.max(err / bound))
+    assert worst <= 1.0, (scheme, worst, float(np.max(err / nrm)))
+    return float(np.max(err / nrm)), worst
+
+
+def _workload_X(A, N, P, T, off, B=24):
+    kw = {} if off is None else {"off": off}
+    m = synthetic.generate(A, N, quadratic=True, seed=0, **kw)
+    E, w, L, Q = m[VMK.E], m[VMK.w], m[VMK.G1], m[VMK.G2]
+    Er = np.diag(E).copy()
+    Lr = np.stack([np.diag(L[j]) for j in range(N)])
+    s = po.Setup((E, w, L, Q), (Er, w, Lr), T, P)
+    q, _ = s.draw_q(np.random.default_rng(3), B)
+    o = po.evaluate(s, q, keep=True)
+    return -s.tau * o["V"].reshape(-1, A, A)
+
+
+@pytest.mark.parametrize("shape,schemes", [
+    ((4, 8, 64, 300.0, None), ["lanes_fp64", "lanes_fp32", "wide_fp64"]),
+    ((8, 16, 64, 300.0, None), ["lanes_fp64", "lanes_fp32", "wide_fp64"]),
+    ((12, 24, 128, 250.0, 0.1), ["wide_fp64"]),
+])
+def test_workload_matrices(shape, schemes):
+    X = _workload_X(*shape)
+    for sc in schemes:
+        e, worst = _check(X, sc)
+        print(shape[0], sc, "max relative M error", e, "of bound", worst)
+
+
+@pytest.mark.parametrize("scheme", list(SCHEMES))
+def test_norm_range_and_spectra(scheme):
+    """Random symmetric matrices with spectral radius 1e-3..1e2 (up to ~11 squarings),
+    clustered and near-degenerate spectra, both signs."""
+    rng = np.random.default_rng(11)
+    mats = []
+    scales = (1e-3, 0.1, 1.0, 5.0, 30.0) + (() if scheme == "lanes_fp32" else (100.0,))
+    for n in (2, 4, 8):
+        for scale in scales:
+            Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+            for lam in (rng.uniform(-1, 1, n), np.r_[1.0, 1.0 + 1e-9, rng.uniform(-1, 1, n - 2)],
+                        -np.abs(rng.uniform(0.5, 1, n))):
+                mats.append(((Q * (scale * lam)) @ Q.T, n))
+    for X, n in mats:
+        X = 0.5 * (X + X.T)
+        _check(X[None], scheme)
+
+
+@pytest.mark.parametrize("scheme", list(SCHEMES))
+def test_special_cases(scheme):
+    f, u, m = SCHEMES[scheme]
+    # V = 0 (uncoupled model): M = I exactly
+    for n in (1, 3, 8):
+        M, s = f(np.zeros((1, n, n)))
+        assert s[0] == 0 and np.array_equal(M.astype(np.float64)[0], np.eye(n))
+    # diagonal X: exp of each entry, exact zeros off the diagonal
+    d = np.array([-3.0, -0.7, 0.0, 0.2, 1.5])
+    M, s = f(np.diag(d)[None])
+    M = M.astype(np.float64)[0]
+    assert np.allclose(np.diag(M), np.exp(d), rtol=16 * (2.0 ** s[0] + m) * u, atol=0.0)
+    assert np.all(M[~np.eye(5, dtype=bool)] == 0.0)
