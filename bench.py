@@ -143,6 +143,9 @@ def learned_leg(data, args, log):
                        log_every=max(1, args.train_steps // 5))
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
+    # the timed draws: single-bf16 layers only where the one-launch layer runs
+    draw_prec = args.flow_draw if (args.flow_gemm == "bf16x3" and fl._fused_ok(args.flow_block)) else "bf16x3"
+    fl.draw_precision = draw_prec
     B = args.flow_block
     K, W = args.steps, args.warmup
     out = torch.empty((_native.NUM_OUT, B), dtype=torch.float64, device="cuda")
@@ -180,19 +183,20 @@ def learned_leg(data, args, log):
     fused = fl.use_fused and fl._fused_ok(B)
     # the coupling layers (the learned step's largest cost) against the bf16 matrix cores:
     # algorithmic flops are the dense products' 2 m n k; BF16x3 issues three bf16 products
-    # for each, so the matrix cores execute 3x that
+    # for each (single-bf16 draws one), so the matrix cores execute that multiple
     layer_tflops = mlp_flops / (layers_ms * 1e-3) / 1e12
+    products = 1 if draw_prec == "bf16" else 3
     layer_roofline = {"kernel": ("k_flow_layer (one launch per coupling layer)" if fused
                                  else "k_gemm_bf16x3_pre + k_coupling per layer"),
                       "bound": "mfma", "achieved": layer_tflops, "peak": BF16_PEAK_TFLOPS,
                       "unit": "TFLOP/s", "frac": layer_tflops / BF16_PEAK_TFLOPS,
-                      "matrix_core_frac": 3 * layer_tflops / BF16_PEAK_TFLOPS,
+                      "matrix_core_frac": products * layer_tflops / BF16_PEAK_TFLOPS,
                       "layers_ms": layers_ms, "flops_per_draw": mlp_flops,
                       "note": "HIP events around the coupling layers of each timed draw"}
     res = weight_summary(sums[W:].sum(dim=0).cpu().numpy(), el)
     res.update(train_seconds=train_s, train_steps=args.train_steps, train_batch=args.train_batch,
                samples_per_step=B, layers=args.flow_layers, hidden=args.flow_hidden,
-               gemm=args.flow_gemm, flow_draw_ms=draw_ms,
+               gemm=args.flow_gemm, draw_precision=draw_prec, flow_draw_ms=draw_ms,
                flow_mlp_tflops=mlp_flops / (draw_ms * 1e-3) / 1e12,
                flow_mlp_flops_per_draw=mlp_flops, roofline=layer_roofline,
                train_final_batch_ess=hist[-1]["ess_frac"] if hist else None)
@@ -329,6 +333,9 @@ def main():
                     help="precision of the flow's dense layers (FP32 MFMA or BF16x3 split)")
     ap.add_argument("--flow-fused", type=int, default=1,
                     help="0: layer-by-layer products in the draws instead of one launch per coupling layer")
+    ap.add_argument("--flow-draw", default="bf16", choices=["bf16x3", "bf16"],
+                    help="precision of the timed draws' one-launch layers (bf16: single bf16 products; "
+                         "log q stays exact for the map applied); training is BF16x3 either way")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this many MiB 3 times "
@@ -449,7 +456,7 @@ def main():
         # the timed step of `value`: a learned draw + FP64 g + sums (or the harmonic step)
         "ms_per_step": (learned["seconds"] if learned else elapsed) / K * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if args.fp32 else ("bf16x3/f32 ring, f64 eig" if args.ring_bf16x3 else "f64"),
+        "dtype": "f32" if args.fp32 else ("f64 M, bf16x3/f32 ring" if args.ring_bf16x3 else "f64"),
         "data": "synthetic (seeded model, device Philox samples)",
         "config": {"workload": f"{name}: A={A} N={N} P={P} T={T:g}K block={B}"
                                + (" +-dbeta" if pm else ""),
@@ -457,7 +464,10 @@ def main():
                    "harmonic_block": B, "lanes_per_sample": lanes,
                    "value_from": ("learned proposal (ms_per_step: one draw of samples_per_step, FP64 g and "
                                   "sums); samples_beads_per_s, samples_per_s: the harmonic FP64 step")
-                                 if learned else "harmonic rho"},
+                                 if learned else "harmonic rho",
+                   "flow_draw": (f"{learned['draw_precision']} coupling-layer products in the timed draws "
+                                 "(trained with BF16x3); g in FP64; log q exact for the map applied")
+                                if learned else None},
         "samples_beads_per_s": n * P / elapsed,
         "samples_per_s": harm["samples_per_s"],
         "eval_kernel_ms": eval_ms,

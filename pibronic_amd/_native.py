@@ -252,7 +252,7 @@ class Context:
     def flow_layer_fused(self, inverse, B, D, hidden, a_off, b_off, smax, z, logdet, w, b1, b2, b3,
                          stream=None):
         """w = ((W1h, W1l), (W2h, W2l), (W3h, W3l)) pre-split transposed weights, W2 and W3
-        split with perm32=True."""
+        split with perm32=True. All three lo parts None: single bf16 products of the hi parts."""
         (w1h, w1l), (w2h, w2l), (w3h, w3l) = w
         self._check(self.lib.pib_flow_layer_fused(
             self.h, int(bool(inverse)), int(B), int(D), int(hidden), int(a_off), int(b_off), float(smax),

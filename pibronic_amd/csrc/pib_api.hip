@@ -622,8 +622,10 @@ int pib_flow_layer_fused(pib_ctx* c, int inverse, int64_t B, int D, int hidden, 
                          const uint16_t* W3_hi, const uint16_t* W3_lo, const float* b1, const float* b2,
                          const float* b3, void* stream) {
   if (!c) return PIB_ERR_ARG;
-  if (!z || !logdet || !W1_hi || !W1_lo || !W2_hi || !W2_lo || !W3_hi || !W3_lo || !b1 || !b2 || !b3)
+  if (!z || !logdet || !W1_hi || !W2_hi || !W3_hi || !b1 || !b2 || !b3)
     return fail(c, PIB_ERR_ARG, "flow_layer_fused: null pointer");
+  if ((W1_lo == nullptr) != (W2_lo == nullptr) || (W1_lo == nullptr) != (W3_lo == nullptr))
+    return fail(c, PIB_ERR_ARG, "flow_layer_fused: the lo parts must be all given or all null");
   if (!fused_layer_supported(B, D, hidden))
     return fail(c, PIB_ERR_UNSUPPORTED, "flow_layer_fused: needs hidden 256, D = 512 and B a multiple of 128");
   const int H = D / 2;

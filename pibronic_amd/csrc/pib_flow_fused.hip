@@ -71,27 +71,28 @@ __device__ __forceinline__ f32x4f mfma3(const bf16x8f& ah, const bf16x8f& al, co
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
 }
 
-// The stage sequence of one layer; a stage is two 16-row weight tiles (hi and lo), staged in
-// LDS together behind one barrier: the HID/32 row pairs of W1^T (K = H), the HID/32 row pairs
-// of W2^T (K = HID), then for q = 0..H/16-1 the rows 16q.. (scales) with H + 16q.. (shifts)
-// of W3^T (K = HID).
-template <int H, int HID>
+// The stage sequence of one layer; a stage is two 16-row weight tiles (hi and, with LO, lo),
+// staged in LDS together behind one barrier: the HID/32 row pairs of W1^T (K = H), the HID/32
+// row pairs of W2^T (K = HID), then for q = 0..H/16-1 the rows 16q.. (scales) with H + 16q..
+// (shifts) of W3^T (K = HID). Without LO (single bf16 products) only the hi parts are staged.
+template <int H, int HID, bool LO>
 struct Stages {
   static constexpr int n1 = HID / 32, n2 = HID / 32, n3 = H / 16;
   static constexpr int total = n1 + n2 + n3;
   static constexpr int kmax = H > HID ? H : HID;
+  static constexpr int parts = LO ? 2 : 1;     // hi (and lo) per tile
   static constexpr int ld = kmax + kPadF;      // LDS row stride (elements)
   static constexpr int tile = 16 * ld;         // elements of one hi (or lo) tile
-  static constexpr int stage = 4 * tile;       // [tile 0 hi, tile 0 lo, tile 1 hi, tile 1 lo]
-  static constexpr int units = 2 * 2 * 16 * kmax / 8 / kThreads;  // 16-byte loads per thread
+  static constexpr int stage = 2 * parts * tile;  // [tile 0 hi, (tile 0 lo), tile 1 hi, (tile 1 lo)]
+  static constexpr int units = parts * 2 * 16 * kmax / 8 / kThreads;  // 16-byte loads per thread
   __device__ static constexpr int K(int s) { return s < n1 ? H : HID; }
 };
 
 // source of stage s: weight arrays and the first rows of its two tiles
-template <int H, int HID>
+template <int H, int HID, bool LO>
 __device__ __forceinline__ void stage_source(const FusedLayerArgs& a, int s, const unsigned short*& wh,
                                              const unsigned short*& wl, int& rowA, int& rowB) {
-  using S = Stages<H, HID>;
+  using S = Stages<H, HID, LO>;
   if (s < S::n1) {
     wh = a.W1h;
     wl = a.W1l;
@@ -113,33 +114,37 @@ __device__ __forceinline__ void stage_source(const FusedLayerArgs& a, int s, con
 
 // Unit i of this thread is 16-byte unit tid + kThreads * i of the stage, ordered
 // (tile, hi/lo, row, 8-element column group); units beyond the stage's K are skipped.
-template <int H, int HID>
-__device__ __forceinline__ void fetch(const FusedLayerArgs& a, int s, int tid, uvec4f (&pre)[Stages<H, HID>::units]) {
-  using S = Stages<H, HID>;
+template <int H, int HID, bool LO>
+__device__ __forceinline__ void fetch(const FusedLayerArgs& a, int s, int tid,
+                                      uvec4f (&pre)[Stages<H, HID, LO>::units]) {
+  using S = Stages<H, HID, LO>;
   const unsigned short *wh, *wl;
   int rowA, rowB;
-  stage_source<H, HID>(a, s, wh, wl, rowA, rowB);
+  stage_source<H, HID, LO>(a, s, wh, wl, rowA, rowB);
   const int K = S::K(s), per = 16 * K / 8;  // units per (tile, hi/lo)
 #pragma unroll
   for (int i = 0; i < S::units; ++i) {
     const int u = tid + kThreads * i;
-    if (u < 4 * per) {
-      const int blk = u / per, r = u - blk * per;  // blk = 2 * tile + (lo ? 1 : 0)
+    if (u < 2 * S::parts * per) {
+      const int blk = u / per, r = u - blk * per;  // blk = parts * tile + (lo ? 1 : 0)
       const int row = r / (K / 8), c8 = r - row * (K / 8);
-      const unsigned short* src = ((blk & 1) ? wl : wh) + (int64_t)(((blk >> 1) ? rowB : rowA) + row) * K + 8 * c8;
+      const bool lo = LO && (blk & 1);
+      const int t = LO ? (blk >> 1) : blk;
+      const unsigned short* src = (lo ? wl : wh) + (int64_t)((t ? rowB : rowA) + row) * K + 8 * c8;
       pre[i] = *(const uvec4f*)src;
     }
   }
 }
 
-template <int H, int HID>
-__device__ __forceinline__ void store(int s, int tid, const uvec4f (&pre)[Stages<H, HID>::units], unsigned short* buf) {
-  using S = Stages<H, HID>;
+template <int H, int HID, bool LO>
+__device__ __forceinline__ void store(int s, int tid, const uvec4f (&pre)[Stages<H, HID, LO>::units],
+                                      unsigned short* buf) {
+  using S = Stages<H, HID, LO>;
   const int K = S::K(s), per = 16 * K / 8;
 #pragma unroll
   for (int i = 0; i < S::units; ++i) {
     const int u = tid + kThreads * i;
-    if (u < 4 * per) {
+    if (u < 2 * S::parts * per) {
       const int blk = u / per, r = u - blk * per;
       const int row = r / (K / 8), c8 = r - row * (K / 8);
       *(uvec4f*)(buf + blk * S::tile + row * S::ld + 8 * c8) = pre[i];
@@ -149,7 +154,8 @@ __device__ __forceinline__ void store(int s, int tid, const uvec4f (&pre)[Stages
 
 // W^T tile (LDS, 16 rows x K, lo part tile_elems after hi) times X^T for each of the wave's
 // kNT sample tiles (operands in registers): one LDS read of a weight block feeds 3 kNT MFMAs
-template <int NB, int LD>
+// (LO: BF16x3) or kNT (single bf16 products of the hi parts)
+template <int NB, int LD, bool LO>
 __device__ __forceinline__ void tile_product(const unsigned short* buf, const Op8 (&bh)[kNT][NB],
                                              const Op8 (&bl)[kNT][NB], int li, int g, int tile_elems,
                                              f32x4f (&acc)[kNT]) {
@@ -159,15 +165,31 @@ __device__ __forceinline__ void tile_product(const unsigned short* buf, const Op
   for (int c = 0; c < NB; ++c) {
     const unsigned short* p = buf + li * LD + 32 * c + 8 * g;
     const bf16x8f ah = __builtin_bit_cast(bf16x8f, *(const uvec4f*)p);
-    const bf16x8f al = __builtin_bit_cast(bf16x8f, *(const uvec4f*)(p + tile_elems));
+    if constexpr (LO) {
+      const bf16x8f al = __builtin_bit_cast(bf16x8f, *(const uvec4f*)(p + tile_elems));
 #pragma unroll
-    for (int j = 0; j < kNT; ++j) acc[j] = mfma3(ah, al, as_bf(bh[j][c]), as_bf(bl[j][c]), acc[j]);
+      for (int j = 0; j < kNT; ++j) acc[j] = mfma3(ah, al, as_bf(bh[j][c]), as_bf(bl[j][c]), acc[j]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < kNT; ++j)
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf(bh[j][c]), acc[j], 0, 0, 0);
+    }
   }
 }
 
-template <int H, int HID>
+// operand element e of a block: bf16 hi (and with LO the lo remainder)
+template <bool LO>
+__device__ __forceinline__ void split_op(float v, int e, Op8& h, Op8& l) {
+  if constexpr (LO) {
+    split4(v, e, h, l);
+  } else {
+    put(h, e, f2bf_f(v));
+  }
+}
+
+template <int H, int HID, bool LO>
 __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
-  using S = Stages<H, HID>;
+  using S = Stages<H, HID, LO>;
   constexpr int NBX = H / 32, NBH = HID / 32;
   extern __shared__ __align__(16) unsigned short lds[];  // [2 buffers][stage]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -189,7 +211,7 @@ __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
       const float4 v0 = src[0], v1 = src[1];
       const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
-      for (int e = 0; e < 8; ++e) split4(v[e], e, xh[j][c], xl[j][c]);
+      for (int e = 0; e < 8; ++e) split_op<LO>(v[e], e, xh[j][c], xl[j][c]);
     }
 
   Op8 h1h[kNT][NBH], h1l[kNT][NBH], h2h[kNT][NBH], h2l[kNT][NBH];
@@ -198,32 +220,32 @@ __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
   for (int j = 0; j < kNT; ++j) part[j] = 0.0;
 
   uvec4f pre[S::units];
-  fetch<H, HID>(a, 0, tid, pre);
-  store<H, HID>(0, tid, pre, lds);
+  fetch<H, HID, LO>(a, 0, tid, pre);
+  store<H, HID, LO>(0, tid, pre, lds);
   // pipeline step around stage s: prefetch s+1 into registers, wait for stage s in LDS, then
   // (after the caller's products) write s+1 into the other buffer
 #define PIB_STAGE_BEGIN(s)                                        \
-  if ((s) + 1 < S::total) fetch<H, HID>(a, (s) + 1, tid, pre);    \
+  if ((s) + 1 < S::total) fetch<H, HID, LO>(a, (s) + 1, tid, pre); \
   __syncthreads();                                                \
   const unsigned short* buf = lds + ((s) & 1) * S::stage;
 #define PIB_STAGE_END(s) \
-  if ((s) + 1 < S::total) store<H, HID>((s) + 1, tid, pre, lds + (((s) + 1) & 1) * S::stage);
+  if ((s) + 1 < S::total) store<H, HID, LO>((s) + 1, tid, pre, lds + (((s) + 1) & 1) * S::stage);
 
   // first product: stage c gives output tiles 2c, 2c+1 = operand block c of the second
 #pragma clang loop unroll(full)
   for (int c = 0; c < S::n1; ++c) {
     PIB_STAGE_BEGIN(c)
     f32x4f acc0[kNT], acc1[kNT];
-    tile_product<NBX, S::ld>(buf, xh, xl, li, g, S::tile, acc0);
-    tile_product<NBX, S::ld>(buf + 2 * S::tile, xh, xl, li, g, S::tile, acc1);
+    tile_product<NBX, S::ld, LO>(buf, xh, xl, li, g, S::tile, acc0);
+    tile_product<NBX, S::ld, LO>(buf + S::parts * S::tile, xh, xl, li, g, S::tile, acc1);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const float b0 = a.b1[32 * c + 4 * g + r], b1 = a.b1[32 * c + 16 + 4 * g + r];
 #pragma unroll
       for (int j = 0; j < kNT; ++j) {
         const float v0 = acc0[j][r] + b0, v1 = acc1[j][r] + b1;
-        split4(v0 > 0.f ? v0 : 0.f, r, h1h[j][c], h1l[j][c]);
-        split4(v1 > 0.f ? v1 : 0.f, 4 + r, h1h[j][c], h1l[j][c]);
+        split_op<LO>(v0 > 0.f ? v0 : 0.f, r, h1h[j][c], h1l[j][c]);
+        split_op<LO>(v1 > 0.f ? v1 : 0.f, 4 + r, h1h[j][c], h1l[j][c]);
       }
     }
     PIB_STAGE_END(c)
@@ -233,16 +255,16 @@ __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
     const int s = S::n1 + c;
     PIB_STAGE_BEGIN(s)
     f32x4f acc0[kNT], acc1[kNT];
-    tile_product<NBH, S::ld>(buf, h1h, h1l, li, g, S::tile, acc0);
-    tile_product<NBH, S::ld>(buf + 2 * S::tile, h1h, h1l, li, g, S::tile, acc1);
+    tile_product<NBH, S::ld, LO>(buf, h1h, h1l, li, g, S::tile, acc0);
+    tile_product<NBH, S::ld, LO>(buf + S::parts * S::tile, h1h, h1l, li, g, S::tile, acc1);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const float b0 = a.b2[32 * c + 4 * g + r], b1 = a.b2[32 * c + 16 + 4 * g + r];
 #pragma unroll
       for (int j = 0; j < kNT; ++j) {
         const float v0 = acc0[j][r] + b0, v1 = acc1[j][r] + b1;
-        split4(v0 > 0.f ? v0 : 0.f, r, h2h[j][c], h2l[j][c]);
-        split4(v1 > 0.f ? v1 : 0.f, 4 + r, h2h[j][c], h2l[j][c]);
+        split_op<LO>(v0 > 0.f ? v0 : 0.f, r, h2h[j][c], h2l[j][c]);
+        split_op<LO>(v1 > 0.f ? v1 : 0.f, 4 + r, h2h[j][c], h2l[j][c]);
       }
     }
     PIB_STAGE_END(s)
@@ -266,8 +288,8 @@ __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
     }
     PIB_STAGE_BEGIN(s)
     f32x4f sraw[kNT], acc[kNT];
-    tile_product<NBH, S::ld>(buf, h2h, h2l, li, g, S::tile, sraw);
-    tile_product<NBH, S::ld>(buf + 2 * S::tile, h2h, h2l, li, g, S::tile, acc);
+    tile_product<NBH, S::ld, LO>(buf, h2h, h2l, li, g, S::tile, sraw);
+    tile_product<NBH, S::ld, LO>(buf + S::parts * S::tile, h2h, h2l, li, g, S::tile, acc);
     const int f0 = 16 * q + 4 * g;  // this lane's four features of the transformed half
 #pragma unroll
     for (int j = 0; j < kNT; ++j) {
@@ -295,13 +317,13 @@ __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
   }
 }
 
-template <int H, int HID>
+template <int H, int HID, bool LO>
 hipError_t launch_hh(const FusedLayerArgs& a, hipStream_t s) {
-  using S = Stages<H, HID>;
+  using S = Stages<H, HID, LO>;
   const size_t lds = (size_t)2 * S::stage * sizeof(unsigned short);
-  static unsigned long long lds_set = 0;  // 67.6 KB of dynamic LDS: above the default 64 KB
-  if (hipError_t e = ensure_dynamic_lds((const void*)k_flow_layer<H, HID>, lds, lds_set)) return e;
-  hipLaunchKernelGGL((k_flow_layer<H, HID>), dim3((unsigned)(a.B / kRows)), dim3(kThreads), lds, s, a);
+  static unsigned long long lds_set = 0;  // 67.6 KB of dynamic LDS with LO: above the default 64 KB
+  if (hipError_t e = ensure_dynamic_lds((const void*)k_flow_layer<H, HID, LO>, lds, lds_set)) return e;
+  hipLaunchKernelGGL((k_flow_layer<H, HID, LO>), dim3((unsigned)(a.B / kRows)), dim3(kThreads), lds, s, a);
   return hipGetLastError();
 }
 
@@ -318,7 +340,9 @@ hipError_t launch_flow_layer(const FusedLayerArgs& a, hipStream_t s) {
   if (!fused_layer_supported(a.B, a.D, a.hidden)) return hipErrorInvalidValue;
   if ((a.a_off != 0 && a.a_off != a.D / 2) || (a.b_off != 0 && a.b_off != a.D / 2) || a.a_off == a.b_off)
     return hipErrorInvalidValue;
-  return launch_hh<256, 256>(a, s);
+  const bool lo = a.W1l != nullptr;
+  if (lo != (a.W2l != nullptr) || lo != (a.W3l != nullptr)) return hipErrorInvalidValue;
+  return lo ? launch_hh<256, 256, true>(a, s) : launch_hh<256, 256, false>(a, s);
 }
 
 }  // namespace pib

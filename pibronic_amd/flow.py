@@ -80,6 +80,9 @@ class Flow:
         self._bufs = {}
         self._version = 0          # bumped whenever the parameters change
         self.use_fused = True      # bf16x3 draws and densities: one launch per coupling layer where supported
+        # draws only: "bf16x3" (the trained function to FP32 accuracy) or "bf16" (single bf16
+        # products in the one-launch layers; log q stays exact for the map applied, see draw())
+        self.draw_precision = "bf16x3"
         self._split_version = -1   # parameter version of the pre-split bf16 weights
         self._fsplit_version = -1  # parameter version of the fused layer's permuted weights
 
@@ -217,13 +220,16 @@ class Flow:
     def _fused_ok(self, B):
         return self.use_fused and self.hidden == 256 and self.H == 256 and B % 128 == 0
 
-    def _layer_fused(self, inverse, layer, zp, B, logdet, stream):
-        """The whole coupling layer in one launch (pib_flow_layer_fused)."""
+    def _layer_fused(self, inverse, layer, zp, B, logdet, stream, single_bf16=False):
+        """The whole coupling layer in one launch (pib_flow_layer_fused); single_bf16: one bf16
+        product per weight block (the hi parts) instead of BF16x3."""
         a_off, b_off = self.halves(layer)
         self._split_weights_fused(stream)
+        w = self._wfused[layer]
+        if single_bf16:
+            w = tuple((hi, None) for hi, _ in w)
         self.ctx.flow_layer_fused(inverse, B, self.D, self.hidden, a_off, b_off, self.smax, zp, logdet,
-                                  self._wfused[layer], self.p(layer, "b1"), self.p(layer, "b2"),
-                                  self.p(layer, "b3"), stream)
+                                  w, self.p(layer, "b1"), self.p(layer, "b2"), self.p(layer, "b3"), stream)
 
     # ------------------------------------------------------------------ directions
     def forward(self, R0, stream=None):
@@ -241,18 +247,21 @@ class Flow:
         self.ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, R, stream)
         return R, logdet
 
-    def _layers_forward(self, zp, B, logdet, stream):
+    def _layers_forward(self, zp, B, logdet, stream, single_bf16=False):
         """The coupling layers in the sampling direction on zp (B, D) in place; adds
-        log|det| to logdet."""
+        log|det| to logdet. single_bf16 (one-launch layers only): single bf16 products."""
         D, h = self.D, self.hidden
         h1, h2, o = self._buf("h1", (B, h)), self._buf("h2", (B, h)), self._buf("o", (B, D))
         fast = self.gemm == "bf16x3"
+        if single_bf16 and not (fast and self._fused_ok(B)):
+            raise ValueError("single-bf16 layers need the one-launch layer (gemm='bf16x3', hidden 256, "
+                             "D = 512, B a multiple of 128)")
         if fast:
             self._split_weights(stream)
         for l in range(self.layers):
             a_off, b_off = self.halves(l)
             if fast and self._fused_ok(B):
-                self._layer_fused(False, l, zp, B, logdet, stream)
+                self._layer_fused(False, l, zp, B, logdet, stream, single_bf16)
                 continue
             if fast:
                 self._mlp_fast(l, zp, B, o, stream)
@@ -353,7 +362,13 @@ class Flow:
         """B proposal samples: R (B, N, P) double and log q(R) (B,). The base sample is drawn
         directly in the flow's coordinates with its log rho (pib_flow_draw_base).
         layer_events: optional pair of torch.cuda.Event recorded on `stream` around the
-        coupling layers (to time them apart from the base draw and the unwhitening)."""
+        coupling layers (to time them apart from the base draw and the unwhitening).
+
+        With draw_precision "bf16" the layers' networks run as single bf16 products. log q is
+        still exact for the map applied: a coupling layer passes its conditioning half through
+        unchanged, so its log-determinant is the sum of the scales it applied, whatever
+        function of that half computed them; the weights g/q stay unbiased, and only how well
+        q matches the trained function (the ESS) depends on the precision."""
         import torch
         self._check_batch(B)
         zp = self._buf("zp", (B, self.D))
@@ -362,7 +377,7 @@ class Flow:
         logdet = torch.zeros(B, dtype=torch.float64, device=self.dev)
         if layer_events is not None:
             layer_events[0].record(stream)
-        self._layers_forward(zp, B, logdet, stream)
+        self._layers_forward(zp, B, logdet, stream, single_bf16=self.draw_precision == "bf16")
         if layer_events is not None:
             layer_events[1].record(stream)
         R = torch.empty((B, self.N, self.P), dtype=torch.float64, device=self.dev)

@@ -408,6 +408,75 @@ def test_fused_layer_matches_layer_by_layer(tmp_path, N, P):
     assert dR <= 5e-5 * scale and dld <= 1e-2
 
 
+def test_single_bf16_layer_is_an_exact_coupling(tmp_path):
+    """One-launch layers with single bf16 products (lo parts null): the network differs from
+    the BF16x3 one by bf16 rounding, but the layer is still an exact coupling. The conditioning
+    half passes bit for bit; the inverse layer (the same scales, computed from the same half)
+    returns the input to FP32 rounding; and the forward and inverse log-determinants cancel
+    exactly (same scales, same summation order). The transformed half stays within bf16
+    rounding of the network of the BF16x3 layer's."""
+    _announce("fused coupling layer, single bf16")
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, 8, quadratic=False, seed=2))
+    data = make_data(FS, 300.0, 64, 64, 64)
+    fl = flow.Flow(data, layers=2, hidden=256, gemm="bf16x3")
+    fl.set_params(random_params(ref_for(data, 2, 256, 2.0), 5, 0.05))
+    B, D = 256, fl.D
+    fl._split_weights(None)
+    z0 = torch.randn((B, D), generator=torch.Generator().manual_seed(1)).to(DEV)
+    for l in range(2):
+        a_off, b_off = fl.halves(l)
+        z1, l1 = z0.clone(), torch.zeros(B, dtype=torch.float64, device=DEV)
+        z3, l3 = z0.clone(), torch.zeros(B, dtype=torch.float64, device=DEV)
+        fl._layer_fused(False, l, z1, B, l1, None, single_bf16=True)
+        fl._layer_fused(False, l, z3, B, l3, None)
+        torch.cuda.synchronize()
+        scale = float(z3.abs().max())
+        dz, dl = float((z1 - z3).abs().max()), float((l1 - l3).abs().max())
+        print(f"layer {l}: single bf16 vs bf16x3 max |dz| {dz:.3e} (|z| <= {scale:.1f}), |d logdet| {dl:.3e}")
+        assert torch.equal(z1[:, a_off:a_off + fl.H], z0[:, a_off:a_off + fl.H])
+        assert 0.0 < dz <= 2e-2 * scale
+        fl._layer_fused(True, l, z1, B, l1, None, single_bf16=True)
+        torch.cuda.synchronize()
+        assert float((z1 - z0).abs().max()) <= 1e-6 * scale
+        assert torch.equal(l1, torch.zeros_like(l1))
+    # the lo parts are all given or all null
+    w = fl._wfused[0]
+    with pytest.raises(_native.NativeError):
+        fl.ctx.flow_layer_fused(False, B, D, 256, 0, D // 2, fl.smax, z0, l1,
+                                ((w[0][0], None), w[1], w[2]), fl.p(0, "b1"), fl.p(0, "b2"), fl.p(0, "b3"))
+
+
+def test_single_bf16_draws_after_training(tmp_path):
+    """A trained flow at A=4, N=8, P=64 (D = 512: the one-launch layers) drawn with single-bf16
+    layers: the same unbiased Z/Z_rho as its BF16x3 draws within their errors, and an ESS
+    fraction within 10% of theirs (log q is exact for the map applied either way)."""
+    _announce("single-bf16 draws of a trained flow")
+    FS = stage_dict_models(tmp_path, synthetic.generate(4, 8, quadratic=True, seed=0))
+    data = make_data(FS, 300.0, 64, 1024, 1024, seed=3)
+    from pibronic_amd import learn
+    fl = flow.Flow(data, layers=8, hidden=256, seed=1, gemm="bf16x3")
+    learn.train(fl, 400, 2048, 1e-3, 2)
+    B, n = 65536, 6
+    lws = {}
+    for prec in ("bf16x3", "bf16"):
+        fl.draw_precision = prec
+        parts = []
+        for i in range(n):
+            R, lq = fl.draw(B, 77, i * B)
+            lw, sg = flow.log_weights(data, R, lq)  # log|g/q| (-inf where g underflows), sign
+            parts.append((lw.cpu().numpy(), sg.cpu().numpy()))
+        lws[prec] = (np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts]))
+    ref = max(v[0][np.isfinite(v[0])].max() for v in lws.values())
+    res = {}
+    for prec, (lw, sg) in lws.items():
+        w = sg * np.exp(lw - ref)
+        res[prec] = (w.mean(), w.std() / np.sqrt(w.size), w.sum() ** 2 / (w * w).sum() / w.size)
+    print(res)
+    (z3, e3, f3), (z1, e1, f1) = res["bf16x3"], res["bf16"]
+    assert abs(z1 - z3) < 4 * np.hypot(e1, e3)
+    assert f1 >= 0.9 * f3
+
+
 def test_base_draw_density_matches_oracle(model):
     """pib_flow_draw_base: log rho at the drawn point equals the reference's log rho(R0) for
     R0 = m + (Z sigma) U^T formed in float64 from the float Z (oracle: the O-matrix product of
