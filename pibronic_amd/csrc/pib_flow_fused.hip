@@ -159,8 +159,15 @@ template <int NB, int LD, bool LO>
 __device__ __forceinline__ void tile_product(const unsigned short* buf, const Op8 (&bh)[kNT][NB],
                                              const Op8 (&bl)[kNT][NB], int li, int g, int tile_elems,
                                              f32x4f (&acc)[kNT]) {
+  // single bf16 products: even and odd blocks in two accumulator chains, added at the end
+  // (with one chain of NB the register allocator emitted products whose destination partly
+  // overlaps their accumulator input)
+  f32x4f odd[kNT];
 #pragma unroll
-  for (int j = 0; j < kNT; ++j) acc[j] = f32x4f{0.f, 0.f, 0.f, 0.f};
+  for (int j = 0; j < kNT; ++j) {
+    acc[j] = f32x4f{0.f, 0.f, 0.f, 0.f};
+    odd[j] = acc[j];
+  }
 #pragma unroll
   for (int c = 0; c < NB; ++c) {
     const unsigned short* p = buf + li * LD + 32 * c + 8 * g;
@@ -171,9 +178,17 @@ __device__ __forceinline__ void tile_product(const unsigned short* buf, const Op
       for (int j = 0; j < kNT; ++j) acc[j] = mfma3(ah, al, as_bf(bh[j][c]), as_bf(bl[j][c]), acc[j]);
     } else {
 #pragma unroll
-      for (int j = 0; j < kNT; ++j)
-        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf(bh[j][c]), acc[j], 0, 0, 0);
+      for (int j = 0; j < kNT; ++j) {
+        if (c & 1)
+          odd[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf(bh[j][c]), odd[j], 0, 0, 0);
+        else
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf(bh[j][c]), acc[j], 0, 0, 0);
+      }
     }
+  }
+  if constexpr (!LO) {
+#pragma unroll
+    for (int j = 0; j < kNT; ++j) acc[j] += odd[j];
   }
 }
 
@@ -215,6 +230,7 @@ __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
     }
 
   Op8 h1h[kNT][NBH], h1l[kNT][NBH], h2h[kNT][NBH], h2l[kNT][NBH];
+  const float ismax = 1.0f / a.smax;
   double part[kNT];
 #pragma unroll
   for (int j = 0; j < kNT; ++j) part[j] = 0.0;
@@ -296,9 +312,21 @@ __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
       float zz[4] = {zv[j].x, zv[j].y, zv[j].z, zv[j].w};
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float sc = a.smax * tanhf((sraw[j][r] + a.b3[f0 + r]) / a.smax);
         const float sh = acc[j][r] + a.b3[H + f0 + r];
-        zz[r] = a.inverse ? (zz[r] - sh) * expf(-sc) : fmaf(zz[r], expf(sc), sh);
+        float sc, es;
+        if constexpr (LO) {  // the layer-by-layer path's arithmetic (pib_flow_coupling)
+          sc = a.smax * tanhf((sraw[j][r] + a.b3[f0 + r]) / a.smax);
+          es = expf(a.inverse ? -sc : sc);
+        } else {
+          // single-bf16 draws: tanh from the hardware exp2 and reciprocal (it only shapes the
+          // network's output), e^{+-s} from the hardware exp2 (~2e-7 relative; the log-det
+          // adds the same s the map exponentiates)
+          const float x = (sraw[j][r] + a.b3[f0 + r]) * ismax;
+          const float e2 = __builtin_amdgcn_exp2f(2.8853900817779268f * x);  // e^{2x}
+          sc = a.smax * fmaf(-2.0f, __builtin_amdgcn_rcpf(e2 + 1.0f), 1.0f);
+          es = __builtin_amdgcn_exp2f((a.inverse ? -1.4426950408889634f : 1.4426950408889634f) * sc);
+        }
+        zz[r] = a.inverse ? (zz[r] - sh) * es : fmaf(zz[r], es, sh);
         part[j] += (double)sc;
       }
       *(float4*)(zrow[j] + a.b_off + f0) = float4{zz[0], zz[1], zz[2], zz[3]};
