@@ -246,20 +246,36 @@ __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
 #pragma unroll
   for (int b = 0; b < kBaseMaxA; ++b) dot[b] = 0.0;
   if (live) {
-    for (int m = sl; 2 * m < a.D; m += kBaseLanes) {
+    // four features (two Box-Muller pairs) per Philox call: 32-bit uniforms, the radius one
+    // refined to 64 bits from a second call when it is below 2^-20, so that the tails reach
+    // ~9.4 sigma (a 32-bit radius uniform alone would stop at 6.7 sigma); the transform in
+    // FP32 with the hardware log2, sqrt and sin/cos (z is float)
+    for (int m = sl; 4 * m < a.D; m += kBaseLanes) {
       const u32x4 r = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), (uint32_t)m, 2u}, a.key0, a.key1);
-      const double u1 = 1.0 - u53(r.x, r.y);  // (0, 1]
-      const double u2 = u53(r.z, r.w);
-      const double rad = sqrt(-2.0 * log(u1));
-      double sn, cs;
-      sincospi(2.0 * u2, &sn, &cs);
-      const float z0 = (float)(rad * cs + mus[2 * m]), z1 = (float)(rad * sn + mus[2 * m + 1]);
-      *(float2*)(zr + 2 * m) = float2{z0, z1};
-      const double d0 = z0, d1 = z1;
-      zz = fma(d0, d0, fma(d1, d1, zz));
+      float l2a = __log2f(((float)r.x + 0.5f) * 0x1.0p-32f);
+      float l2b = __log2f(((float)r.z + 0.5f) * 0x1.0p-32f);
+      if (r.x < (1u << 12) || r.z < (1u << 12)) {  // rare: P = 2^-19 per call
+        const u32x4 t = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), (uint32_t)m, 3u}, a.key0, a.key1);
+        if (r.x < (1u << 12)) l2a = (float)log2(((double)r.x * 0x1.0p32 + (double)t.x + 0.5) * 0x1.0p-64);
+        if (r.z < (1u << 12)) l2b = (float)log2(((double)r.z * 0x1.0p32 + (double)t.z + 0.5) * 0x1.0p-64);
+      }
+      const float ra = __fsqrt_rn(-1.3862943611198906f * l2a), rb = __fsqrt_rn(-1.3862943611198906f * l2b);
+      const float ua = ((float)r.y + 0.5f) * 0x1.0p-32f, ub = ((float)r.w + 0.5f) * 0x1.0p-32f;
+      // v_cos_f32 / v_sin_f32 take their argument in revolutions
+      const float zf[4] = {ra * __builtin_amdgcn_cosf(ua), ra * __builtin_amdgcn_sinf(ua),
+                           rb * __builtin_amdgcn_cosf(ub), rb * __builtin_amdgcn_sinf(ub)};
+      float zo[4];
 #pragma unroll
-      for (int b = 0; b < kBaseMaxA; ++b)
-        if (b < a.A_rho) dot[b] = fma(d0, a.mu[(int64_t)b * a.D + 2 * m], fma(d1, a.mu[(int64_t)b * a.D + 2 * m + 1], dot[b]));
+      for (int e = 0; e < 4; ++e) zo[e] = (float)((double)zf[e] + mus[4 * m + e]);
+      *(float4*)(zr + 4 * m) = float4{zo[0], zo[1], zo[2], zo[3]};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const double d = zo[e];
+        zz = fma(d, d, zz);
+#pragma unroll
+        for (int b = 0; b < kBaseMaxA; ++b)
+          if (b < a.A_rho) dot[b] = fma(d, a.mu[(int64_t)b * a.D + 4 * m + e], dot[b]);
+      }
     }
   }
   zz = group16_sum(zz);
@@ -281,7 +297,7 @@ __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
 }
 
 hipError_t launch_flow_base(const BaseArgs& a, hipStream_t s) {
-  if (a.B <= 0 || a.D <= 0 || (a.D & 1) || a.A_rho < 1 || a.A_rho > kBaseMaxA) return hipErrorInvalidValue;
+  if (a.B <= 0 || a.D <= 0 || (a.D & 3) || a.A_rho < 1 || a.A_rho > kBaseMaxA) return hipErrorInvalidValue;
   const int64_t grid = (a.B + 256 / kBaseLanes - 1) / (256 / kBaseLanes);
   if (grid > 0x7fffffff) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_flow_base, dim3((unsigned)grid), dim3(256), 0, s, a);
