@@ -216,10 +216,11 @@ hipError_t launch_whiten_fused(int mode, const BasisArgs& b, const void* in, voi
 
 // ------------------------------------------------------------------ flow base draws
 // Sixteen lanes per sample, four samples per wave: the surface a from the cumulative weights
-// (the sampler's counter (s, 0, 1)), normal pairs from Philox counters (s, m, 2) by Box-Muller
-// in FP64, z = eps + mu_a rounded to float (the value the flow receives; its density is
-// evaluated at that value), and log rho = logsumexp_b (c_b + z . mu_b) - |z|^2 / 2 with
-// fixed-order sums over the sample's sixteen lanes.
+// (the sampler's counter (s, 0, 1)), features 4m..4m+3 from the Philox counter (s, m, 2) by
+// Box-Muller (refinement of a small radius uniform from (s, m, 3)), z = eps + mu_a rounded to
+// float (the value the flow receives; its density is evaluated at that value), and
+// log rho = logsumexp_b (c_b + z . mu_b) - |z|^2 / 2 with fixed-order sums over the sample's
+// sixteen lanes.
 constexpr int kBaseMaxA = 16;
 constexpr int kBaseLanes = 16;
 

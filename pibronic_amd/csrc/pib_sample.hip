@@ -12,9 +12,11 @@
 // from N(0, Lambda_j^-1) by an O(P) back substitution (the standard sampler for Gaussian
 // Markov random fields, Rue 2001). Same distribution, O(P) instead of O(P^2) per row.
 //
-// Random numbers: Philox4x32-10 keyed by the run seed. The normal pair m of row
-// (global sample s, mode j) uses the counter (s*N + j, m, 0): one call, two 53-bit uniforms,
-// two normals by Box-Muller. The surface source of sample s uses the counter (s, 0, 1).
+// Random numbers: Philox4x32-10 keyed by the run seed. The beads 4m..4m+3 of row (global
+// sample s, mode j) take the four normals of the counter (s*N + j, m, 0): two Box-Muller
+// pairs, each from a 32-bit radius uniform and a 32-bit angle uniform, in FP64; a radius
+// uniform below 2^-20 is refined to 64 bits by the counter (s*N + j, m, 2), so that the tails
+// reach ~9.4 sigma. The surface source of sample s uses the counter (s, 0, 1).
 // Sources are drawn per sample rather than once per run (quirk Q5 of the reference draws them
 // all up front; same distribution).
 //
@@ -40,16 +42,23 @@ __device__ __forceinline__ int draw_source(const SampleArgs& a, int64_t gsample)
   return s;
 }
 
-__device__ __forceinline__ void normal_pair(uint64_t grow, int m, uint32_t k0, uint32_t k1,
-                                            double& ze, double& zo) {
+// the four normals of beads 4m..4m+3 of row grow
+__device__ __forceinline__ void normal_quad(uint64_t grow, int m, uint32_t k0, uint32_t k1, double (&z)[4]) {
   const u32x4 r = philox4x32_10(u32x4{(uint32_t)grow, (uint32_t)(grow >> 32), (uint32_t)m, 0u}, k0, k1);
-  const double u1 = 1.0 - u53(r.x, r.y);  // (0, 1]
-  const double u2 = u53(r.z, r.w);
-  const double rad = sqrt(-2.0 * log(u1));
-  double sn, cs;
-  sincospi(2.0 * u2, &sn, &cs);
-  ze = rad * cs;
-  zo = rad * sn;
+  double ua = ((double)r.x + 0.5) * 0x1.0p-32, ub = ((double)r.z + 0.5) * 0x1.0p-32;  // (0, 1)
+  if (r.x < (1u << 12) || r.z < (1u << 12)) {  // rare: P = 2^-19 per call
+    const u32x4 t = philox4x32_10(u32x4{(uint32_t)grow, (uint32_t)(grow >> 32), (uint32_t)m, 2u}, k0, k1);
+    if (r.x < (1u << 12)) ua = ((double)r.x * 0x1.0p32 + (double)t.x + 0.5) * 0x1.0p-64;
+    if (r.z < (1u << 12)) ub = ((double)r.z * 0x1.0p32 + (double)t.z + 0.5) * 0x1.0p-64;
+  }
+  const double ra = sqrt(-2.0 * log(ua)), rb = sqrt(-2.0 * log(ub));
+  double sa, ca, sb, cb;
+  sincospi(2.0 * (((double)r.y + 0.5) * 0x1.0p-32), &sa, &ca);
+  sincospi(2.0 * (((double)r.w + 0.5) * 0x1.0p-32), &sb, &cb);
+  z[0] = ra * ca;
+  z[1] = ra * sa;
+  z[2] = rb * cb;
+  z[3] = rb * sb;
 }
 
 __global__ void __launch_bounds__(kWave * kSampleWaves) k_sample_fp64(const SampleArgs a) {
@@ -76,25 +85,35 @@ __global__ void __launch_bounds__(kWave * kSampleWaves) k_sample_fp64(const Samp
     const double d = a.shift[src * N + j];
     if (live && a.src && j == 0) a.src[s] = src;
     const int64_t nrow = (rows - r0) < kWave ? (rows - r0) : kWave;
-    double ze = 0.0, zo = 0.0, xl = 0.0, xn = 0.0;
-    // beads in descending order, in chunks [kb, ke)
-    for (int ke = P; ke > 0; ke -= kChunk) {
-      const int kb = ke - kChunk > 0 ? ke - kChunk : 0;
+    double zq[4], xl = 0.0, xn = 0.0;
+    // beads in descending order, in chunks [kb, ke) aligned to multiples of kChunk, and within
+    // a chunk quad by quad: the quad's factor entries are loaded before its four normals are
+    // drawn, which hides their latency. One formula for every bead: the host stores sub[k] = 0
+    // for k >= P-2 and last[P-1] = 0, and x_{P-1} (xl) and x_{k+1} (xn) start at 0, so bead P-1
+    // gives z dinv and bead P-2 (z - last xl) dinv exactly.
+    for (int kb = ((P - 1) / kChunk) * kChunk; kb >= 0; kb -= kChunk) {
+      const int ke = kb + kChunk < P ? kb + kChunk : P;
       if (live) {
-        for (int k = ke - 1; k >= kb; --k) {
-          if (k == P - 1 || (k & 1)) normal_pair(grow, k >> 1, a.key0, a.key1, ze, zo);
-          const double z = (k & 1) ? zo : ze;
-          double x;
-          if (k == P - 1) {
-            x = z * dinv[k];
-            xl = x;
-          } else if (k == P - 2) {
-            x = fma(-last[k], xl, z) * dinv[k];
-          } else {
-            x = fma(-last[k], xl, fma(-sub[k], xn, z)) * dinv[k];
+        for (int m = (ke - 1) >> 2; m >= (kb >> 2); --m) {
+          double cd[4], cs[4], cl[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int k = 4 * m + e < P ? 4 * m + e : P - 1;  // the top quad may be partial
+            cd[e] = dinv[k];
+            cs[e] = sub[k];
+            cl[e] = last[k];
           }
-          xn = x;
-          stage[lane * kLd + (k - kb)] = x + d;
+          normal_quad(grow, m, a.key0, a.key1, zq);
+#pragma unroll
+          for (int e = 3; e >= 0; --e) {
+            const int k = 4 * m + e;
+            if (k < ke) {  // uniform over the wave
+              const double x = fma(-cl[e], xl, fma(-cs[e], xn, zq[e])) * cd[e];
+              if (k == P - 1) xl = x;
+              xn = x;
+              stage[lane * kLd + (k - kb)] = x + d;
+            }
+          }
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -102,9 +121,17 @@ __global__ void __launch_bounds__(kWave * kSampleWaves) k_sample_fp64(const Samp
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       // write the chunk: row rr's beads [kb, ke) are contiguous in q
       const int w_ = ke - kb;
-      for (int e = lane; e < nrow * w_; e += kWave) {
-        const int rr = e / w_, kk = e - rr * w_;
-        a.q[(r0 + rr) * P + kb + kk] = stage[rr * kLd + kk];
+      if (w_ == kChunk && nrow == kWave) {  // full chunk: no divisions
+#pragma unroll
+        for (int it = 0; it < kChunk; ++it) {
+          const int e = lane + kWave * it, rr = e / kChunk, kk = e % kChunk;
+          a.q[(r0 + rr) * P + kb + kk] = stage[rr * kLd + kk];
+        }
+      } else {
+        for (int e = lane; e < nrow * w_; e += kWave) {
+          const int rr = e / w_, kk = e - rr * w_;
+          a.q[(r0 + rr) * P + kb + kk] = stage[rr * kLd + kk];
+        }
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
