@@ -144,7 +144,7 @@ def learned_leg(data, args, log):
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     # the timed draws: single-bf16 layers only where the one-launch layer runs
-    draw_prec = args.flow_draw if (args.flow_gemm == "bf16x3" and fl._fused_ok(args.flow_block)) else "bf16x3"
+    draw_prec = args.flow_draw if (args.flow_gemm == "bf16x3" and fl._fused_ok(args.flow_block, True)) else "bf16x3"
     fl.draw_precision = draw_prec
     B = args.flow_block
     K, W = args.steps, args.warmup
@@ -180,7 +180,7 @@ def learned_leg(data, args, log):
     layers_ms = sum(a.elapsed_time(b) for a, b in zip(l0, l1)) / K
     H, hdn, D = fl.H, fl.hidden, fl.D
     mlp_flops = 2.0 * B * fl.layers * (H * hdn + hdn * hdn + hdn * D)
-    fused = fl.use_fused and fl._fused_ok(B)
+    fused = fl.use_fused and fl._fused_ok(B, draw_prec == "bf16")
     # the coupling layers (the learned step's largest cost) against the bf16 matrix cores:
     # algorithmic flops are the dense products' 2 m n k; BF16x3 issues three bf16 products
     # for each (single-bf16 draws one), so the matrix cores execute that multiple

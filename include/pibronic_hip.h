@@ -241,7 +241,8 @@ int pib_flow_coupling_inverse_backward(pib_ctx* ctx, int64_t B, int D, int a_off
  * PIB_SPLIT_TRANSPOSE | PIB_SPLIT_PERM32 (hidden x hidden, D x hidden). Same arithmetic as the
  * layer-by-layer calls (pib_gemm_bf16x3_presplit x3 + pib_flow_coupling) up to the FP32
  * summation order; the activations and the MLP output never leave the chip. Supported:
- * hidden 256, D = 512, B a multiple of 128. W1_lo_dev, W2_lo_dev and W3_lo_dev may all be NULL:
+ * hidden 256, D = 512 (and D = 1024 with the lo parts NULL), B a multiple of 128.
+ * W1_lo_dev, W2_lo_dev and W3_lo_dev may all be NULL:
  * the products are then single bf16 products of the hi parts (a third of the matrix-core work;
  * the coupling and its log-determinant are exact for whatever scales and shifts the network
  * produced, so densities of the map applied stay exact). No counterpart in the reference. */

@@ -626,8 +626,9 @@ int pib_flow_layer_fused(pib_ctx* c, int inverse, int64_t B, int D, int hidden, 
     return fail(c, PIB_ERR_ARG, "flow_layer_fused: null pointer");
   if ((W1_lo == nullptr) != (W2_lo == nullptr) || (W1_lo == nullptr) != (W3_lo == nullptr))
     return fail(c, PIB_ERR_ARG, "flow_layer_fused: the lo parts must be all given or all null");
-  if (!fused_layer_supported(B, D, hidden))
-    return fail(c, PIB_ERR_UNSUPPORTED, "flow_layer_fused: needs hidden 256, D = 512 and B a multiple of 128");
+  if (!fused_layer_supported(B, D, hidden, W1_lo != nullptr))
+    return fail(c, PIB_ERR_UNSUPPORTED, "flow_layer_fused: needs hidden 256, D = 512 (or 1024 with single "
+                                        "bf16 products) and B a multiple of 128");
   const int H = D / 2;
   if (!((a_off == 0 && b_off == H) || (a_off == H && b_off == 0)))
     return fail(c, PIB_ERR_ARG, "flow_layer_fused: the halves must be (0, D/2) or (D/2, 0)");

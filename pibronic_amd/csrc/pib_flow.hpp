@@ -117,7 +117,7 @@ struct FusedLayerArgs {
   const unsigned short *W1h, *W1l, *W2h, *W2l, *W3h, *W3l;
   const float *b1, *b2, *b3;
 };
-bool fused_layer_supported(int64_t B, int D, int hidden);
+bool fused_layer_supported(int64_t B, int D, int hidden, bool lo);
 hipError_t launch_flow_layer(const FusedLayerArgs& a, hipStream_t s);
 hipError_t launch_coupling_inverse_backward(const CouplingArgs& a, hipStream_t s);
 

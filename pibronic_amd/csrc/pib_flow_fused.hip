@@ -357,19 +357,21 @@ hipError_t launch_hh(const FusedLayerArgs& a, hipStream_t s) {
 
 }  // namespace
 
-// D/2 = 256 only: at D/2 = 512 the first product's operands (128 registers) push the kernel
-// past 256 VGPRs, and the compiler then assigns some chained MFMAs a destination that partly
-// overlaps their accumulator input, which the matrix cores do not compute correctly.
-bool fused_layer_supported(int64_t B, int D, int hidden) {
-  return B > 0 && B % kRows == 0 && hidden == 256 && D == 512;
+// BF16x3 at D/2 = 256 only: at D/2 = 512 the first product's hi and lo operands (128
+// registers) push the kernel past 256 VGPRs, and the compiler then assigns some chained MFMAs
+// a destination that partly overlaps their accumulator input, which the matrix cores do not
+// compute correctly. Single bf16 products (no lo operands) also run at D/2 = 512.
+bool fused_layer_supported(int64_t B, int D, int hidden, bool lo) {
+  return B > 0 && B % kRows == 0 && hidden == 256 && (D == 512 || (!lo && D == 1024));
 }
 
 hipError_t launch_flow_layer(const FusedLayerArgs& a, hipStream_t s) {
-  if (!fused_layer_supported(a.B, a.D, a.hidden)) return hipErrorInvalidValue;
-  if ((a.a_off != 0 && a.a_off != a.D / 2) || (a.b_off != 0 && a.b_off != a.D / 2) || a.a_off == a.b_off)
-    return hipErrorInvalidValue;
   const bool lo = a.W1l != nullptr;
   if (lo != (a.W2l != nullptr) || lo != (a.W3l != nullptr)) return hipErrorInvalidValue;
+  if (!fused_layer_supported(a.B, a.D, a.hidden, lo)) return hipErrorInvalidValue;
+  if ((a.a_off != 0 && a.a_off != a.D / 2) || (a.b_off != 0 && a.b_off != a.D / 2) || a.a_off == a.b_off)
+    return hipErrorInvalidValue;
+  if (a.D == 1024) return launch_hh<512, 256, false>(a, s);
   return lo ? launch_hh<256, 256, true>(a, s) : launch_hh<256, 256, false>(a, s);
 }
 

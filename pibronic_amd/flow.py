@@ -217,8 +217,11 @@ class Flow:
         ctx.gemm_bf16x3_presplit(B, D, h, bh, bl, h, w3h, w3l, h, C=o, ldc=D,
                                  bias=self.p(layer, "b3"), stream=stream)
 
-    def _fused_ok(self, B):
-        return self.use_fused and self.hidden == 256 and self.H == 256 and B % 128 == 0
+    def _fused_ok(self, B, single_bf16=False):
+        """The one-launch layer runs hidden 256 at D = 512, and with single bf16 products
+        also at D = 1024; B a multiple of 128."""
+        return (self.use_fused and self.hidden == 256 and B % 128 == 0
+                and (self.H == 256 or (single_bf16 and self.H == 512)))
 
     def _layer_fused(self, inverse, layer, zp, B, logdet, stream, single_bf16=False):
         """The whole coupling layer in one launch (pib_flow_layer_fused); single_bf16: one bf16
@@ -253,14 +256,14 @@ class Flow:
         D, h = self.D, self.hidden
         h1, h2, o = self._buf("h1", (B, h)), self._buf("h2", (B, h)), self._buf("o", (B, D))
         fast = self.gemm == "bf16x3"
-        if single_bf16 and not (fast and self._fused_ok(B)):
+        if single_bf16 and not (fast and self._fused_ok(B, True)):
             raise ValueError("single-bf16 layers need the one-launch layer (gemm='bf16x3', hidden 256, "
-                             "D = 512, B a multiple of 128)")
+                             "D = 512 or 1024, B a multiple of 128)")
         if fast:
             self._split_weights(stream)
         for l in range(self.layers):
             a_off, b_off = self.halves(l)
-            if fast and self._fused_ok(B):
+            if fast and self._fused_ok(B, single_bf16):
                 self._layer_fused(False, l, zp, B, logdet, stream, single_bf16)
                 continue
             if fast:

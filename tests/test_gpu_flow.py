@@ -408,15 +408,17 @@ def test_fused_layer_matches_layer_by_layer(tmp_path, N, P):
     assert dR <= 5e-5 * scale and dld <= 1e-2
 
 
-def test_single_bf16_layer_is_an_exact_coupling(tmp_path):
+@pytest.mark.parametrize("N", [8, 16])
+def test_single_bf16_layer_is_an_exact_coupling(tmp_path, N):
     """One-launch layers with single bf16 products (lo parts null): the network differs from
     the BF16x3 one by bf16 rounding, but the layer is still an exact coupling. The conditioning
     half passes bit for bit; the inverse layer (the same scales, computed from the same half)
     returns the input to FP32 rounding; and the forward and inverse log-determinants cancel
     exactly (same scales, same summation order). The transformed half stays within bf16
-    rounding of the network of the BF16x3 layer's."""
-    _announce("fused coupling layer, single bf16")
-    FS = stage_dict_models(tmp_path, synthetic.generate(2, 8, quadratic=False, seed=2))
+    rounding of the network of the BF16x3 layer's (at D = 1024, which has no one-launch BF16x3
+    layer, of the layer-by-layer BF16x3 path's)."""
+    _announce(f"fused coupling layer, single bf16, D={N * 64}")
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, N, quadratic=False, seed=2))
     data = make_data(FS, 300.0, 64, 64, 64)
     fl = flow.Flow(data, layers=2, hidden=256, gemm="bf16x3")
     fl.set_params(random_params(ref_for(data, 2, 256, 2.0), 5, 0.05))
@@ -428,7 +430,12 @@ def test_single_bf16_layer_is_an_exact_coupling(tmp_path):
         z1, l1 = z0.clone(), torch.zeros(B, dtype=torch.float64, device=DEV)
         z3, l3 = z0.clone(), torch.zeros(B, dtype=torch.float64, device=DEV)
         fl._layer_fused(False, l, z1, B, l1, None, single_bf16=True)
-        fl._layer_fused(False, l, z3, B, l3, None)
+        if D == 512:
+            fl._layer_fused(False, l, z3, B, l3, None)
+        else:
+            o = torch.empty((B, D), device=DEV)
+            fl._mlp_fast(l, z3, B, o, None)
+            fl.ctx.flow_coupling(False, B, D, a_off, b_off, fl.smax, z3, o, l3, None)
         torch.cuda.synchronize()
         scale = float(z3.abs().max())
         dz, dl = float((z1 - z3).abs().max()), float((l1 - l3).abs().max())
