@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix peaks coincide)
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 matrix cores
+HBM_PEAK_BPS = 8.0e12  # MI355X HBM3E
 HBM_PEAK_GBS = 8000.0
 
 
@@ -442,7 +443,10 @@ def main():
             f"void pib::k_eval_fp64<{A}, {A}, false, false>")
 
     learned = None
-    if not args.no_learned and not pm:
+    if not args.no_learned and not pm and P % 64:
+        # the flow's normal-mode whitening works in MFMA tiles of 64 beads
+        log(f"[bench] learned leg skipped: P = {P} is not a multiple of 64")
+    elif not args.no_learned and not pm:
         learned = learned_leg(data, args, log)
         log(f"[bench] learned: ess fraction {learned['ess_fraction']:.3e}, "
             f"{learned['ess_per_s']:.3e} ESS/s (harmonic {harm['ess_per_s']:.3e})")
@@ -477,6 +481,8 @@ def main():
         "reduced_precision_error_vs_fp64": fp32_err,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "hbm_GBps": (traffic / (eval_ms * 1e-3) / 1e9) if (traffic and eval_ms) else None,
+                     "hbm_frac": (traffic / (eval_ms * 1e-3) / HBM_PEAK_BPS) if (traffic and eval_ms) else None,
                      "kernel": ("k_eval_fp32" if args.fp32 else "k_eval_wide" if (A > 8 or args.ring_bf16x3 or args.wide)
                                 else "k_eval_fp64"), "flops_per_sample": fl["eval"],
                      "algorithmic_bytes": B * N * P * 8 + B * (5 if pm else 3) * 8,
