@@ -453,6 +453,54 @@ def test_single_bf16_layer_is_an_exact_coupling(tmp_path, N):
                                 ((w[0][0], None), w[1], w[2]), fl.p(0, "b1"), fl.p(0, "b2"), fl.p(0, "b3"))
 
 
+@pytest.mark.parametrize("layer", [0, 1])
+def test_single_bf16_layer_matches_restatement(tmp_path, layer):
+    """The one-launch single-bf16 layer at D = 512 against the network it computes, restated in
+    float64 torch: the conditioning half and the weights rounded to bf16 (round to nearest
+    even), h1 = bf16(relu(x W1 + b1)), h2 = bf16(relu(h1 W2 + b2)), (raw, t) = h2 W3 + b3,
+    s = smax tanh(raw / smax), y = z e^s + t and log-det sum s. The kernel accumulates in FP32,
+    so a hidden activation within an FP32 rounding of a bf16 rounding boundary may round the
+    other way; such flips are rare and each moves y by about a bf16 ulp of one hidden unit.
+    Bounds: 99% of the y elements within 1e-5 of the largest |y| (median within 1e-6), all
+    within 2e-2; 95% of the log-dets within 1e-4 relative (or absolute below 1)."""
+    _announce(f"single-bf16 layer {layer} vs float64 restatement, D=512")
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, 8, quadratic=False, seed=2))
+    data = make_data(FS, 300.0, 64, 64, 64)
+    fl = flow.Flow(data, layers=2, hidden=256, gemm="bf16x3")
+    fl.set_params(random_params(ref_for(data, 2, 256, 2.0), 7, 0.05))
+    B, D, H = 2048, fl.D, fl.H
+    assert D == 512
+    fl._split_weights(None)
+    z0 = torch.randn((B, D), generator=torch.Generator().manual_seed(3)).to(DEV)
+    z1, l1 = z0.clone(), torch.zeros(B, dtype=torch.float64, device=DEV)
+    fl._layer_fused(False, layer, z1, B, l1, None, single_bf16=True)
+    torch.cuda.synchronize()
+
+    def bf(t):
+        return t.float().to(torch.bfloat16).double()
+
+    a_off, b_off = fl.halves(layer)
+    x = bf(z0[:, a_off:a_off + H].cpu())
+    W1, W2, W3 = (bf(fl.p(layer, n).cpu()) for n in ("W1", "W2", "W3"))
+    b1, b2, b3 = (fl.p(layer, n).cpu().double() for n in ("b1", "b2", "b3"))
+    h1 = bf(torch.relu(x @ W1 + b1))
+    h2 = bf(torch.relu(h1 @ W2 + b2))
+    o = h2 @ W3 + b3
+    sc = fl.smax * torch.tanh(o[:, :H] / fl.smax)
+    y = z0[:, b_off:b_off + H].cpu().double() * torch.exp(sc) + o[:, H:]
+    ld = sc.sum(1)
+    got = z1[:, b_off:b_off + H].cpu().double()
+    scale = float(y.abs().max())
+    dz = ((got - y).abs() / scale).flatten()
+    dl = (l1.cpu() - ld).abs() / ld.abs().clamp_min(1.0)
+    q99, med = float(dz.quantile(0.99)), float(dz.median())
+    print(f"y: median {med:.2e} q99 {q99:.2e} max {float(dz.max()):.2e} of {scale:.2f}; "
+          f"log-det q95 {float(dl.quantile(0.95)):.2e} max {float(dl.max()):.2e}")
+    assert torch.equal(z1[:, a_off:a_off + H], z0[:, a_off:a_off + H])
+    assert med <= 1e-6 and q99 <= 1e-5 and float(dz.max()) <= 2e-2
+    assert float(dl.quantile(0.95)) <= 1e-4
+
+
 def test_single_bf16_draws_after_training(tmp_path):
     """A trained flow at A=4, N=8, P=64 (D = 512: the one-launch layers) drawn with single-bf16
     layers: the same unbiased Z/Z_rho as its BF16x3 draws within their errors, and an ESS
