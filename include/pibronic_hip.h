@@ -206,6 +206,12 @@ int pib_gemm_bf16x3_presplit(pib_ctx* ctx, int64_t M, int64_t N, int64_t K,
  * order of pib_flow_layer_fused's second and third products. */
 #define PIB_SPLIT_TRANSPOSE 1
 #define PIB_SPLIT_PERM32 2
+/* PIB_SPLIT_FRAG32 (with TRANSPOSE only; cols a multiple of 32, rows of 16): the transposed
+ * matrix (cols x rows) in the A-operand order of the 32x32x16 bf16 matrix-core product, as
+ * pib_flow_layer_fused_frag reads it: element (n, k) at ((n/32 * rows/16 + k/16) * 64 +
+ * 32 ((k % 16) / 8) + n % 32) * 8 + k % 8; ldo is not used. For any mode lo_dev may be NULL
+ * (only hi is written). */
+#define PIB_SPLIT_FRAG32 4
 int pib_split_bf16(pib_ctx* ctx, int64_t rows, int64_t cols, const float* X_dev, int64_t ldx,
                    int mode, uint16_t* hi_dev, uint16_t* lo_dev, int64_t ldo, void* stream);
 
@@ -252,6 +258,18 @@ int pib_flow_layer_fused(pib_ctx* ctx, int inverse, int64_t B, int D, int hidden
                          const uint16_t* W2_hi_dev, const uint16_t* W2_lo_dev,
                          const uint16_t* W3_hi_dev, const uint16_t* W3_lo_dev, const float* b1_dev,
                          const float* b2_dev, const float* b3_dev, void* stream);
+
+/* The layer of pib_flow_layer_fused with single bf16 products (its NULL lo parts), on weights
+ * split with PIB_SPLIT_TRANSPOSE | PIB_SPLIT_FRAG32 (all three, inputs in natural order; only
+ * the hi parts): each wave streams its weight tiles as contiguous 1 KB blocks. Same network,
+ * coupling and exactness of the log-determinant; the FP32 summation order differs. Supported:
+ * hidden 256, D = 512, B a multiple of 128; weights and z 16-byte aligned. No counterpart in
+ * the reference. */
+int pib_flow_layer_fused_frag(pib_ctx* ctx, int inverse, int64_t B, int D, int hidden, int a_off,
+                              int b_off, float smax, float* z_dev, double* logdet_dev,
+                              const uint16_t* W1_frag_dev, const uint16_t* W2_frag_dev,
+                              const uint16_t* W3_frag_dev, const float* b1_dev, const float* b2_dev,
+                              const float* b3_dev, void* stream);
 
 /* Base distribution of the flow drawn directly in its whitened, permuted coordinates: the
  * harmonic rho is the mixture sum_a w_a N(mu_a, I) there (mu_a (D, double) = the surface's

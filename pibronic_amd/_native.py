@@ -22,7 +22,7 @@ PIB_ERR_UNSUPPORTED = -4
 SET_RHO, SET_VIB, SET_VIB_PLUS, SET_VIB_MINUS = 0, 1, 2, 3
 EVAL_PM, EVAL_NOSCALE, EVAL_WIDE, EVAL_FP32, EVAL_BF16X3 = 1, 2, 4, 8, 16
 BATCH_LANES, BATCH_WIDE, BATCH_SEQUENTIAL = 0, 1, 2
-SPLIT_TRANSPOSE, SPLIT_PERM32 = 1, 2
+SPLIT_TRANSPOSE, SPLIT_PERM32, SPLIT_FRAG32 = 1, 2, 4
 OUT_RHO, OUT_G, OUT_GP, OUT_GM, OUT_LOGS = 0, 1, 2, 3, 4
 NUM_OUT = 5
 (SUM_N, SUM_R, SUM_R2, SUM_ABSR, SUM_NEG, SUM_D1, SUM_D1SQ, SUM_D2, SUM_D2SQ,
@@ -58,6 +58,8 @@ SIGNATURES = {
     "pib_split_bf16": (_c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_int, _vp, _vp, _c_i64, _vp]),
     "pib_flow_layer_fused": (_c_int, [_vp, _c_int, _c_i64, _c_int, _c_int, _c_int, _c_int, ctypes.c_float,
                                       _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pib_flow_layer_fused_frag": (_c_int, [_vp, _c_int, _c_i64, _c_int, _c_int, _c_int, _c_int,
+                                           ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pib_flow_set_basis": (_c_int, [_vp, _c_int, _c_int, _dp, _dp, _dp]),
     "pib_flow_whiten": (_c_int, [_vp, _c_int, _c_i64, _vp, _vp, _vp, _vp]),
     "pib_flow_coupling": (_c_int, [_vp, _c_int, _c_i64, _c_int, _c_int, _c_int, ctypes.c_float,
@@ -244,8 +246,12 @@ class Context:
             _devptr(Bt_lo), int(ldb), _devptr(C), int(ldc), _devptr(O_hi), _devptr(O_lo), int(ldo),
             _devptr(bias), int(bool(relu)), _stream(stream)))
 
-    def split_bf16(self, rows, cols, X, ldx, hi, lo, ldo, transpose=False, stream=None, perm32=False):
-        mode = (SPLIT_TRANSPOSE if transpose else 0) | (SPLIT_PERM32 if perm32 else 0)
+    def split_bf16(self, rows, cols, X, ldx, hi, lo, ldo, transpose=False, stream=None, perm32=False,
+                   frag32=False):
+        """lo may be None (only hi written); frag32: the transposed matrix in the fragment order of
+        pib_flow_layer_fused_frag (with transpose=True)."""
+        mode = (SPLIT_TRANSPOSE if transpose else 0) | (SPLIT_PERM32 if perm32 else 0) | \
+            (SPLIT_FRAG32 if frag32 else 0)
         self._check(self.lib.pib_split_bf16(self.h, int(rows), int(cols), _devptr(X), int(ldx),
                                             mode, _devptr(hi), _devptr(lo), int(ldo), _stream(stream)))
 
@@ -258,6 +264,16 @@ class Context:
             self.h, int(bool(inverse)), int(B), int(D), int(hidden), int(a_off), int(b_off), float(smax),
             _devptr(z), _devptr(logdet), _devptr(w1h), _devptr(w1l), _devptr(w2h), _devptr(w2l),
             _devptr(w3h), _devptr(w3l), _devptr(b1), _devptr(b2), _devptr(b3), _stream(stream)))
+
+    def flow_layer_fused_frag(self, inverse, B, D, hidden, a_off, b_off, smax, z, logdet, w, b1, b2, b3,
+                              stream=None):
+        """Single-bf16 layer on w = (W1, W2, W3) split with transpose=True, frag32=True (hi
+        parts)."""
+        w1, w2, w3 = w
+        self._check(self.lib.pib_flow_layer_fused_frag(
+            self.h, int(bool(inverse)), int(B), int(D), int(hidden), int(a_off), int(b_off), float(smax),
+            _devptr(z), _devptr(logdet), _devptr(w1), _devptr(w2), _devptr(w3), _devptr(b1), _devptr(b2),
+            _devptr(b3), _stream(stream)))
 
     def flow_set_basis(self, U, sigma, m):
         Uh, sh, mh = _host(U), _host(sigma), _host(m)

@@ -603,12 +603,15 @@ int pib_gemm_bf16x3_presplit(pib_ctx* c, int64_t M, int64_t N, int64_t K, const 
 int pib_split_bf16(pib_ctx* c, int64_t rows, int64_t cols, const float* X, int64_t ldx, int mode,
                    uint16_t* hi, uint16_t* lo, int64_t ldo, void* stream) {
   if (!c) return PIB_ERR_ARG;
-  if (mode & ~(PIB_SPLIT_TRANSPOSE | PIB_SPLIT_PERM32))
+  if (mode & ~(PIB_SPLIT_TRANSPOSE | PIB_SPLIT_PERM32 | PIB_SPLIT_FRAG32))
     return fail(c, PIB_ERR_ARG, "split_bf16: unknown mode bit");
   if ((mode & PIB_SPLIT_PERM32) && (!(mode & PIB_SPLIT_TRANSPOSE) || rows % 32))
     return fail(c, PIB_ERR_ARG, "split_bf16: PIB_SPLIT_PERM32 needs PIB_SPLIT_TRANSPOSE and rows a multiple of 32");
+  if ((mode & PIB_SPLIT_FRAG32) && (mode != (PIB_SPLIT_TRANSPOSE | PIB_SPLIT_FRAG32) || cols % 32 || rows % 16))
+    return fail(c, PIB_ERR_ARG, "split_bf16: PIB_SPLIT_FRAG32 needs PIB_SPLIT_TRANSPOSE alone beside it, "
+                                "cols a multiple of 32 and rows of 16");
   const bool tr = (mode & PIB_SPLIT_TRANSPOSE) != 0;
-  if (rows < 1 || cols < 1 || !X || !hi || !lo || ldx < cols || ldo < (tr ? rows : cols))
+  if (rows < 1 || cols < 1 || !X || !hi || ldx < cols || ldo < (tr ? rows : cols))
     return fail(c, PIB_ERR_ARG, "split_bf16: bad size, leading dimension or null pointer");
   PIB_HIP(c, hipSetDevice(c->device));
   hipError_t e = launch_split_bf16(rows, cols, X, ldx, mode, hi, lo, ldo, (hipStream_t)stream);
@@ -638,6 +641,29 @@ int pib_flow_layer_fused(pib_ctx* c, int inverse, int64_t B, int D, int hidden, 
   PIB_HIP(c, hipSetDevice(c->device));
   hipError_t e = launch_flow_layer(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "flow_layer_fused launch");
+  return PIB_OK;
+}
+
+int pib_flow_layer_fused_frag(pib_ctx* c, int inverse, int64_t B, int D, int hidden, int a_off, int b_off,
+                              float smax, float* z, double* logdet, const uint16_t* W1_frag,
+                              const uint16_t* W2_frag, const uint16_t* W3_frag, const float* b1,
+                              const float* b2, const float* b3, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (!z || !logdet || !W1_frag || !W2_frag || !W3_frag || !b1 || !b2 || !b3)
+    return fail(c, PIB_ERR_ARG, "flow_layer_fused_frag: null pointer");
+  if (!frag_layer_supported(B, D, hidden))
+    return fail(c, PIB_ERR_UNSUPPORTED, "flow_layer_fused_frag: needs hidden 256, D = 512 and B a multiple of 128");
+  const int H = D / 2;
+  if (!((a_off == 0 && b_off == H) || (a_off == H && b_off == 0)))
+    return fail(c, PIB_ERR_ARG, "flow_layer_fused_frag: the halves must be (0, D/2) or (D/2, 0)");
+  if (!(smax > 0.f)) return fail(c, PIB_ERR_ARG, "flow_layer_fused_frag: smax must be > 0");
+  if (((uintptr_t)W1_frag | (uintptr_t)W2_frag | (uintptr_t)W3_frag | (uintptr_t)z) % 16)
+    return fail(c, PIB_ERR_ARG, "flow_layer_fused_frag: weights and z must be 16-byte aligned");
+  FusedLayerArgs a{B, D, hidden, a_off, b_off, inverse ? 1 : 0, smax, z, logdet,
+                   W1_frag, nullptr, W2_frag, nullptr, W3_frag, nullptr, b1, b2, b3};
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_flow_layer_frag(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "flow_layer_fused_frag launch");
   return PIB_OK;
 }
 

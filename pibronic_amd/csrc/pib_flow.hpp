@@ -119,6 +119,9 @@ struct FusedLayerArgs {
 };
 bool fused_layer_supported(int64_t B, int D, int hidden, bool lo);
 hipError_t launch_flow_layer(const FusedLayerArgs& a, hipStream_t s);
+// single-bf16 layer on fragment-ordered weights (W*h; the lo pointers unused)
+bool frag_layer_supported(int64_t B, int D, int hidden);
+hipError_t launch_flow_layer_frag(const FusedLayerArgs& a, hipStream_t s);
 hipError_t launch_coupling_inverse_backward(const CouplingArgs& a, hipStream_t s);
 
 struct LogRhoArgs {

@@ -1,7 +1,8 @@
 // One affine-coupling layer in one kernel (flow sampling and density directions). Two kernels:
-// k_flow_layer (below: BF16x3 at D = 512, single bf16 at D = 1024), activations resident in
-// registers and weights staged through LDS; and k_flow_layer_ws (further down: single bf16 at
-// D = 512), weights streamed into registers per wave and activations staged in LDS.
+// k_flow_layer (below: pib_flow_layer_fused, BF16x3 or single bf16), activations resident in
+// registers and weights staged through LDS; and k_flow_layer_ws (further down:
+// pib_flow_layer_fused_frag, single bf16 at D = 512), fragment-ordered weights streamed into
+// registers per wave and activations staged in LDS.
 //
 // The three products of the layer's MLP (H -> h -> h -> 2H, ReLU, ReLU) are computed
 // transposed, Y^T = W^T X^T, on the bf16 matrix cores as BF16x3 (hi/lo splits, FP32
@@ -355,26 +356,27 @@ __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------
-// k_flow_layer_ws: the single-bf16 layer at D = 512, weights streamed per wave
-// (v_mfma_f32_32x32x16_bf16: lane l supplies A[l%32][8(l/32)+e] and B[8(l/32)+e][l%32] and
-// holds D[8(r/4)+4(l/32)+r%4][l%32] in register r). Work-group: 8 waves and SB = 128 samples.
-// The layer is a sequence of weight blocks of 16 inputs: the first product's, the second's,
-// then those of H/256 passes of the last. In the hidden products wave w owns output rows
-// 32w..32w+31 (one M tile); in pass p of the last it owns the scale rows 32q.. and shift rows
-// H + 32q.. of features 32q..32q+31, q = (H/256) w + p (two M tiles). Its weight blocks (16
-// bytes per lane and M tile) stream from L2 into a register ring kRing - 1 blocks ahead of
-// use; the biases sit in LDS and the coupling's z is loaded ahead of its epilogue, because
-// vector loads complete in order: a load issued behind the ring and waited for at once would
-// wait for the whole ring. The activations of all SB samples are staged in LDS as bf16 rows --
-// the conditioning half (natural order: W1's inputs), then h1, then h2 over the conditioning
-// half, each element at the position its product's permuted weights expect -- and every wave
-// reads them as B operands one block ahead, each 16-byte read feeding one 32x32x16 MFMA (two
-// in the last product) for 32 samples. Against k_flow_layer (each wave reading every staged
-// weight tile for its own 16 samples, a barrier per weight stage) this halves the LDS bytes
-// per flop, removes its bank conflicts, takes four barriers per layer instead of 32, and
-// measured 71.5 against 85.3 us per layer of 65536 samples. The 16-register MFMA results
-// are early-clobber for the compiler, so a destination never partly overlaps its accumulator
-// input.
+// k_flow_layer_ws: the single-bf16 layer at D = 512 on fragment-ordered weights
+// (pib_flow_layer_fused_frag), weights streamed per wave (v_mfma_f32_32x32x16_bf16: lane l
+// supplies A[l%32][8(l/32)+e] and B[8(l/32)+e][l%32] and holds D[8(r/4)+4(l/32)+r%4][l%32] in
+// register r). Work-group: 8 waves and SB = 128 samples. The layer is a sequence of weight
+// blocks of 16 inputs: the first product's, the second's, then those of H/256 passes of the
+// last. In the hidden products wave w owns output rows 32w..32w+31 (one M tile); in pass p of
+// the last it owns the scale rows 32q.. and shift rows H + 32q.. of features 32q..32q+31, q =
+// (H/256) w + p (two M tiles). The weights are stored in the MFMA's A-operand order, so a
+// wave's block of a tile is one contiguous 1 KB load (with rows of W^T, each lane's 16 bytes
+// would come from a different 32-byte piece of 32 rows, and the address and L1 units, not the
+// matrix cores, would bound the layer). The blocks stream from L2 into a register ring
+// kRing - 1 blocks ahead of use; the biases sit in LDS and the coupling's z is loaded ahead of
+// its epilogue, because vector loads complete in order: a load issued behind the ring and
+// waited for at once would wait for the whole ring. The activations of all SB samples are
+// staged in LDS as bf16 rows in natural order -- the conditioning half, then h1, then h2 over
+// the conditioning half -- and every wave reads them as B operands one block ahead, each
+// 16-byte read feeding one 32x32x16 MFMA (two in the last product) for 32 samples. Against
+// k_flow_layer (each wave reading every staged weight tile for its own 16 samples, a barrier
+// per weight stage) this halves the LDS bytes per flop, removes its bank conflicts and takes
+// four barriers per layer instead of 32. The 16-register MFMA results are early-clobber for
+// the compiler, so a destination never partly overlaps its accumulator input.
 // ---------------------------------------------------------------------------------------
 typedef float f32x16f __attribute__((ext_vector_type(16)));
 constexpr int kRing = 7;  // weight blocks in registers: six in flight ahead of the one in use
@@ -392,10 +394,6 @@ struct WsGeo {
   static constexpr size_t lds_bytes =
       (size_t)(elemsA + elemsB) * 2 + (size_t)8 * SB * sizeof(double) + (size_t)nbias * sizeof(float);
 };
-
-// position of row o (0..31) of a 32-row output tile among the next product's inputs 0..31
-// (PIB_SPLIT_PERM32 order) for the four rows 8j + 4h + 0..3 held in registers 4j..4j+3
-__device__ __forceinline__ int perm_pos(int j, int h) { return 16 * (j & 1) + 8 * h + 4 * (j >> 1); }
 
 template <int H, int SB>
 __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
@@ -416,18 +414,18 @@ __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
   const int D = a.D;
 
   // weight block b of this wave: the A operands of its M tiles (the second in the last
-  // product only)
+  // product only), 1 KB contiguous per tile in the fragment order of pib_split_bf16's
+  // PIB_SPLIT_FRAG32: tile m, block c of a K-input matrix at ((m K/16 + c) 64 + lane) 8
   uvec4f ring[kRing][2];
   auto fetch_block = [&](int b, uvec4f(&dst)[2]) {
     if (b < KB1 + KB2) {
       const bool first = b < KB1;
-      const int K = first ? H : HID, c = first ? b : b - KB1;
-      dst[0] = *(const uvec4f*)((first ? a.W1h : a.W2h) + (int64_t)(32 * w + l32) * K + 16 * c + 8 * h);
+      const int nb = first ? KB1 : KB2, c = first ? b : b - KB1;
+      dst[0] = *(const uvec4f*)((first ? a.W1h : a.W2h) + ((int64_t)(w * nb + c) * 64 + lane) * 8);
     } else {
       const int t = b - KB1 - KB2, q = G::P3 * w + t / KB2, c = t % KB2;
-      const unsigned short* p = a.W3h + (int64_t)(32 * q + l32) * HID + 16 * c + 8 * h;
-      dst[0] = *(const uvec4f*)p;
-      dst[1] = *(const uvec4f*)(p + (int64_t)H * HID);
+      dst[0] = *(const uvec4f*)(a.W3h + ((int64_t)(q * KB2 + c) * 64 + lane) * 8);
+      dst[1] = *(const uvec4f*)(a.W3h + ((int64_t)((H / 32 + q) * KB2 + c) * 64 + lane) * 8);
     }
   };
 #pragma unroll
@@ -506,7 +504,7 @@ __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
     }
     if (c != last) continue;
     if (step < 2) {
-      // bias, ReLU, bf16 into the next product's input rows
+      // bias, ReLU, bf16 into the next product's input rows (natural order)
       const float* bias = step == 0 ? bl1 : bl2;
       unsigned short* dst = step == 0 ? bufB : bufA;
       const int ldd = step == 0 ? G::ldh : G::ldx;
@@ -523,7 +521,7 @@ __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
           uint2 pk;
           pk.x = pack_bf16(hv[0], hv[1]);
           pk.y = pack_bf16(hv[2], hv[3]);
-          *(uint2*)(dst + (32 * n + l32) * ldd + 32 * w + perm_pos(j, h)) = pk;
+          *(uint2*)(dst + (32 * n + l32) * ldd + 32 * w + 8 * j + 4 * h) = pk;
         }
       }
       __syncthreads();
@@ -609,9 +607,19 @@ hipError_t launch_flow_layer(const FusedLayerArgs& a, hipStream_t s) {
   if (!fused_layer_supported(a.B, a.D, a.hidden, lo)) return hipErrorInvalidValue;
   if ((a.a_off != 0 && a.a_off != a.D / 2) || (a.b_off != 0 && a.b_off != a.D / 2) || a.a_off == a.b_off)
     return hipErrorInvalidValue;
-  if (!lo && a.D == 512) return launch_ws<256, 128>(a, s);
   if (a.D == 1024) return launch_hh<512, 256, false>(a, s);
-  return launch_hh<256, 256, true>(a, s);
+  return lo ? launch_hh<256, 256, true>(a, s) : launch_hh<256, 256, false>(a, s);
+}
+
+bool frag_layer_supported(int64_t B, int D, int hidden) {
+  return B > 0 && B % 128 == 0 && hidden == 256 && D == 512;
+}
+
+hipError_t launch_flow_layer_frag(const FusedLayerArgs& a, hipStream_t s) {
+  if (!frag_layer_supported(a.B, a.D, a.hidden)) return hipErrorInvalidValue;
+  if ((a.a_off != 0 && a.a_off != a.D / 2) || (a.b_off != 0 && a.b_off != a.D / 2) || a.a_off == a.b_off)
+    return hipErrorInvalidValue;
+  return launch_ws<256, 128>(a, s);
 }
 
 }  // namespace pib

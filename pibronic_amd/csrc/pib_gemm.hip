@@ -506,16 +506,26 @@ __device__ __forceinline__ int64_t perm32(int64_t k) {
   return b + 8 * ((w & 15) >> 2) + 4 * (w >> 4) + (w & 3);
 }
 
+// Offset of element (n, k) of the transposed matrix (N x K, N a multiple of 32, K of 16) in
+// the A-operand fragment order of v_mfma_f32_32x32x16_bf16: tile m = n / 32 of 32 rows, block
+// b = k / 16 of 16 inputs; lane 32((k % 16) / 8) + n % 32 holds elements k % 8 = 0..7, 16
+// contiguous bytes, and the 64 lanes of a (tile, block) pair 1 KB.
+__device__ __forceinline__ int64_t frag32(int64_t n, int64_t k, int64_t K) {
+  const int64_t tile = n >> 5, blk = k >> 4;
+  const int lane = 32 * (int)((k & 15) >> 3) + (int)(n & 31);
+  return ((tile * (K >> 4) + blk) * 64 + lane) * 8 + (k & 7);
+}
+
 __global__ void k_split_bf16(int64_t rows, int64_t cols, const float* X, int64_t ldx, int mode,
                              unsigned short* hi, unsigned short* lo, int64_t ldo) {
-  const bool transpose = (mode & 1) != 0, perm = (mode & 2) != 0;
+  const bool transpose = (mode & 1) != 0, perm = (mode & 2) != 0, frag = (mode & 4) != 0;
   for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += (int64_t)gridDim.x * blockDim.x) {
       const float x = X[r * ldx + c];
       const unsigned short h = f2bf(x);
-      const int64_t o = transpose ? c * ldo + (perm ? perm32(r) : r) : r * ldo + c;
+      const int64_t o = frag ? frag32(c, r, rows) : transpose ? c * ldo + (perm ? perm32(r) : r) : r * ldo + c;
       hi[o] = h;
-      lo[o] = f2bf(x - bf2f(h));
+      if (lo) lo[o] = f2bf(x - bf2f(h));
     }
   }
 }
@@ -535,7 +545,9 @@ hipError_t launch_gemm_bf16x3_pre(const GemmPreArgs& g, hipStream_t s) {
 
 hipError_t launch_split_bf16(int64_t rows, int64_t cols, const float* X, int64_t ldx, int transpose,
                              unsigned short* hi, unsigned short* lo, int64_t ldo, hipStream_t s) {
-  if ((transpose & ~3) || ((transpose & 2) && (!(transpose & 1) || rows % 32))) return hipErrorInvalidValue;
+  if ((transpose & ~7) || ((transpose & 2) && (!(transpose & 1) || rows % 32)) ||
+      ((transpose & 4) && (transpose != 5 || cols % 32 || rows % 16)))
+    return hipErrorInvalidValue;
   const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((cols + 255) / 256, 64)),
                   (unsigned)std::max<int64_t>(1, std::min<int64_t>(rows, 65535)));
   hipLaunchKernelGGL(k_split_bf16, grid, dim3(256), 0, s, rows, cols, X, ldx, transpose, hi, lo, ldo);

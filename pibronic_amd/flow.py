@@ -85,6 +85,8 @@ class Flow:
         self.draw_precision = "bf16x3"
         self._split_version = -1   # parameter version of the pre-split bf16 weights
         self._fsplit_version = -1  # parameter version of the fused layer's permuted weights
+        self.use_frag = True       # single-bf16 layers at D = 512 on fragment-ordered weights
+        self._frag_version = -1    # parameter version of those weights
 
     def _set_base(self, rho, sigma, m):
         """rho in the flow's coordinates Z = (R0 - m) U / sigma: the mixture over surfaces a
@@ -223,10 +225,35 @@ class Flow:
         return (self.use_fused and self.hidden == 256 and B % 128 == 0
                 and (self.H == 256 or (single_bf16 and self.H == 512)))
 
+    def _split_weights_frag(self, stream):
+        """bf16 weights (hi parts) in the fragment order of pib_flow_layer_fused_frag, inputs in
+        natural order."""
+        import torch
+        if self._frag_version == self._version:
+            return
+        H, h, D = self.H, self.hidden, self.D
+        self._wfrag = []
+        for l in range(self.layers):
+            mats = []
+            for name, (K, Nn) in (("W1", (H, h)), ("W2", (h, h)), ("W3", (h, D))):
+                f = self._buf(f"{name}f{l}", (Nn * K,), torch.int16)
+                self.ctx.split_bf16(K, Nn, self.p(l, name), Nn, f, None, K, transpose=True, frag32=True,
+                                    stream=stream)
+                mats.append(f)
+            self._wfrag.append(mats)
+        self._frag_version = self._version
+
     def _layer_fused(self, inverse, layer, zp, B, logdet, stream, single_bf16=False):
         """The whole coupling layer in one launch (pib_flow_layer_fused); single_bf16: one bf16
-        product per weight block (the hi parts) instead of BF16x3."""
+        product per weight block (the hi parts) instead of BF16x3, at D = 512 (with use_frag) on
+        fragment-ordered weights (pib_flow_layer_fused_frag)."""
         a_off, b_off = self.halves(layer)
+        if single_bf16 and self.use_frag and self.D == 512:
+            self._split_weights_frag(stream)
+            self.ctx.flow_layer_fused_frag(inverse, B, self.D, self.hidden, a_off, b_off, self.smax, zp, logdet,
+                                           self._wfrag[layer], self.p(layer, "b1"), self.p(layer, "b2"),
+                                           self.p(layer, "b3"), stream)
+            return
         self._split_weights_fused(stream)
         w = self._wfused[layer]
         if single_bf16:

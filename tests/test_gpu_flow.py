@@ -408,9 +408,10 @@ def test_fused_layer_matches_layer_by_layer(tmp_path, N, P):
     assert dR <= 5e-5 * scale and dld <= 1e-2
 
 
-@pytest.mark.parametrize("N", [8, 16])
-def test_single_bf16_layer_is_an_exact_coupling(tmp_path, N):
-    """One-launch layers with single bf16 products (lo parts null): the network differs from
+@pytest.mark.parametrize("N,frag", [(8, True), (8, False), (16, False)])
+def test_single_bf16_layer_is_an_exact_coupling(tmp_path, N, frag):
+    """One-launch layers with single bf16 products (lo parts null; at D = 512 with frag the
+    fragment-ordered weights of pib_flow_layer_fused_frag): the network differs from
     the BF16x3 one by bf16 rounding, but the layer is still an exact coupling. The conditioning
     half passes bit for bit; the inverse layer (the same scales, computed from the same half)
     returns the input to FP32 rounding; and the forward and inverse log-determinants cancel
@@ -422,6 +423,7 @@ def test_single_bf16_layer_is_an_exact_coupling(tmp_path, N):
     data = make_data(FS, 300.0, 64, 64, 64)
     fl = flow.Flow(data, layers=2, hidden=256, gemm="bf16x3")
     fl.set_params(random_params(ref_for(data, 2, 256, 2.0), 5, 0.05))
+    fl.use_frag = frag
     B, D = 256, fl.D
     fl._split_weights(None)
     z0 = torch.randn((B, D), generator=torch.Generator().manual_seed(1)).to(DEV)
@@ -453,9 +455,10 @@ def test_single_bf16_layer_is_an_exact_coupling(tmp_path, N):
                                 ((w[0][0], None), w[1], w[2]), fl.p(0, "b1"), fl.p(0, "b2"), fl.p(0, "b3"))
 
 
-@pytest.mark.parametrize("layer", [0, 1])
-def test_single_bf16_layer_matches_restatement(tmp_path, layer):
-    """The one-launch single-bf16 layer at D = 512 against the network it computes, restated in
+@pytest.mark.parametrize("layer,frag", [(0, True), (1, True), (0, False)])
+def test_single_bf16_layer_matches_restatement(tmp_path, layer, frag):
+    """The one-launch single-bf16 layer at D = 512 (both kernels: fragment-ordered weights with
+    frag, pib_flow_layer_fused with null lo parts without) against the network it computes, restated in
     float64 torch: the conditioning half and the weights rounded to bf16 (round to nearest
     even), h1 = bf16(relu(x W1 + b1)), h2 = bf16(relu(h1 W2 + b2)), (raw, t) = h2 W3 + b3,
     s = smax tanh(raw / smax), y = z e^s + t and log-det sum s. The kernel accumulates in FP32,
@@ -463,11 +466,12 @@ def test_single_bf16_layer_matches_restatement(tmp_path, layer):
     other way; such flips are rare and each moves y by about a bf16 ulp of one hidden unit.
     Bounds: 99% of the y elements within 1e-5 of the largest |y| (median within 1e-6), all
     within 2e-2; 95% of the log-dets within 1e-4 relative (or absolute below 1)."""
-    _announce(f"single-bf16 layer {layer} vs float64 restatement, D=512")
+    _announce(f"single-bf16 layer {layer} (frag {frag}) vs float64 restatement, D=512")
     FS = stage_dict_models(tmp_path, synthetic.generate(2, 8, quadratic=False, seed=2))
     data = make_data(FS, 300.0, 64, 64, 64)
     fl = flow.Flow(data, layers=2, hidden=256, gemm="bf16x3")
     fl.set_params(random_params(ref_for(data, 2, 256, 2.0), 7, 0.05))
+    fl.use_frag = frag
     B, D, H = 2048, fl.D, fl.H
     assert D == 512
     fl._split_weights(None)
@@ -499,6 +503,29 @@ def test_single_bf16_layer_matches_restatement(tmp_path, layer):
     assert torch.equal(z1[:, a_off:a_off + H], z0[:, a_off:a_off + H])
     assert med <= 1e-6 and q99 <= 1e-5 and float(dz.max()) <= 2e-2
     assert float(dl.quantile(0.95)) <= 1e-4
+
+
+def test_split_frag32_layout(tmp_path):
+    """pib_split_bf16 with PIB_SPLIT_FRAG32 against the layout restated in NumPy: element (n, k)
+    of the transposed matrix (RNE bf16) at ((n/32 * K/16 + k/16) * 64 + 32 ((k % 16) / 8) +
+    n % 32) * 8 + k % 8, for a K x N input (K inputs, N outputs); bit for bit. Also: the lo
+    output may be omitted, and shapes outside the layout are refused."""
+    _announce("split_bf16 frag32 layout")
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, 2, quadratic=False, seed=0))
+    ctx = make_data(FS, 300.0, 64, 10, 10).context()
+    K, N = 48, 96
+    X = torch.randn((K, N), generator=torch.Generator().manual_seed(4), dtype=torch.float32)
+    out = torch.empty(N * K, dtype=torch.int16, device=DEV)
+    ctx.split_bf16(K, N, X.to(DEV), N, out, None, K, transpose=True, frag32=True)
+    torch.cuda.synchronize()
+    bits = X.to(torch.bfloat16).view(torch.int16).numpy()  # (K, N), round to nearest even
+    n, k = np.meshgrid(np.arange(N), np.arange(K), indexing="xy")  # k rows, n columns
+    off = ((n // 32 * (K // 16) + k // 16) * 64 + 32 * ((k % 16) // 8) + n % 32) * 8 + k % 8
+    want = np.empty(N * K, dtype=np.int16)
+    want[off.ravel()] = bits.ravel()
+    assert np.array_equal(out.cpu().numpy(), want)
+    with pytest.raises(_native.NativeError):
+        ctx.split_bf16(K, 80, X.to(DEV), N, out, None, K, transpose=True, frag32=True)  # N % 32
 
 
 def test_single_bf16_draws_after_training(tmp_path):
