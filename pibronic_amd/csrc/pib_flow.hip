@@ -230,78 +230,100 @@ __device__ __forceinline__ double group16_sum(double v) {
   return v;
 }
 
+// The displacements mu (A_rho x D doubles) are read for every feature of every sample; with
+// MU_LDS (when they fit in 64 KB) each work-group stages them in LDS once and loops over groups
+// of sixteen samples, so that those reads bypass the vector-memory address units (which the
+// per-feature global loads otherwise saturate).
+template <bool MU_LDS>
 __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
+  extern __shared__ double mu_lds[];
+  const double* __restrict__ mu = a.mu;
+  if constexpr (MU_LDS) {
+    for (int i = threadIdx.x; i < a.A_rho * a.D; i += 256) mu_lds[i] = a.mu[i];
+    __syncthreads();
+    mu = mu_lds;
+  }
   const int sl = threadIdx.x & (kBaseLanes - 1);
-  const int64_t s = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kBaseLanes;
-  const bool live = s < a.B;  // uniform over the sixteen lanes of a sample
-  const int64_t gs = a.offset + (live ? s : 0);
-  int src = 0;
-  if (a.A_rho > 1) {
-    const u32x4 r = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), 0u, 1u}, a.key0, a.key1);
-    const double u = u53(r.x, r.y);
-    while (src < a.A_rho - 1 && u >= a.cdf[src]) ++src;
-  }
-  const double* __restrict__ mus = a.mu + (int64_t)src * a.D;
-  float* zr = a.z + (live ? s : 0) * a.D;
-  double zz = 0.0, dot[kBaseMaxA];
+  const int64_t groups = (a.B + 256 / kBaseLanes - 1) / (256 / kBaseLanes);
+  for (int64_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {  // uniform over the block
+    const int64_t s = grp * (256 / kBaseLanes) + threadIdx.x / kBaseLanes;
+    const bool live = s < a.B;  // uniform over the sixteen lanes of a sample
+    const int64_t gs = a.offset + (live ? s : 0);
+    int src = 0;
+    if (a.A_rho > 1) {
+      const u32x4 r = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), 0u, 1u}, a.key0, a.key1);
+      const double u = u53(r.x, r.y);
+      while (src < a.A_rho - 1 && u >= a.cdf[src]) ++src;
+    }
+    const double* __restrict__ mus = mu + (int64_t)src * a.D;
+    float* zr = a.z + (live ? s : 0) * a.D;
+    double zz = 0.0, dot[kBaseMaxA];
 #pragma unroll
-  for (int b = 0; b < kBaseMaxA; ++b) dot[b] = 0.0;
-  if (live) {
-    // four features (two Box-Muller pairs) per Philox call: 32-bit uniforms, the radius one
-    // refined to 64 bits from a second call when it is below 2^-20, so that the tails reach
-    // ~9.4 sigma (a 32-bit radius uniform alone would stop at 6.7 sigma); the transform in
-    // FP32 with the hardware log2, sqrt and sin/cos (z is float)
-    for (int m = sl; 4 * m < a.D; m += kBaseLanes) {
-      const u32x4 r = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), (uint32_t)m, 2u}, a.key0, a.key1);
-      float l2a = __log2f(((float)r.x + 0.5f) * 0x1.0p-32f);
-      float l2b = __log2f(((float)r.z + 0.5f) * 0x1.0p-32f);
-      if (r.x < (1u << 12) || r.z < (1u << 12)) {  // rare: P = 2^-19 per call
-        const u32x4 t = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), (uint32_t)m, 3u}, a.key0, a.key1);
-        if (r.x < (1u << 12)) l2a = (float)log2(((double)r.x * 0x1.0p32 + (double)t.x + 0.5) * 0x1.0p-64);
-        if (r.z < (1u << 12)) l2b = (float)log2(((double)r.z * 0x1.0p32 + (double)t.z + 0.5) * 0x1.0p-64);
-      }
-      const float ra = __fsqrt_rn(-1.3862943611198906f * l2a), rb = __fsqrt_rn(-1.3862943611198906f * l2b);
-      const float ua = ((float)r.y + 0.5f) * 0x1.0p-32f, ub = ((float)r.w + 0.5f) * 0x1.0p-32f;
-      // v_cos_f32 / v_sin_f32 take their argument in revolutions
-      const float zf[4] = {ra * __builtin_amdgcn_cosf(ua), ra * __builtin_amdgcn_sinf(ua),
-                           rb * __builtin_amdgcn_cosf(ub), rb * __builtin_amdgcn_sinf(ub)};
-      float zo[4];
+    for (int b = 0; b < kBaseMaxA; ++b) dot[b] = 0.0;
+    if (live) {
+      // four features (two Box-Muller pairs) per Philox call: 32-bit uniforms, the radius one
+      // refined to 64 bits from a second call when it is below 2^-20, so that the tails reach
+      // ~9.4 sigma (a 32-bit radius uniform alone would stop at 6.7 sigma); the transform in
+      // FP32 with the hardware log2, sqrt and sin/cos (z is float)
+      for (int m = sl; 4 * m < a.D; m += kBaseLanes) {
+        const u32x4 r = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), (uint32_t)m, 2u}, a.key0, a.key1);
+        float l2a = __log2f(((float)r.x + 0.5f) * 0x1.0p-32f);
+        float l2b = __log2f(((float)r.z + 0.5f) * 0x1.0p-32f);
+        if (r.x < (1u << 12) || r.z < (1u << 12)) {  // rare: P = 2^-19 per call
+          const u32x4 t = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), (uint32_t)m, 3u}, a.key0, a.key1);
+          if (r.x < (1u << 12)) l2a = (float)log2(((double)r.x * 0x1.0p32 + (double)t.x + 0.5) * 0x1.0p-64);
+          if (r.z < (1u << 12)) l2b = (float)log2(((double)r.z * 0x1.0p32 + (double)t.z + 0.5) * 0x1.0p-64);
+        }
+        const float ra = __fsqrt_rn(-1.3862943611198906f * l2a), rb = __fsqrt_rn(-1.3862943611198906f * l2b);
+        const float ua = ((float)r.y + 0.5f) * 0x1.0p-32f, ub = ((float)r.w + 0.5f) * 0x1.0p-32f;
+        // v_cos_f32 / v_sin_f32 take their argument in revolutions
+        const float zf[4] = {ra * __builtin_amdgcn_cosf(ua), ra * __builtin_amdgcn_sinf(ua),
+                             rb * __builtin_amdgcn_cosf(ub), rb * __builtin_amdgcn_sinf(ub)};
+        float zo[4];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) zo[e] = (float)((double)zf[e] + mus[4 * m + e]);
-      *(float4*)(zr + 4 * m) = float4{zo[0], zo[1], zo[2], zo[3]};
+        for (int e = 0; e < 4; ++e) zo[e] = (float)((double)zf[e] + mus[4 * m + e]);
+        *(float4*)(zr + 4 * m) = float4{zo[0], zo[1], zo[2], zo[3]};
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const double d = zo[e];
-        zz = fma(d, d, zz);
+        for (int e = 0; e < 4; ++e) {
+          const double d = zo[e];
+          zz = fma(d, d, zz);
 #pragma unroll
-        for (int b = 0; b < kBaseMaxA; ++b)
-          if (b < a.A_rho) dot[b] = fma(d, a.mu[(int64_t)b * a.D + 4 * m + e], dot[b]);
+          for (int b = 0; b < kBaseMaxA; ++b)
+            if (b < a.A_rho) dot[b] = fma(d, mu[(int64_t)b * a.D + 4 * m + e], dot[b]);
+        }
       }
     }
-  }
-  zz = group16_sum(zz);
-  double mx = -INFINITY, t[kBaseMaxA];
+    zz = group16_sum(zz);
+    double mx = -INFINITY, t[kBaseMaxA];
 #pragma unroll
-  for (int b = 0; b < kBaseMaxA; ++b) {
-    if (b < a.A_rho) {
-      t[b] = a.c[b] + group16_sum(dot[b]);
-      mx = fmax(mx, t[b]);
+    for (int b = 0; b < kBaseMaxA; ++b) {
+      if (b < a.A_rho) {
+        t[b] = a.c[b] + group16_sum(dot[b]);
+        mx = fmax(mx, t[b]);
+      }
     }
-  }
-  if (live && sl == 0) {
-    double acc = 0.0;
+    if (live && sl == 0) {
+      double acc = 0.0;
 #pragma unroll
-    for (int b = 0; b < kBaseMaxA; ++b)
-      if (b < a.A_rho) acc += exp(t[b] - mx);
-    a.logrho[s] = mx + log(acc) - 0.5 * zz;
+      for (int b = 0; b < kBaseMaxA; ++b)
+        if (b < a.A_rho) acc += exp(t[b] - mx);
+      a.logrho[s] = mx + log(acc) - 0.5 * zz;
+    }
   }
 }
 
 hipError_t launch_flow_base(const BaseArgs& a, hipStream_t s) {
   if (a.B <= 0 || a.D <= 0 || (a.D & 3) || a.A_rho < 1 || a.A_rho > kBaseMaxA) return hipErrorInvalidValue;
-  const int64_t grid = (a.B + 256 / kBaseLanes - 1) / (256 / kBaseLanes);
-  if (grid > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_flow_base, dim3((unsigned)grid), dim3(256), 0, s, a);
+  const int64_t groups = (a.B + 256 / kBaseLanes - 1) / (256 / kBaseLanes);
+  const size_t mu_bytes = (size_t)a.A_rho * a.D * sizeof(double);
+  if (mu_bytes <= 65536) {
+    // a few work-groups per CU, each staging mu once
+    const int64_t grid = std::min<int64_t>(groups, 2048);
+    hipLaunchKernelGGL(k_flow_base<true>, dim3((unsigned)grid), dim3(256), mu_bytes, s, a);
+  } else {
+    if (groups > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_flow_base<false>, dim3((unsigned)groups), dim3(256), 0, s, a);
+  }
   return hipGetLastError();
 }
 
