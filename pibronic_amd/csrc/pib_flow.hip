@@ -124,26 +124,46 @@ __global__ void __launch_bounds__(256) k_whiten(BasisArgs b, const void* in, voi
   for (int64_t rw = (int64_t)blockIdx.x * 4 + wave; rw * 16 < rows; rw += (int64_t)gridDim.x * 4) {
     const int64_t gr0 = rw * 16, s0 = gr0 / N;
     const int j0 = (int)(gr0 - s0 * N);
-    int64_t s = s0;
-    int j = j0;
-    for (int i = 0; i < 16; ++i) {
-      const int64_t gr = gr0 + i;
+    // all 16 rows' loads issued before any is stored (a load waited for at once would hold
+    // the wave for a memory round trip per row)
+    constexpr int NO = P / 64;
+    float xv[16][NO];
+    {
+      int64_t s = s0;
+      int j = j0;
 #pragma unroll
-      for (int o = lane; o < P; o += 64) {
-        if (MODE == FLOW_WHITEN) {
-          Xs[i * L::ldx + o] = (float)(((const double*)in)[gr * P + o] - b.m[j]);
-        } else if (MODE == FLOW_GRAD) {
-          Xs[i * L::ldx + o] = ((const float*)in)[gr * P + o];
-        } else {  // o runs over (half h, kk); k = 2 kk + h
-          const int h = o / Ph, kk = o - h * Ph, k = 2 * kk + h;
-          Xs[i * L::ldx + k] = ((const float*)in)[s * NP + h * H + j * Ph + kk] * b.sigma[j * P + k];
+      for (int i = 0; i < 16; ++i) {
+        const int64_t gr = gr0 + i;
+#pragma unroll
+        for (int t = 0; t < NO; ++t) {
+          const int o = lane + 64 * t;
+          if (MODE == FLOW_WHITEN) {
+            xv[i][t] = (float)(((const double*)in)[gr * P + o] - b.m[j]);
+          } else if (MODE == FLOW_GRAD) {
+            xv[i][t] = ((const float*)in)[gr * P + o];
+          } else {  // o runs over (half h, kk); k = 2 kk + h
+            const int h = o / Ph, kk = o - h * Ph, k = 2 * kk + h;
+            xv[i][t] = ((const float*)in)[s * NP + h * H + j * Ph + kk] * b.sigma[j * P + k];
+          }
+        }
+        if (++j == N) {
+          j = 0;
+          ++s;
         }
       }
-      if (++j == N) {
-        j = 0;
-        ++s;
-      }
     }
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+#pragma unroll
+      for (int t = 0; t < NO; ++t) {
+        const int o = lane + 64 * t;
+        if (inv) {
+          const int h = o / Ph, kk = o - h * Ph;
+          Xs[i * L::ldx + 2 * kk + h] = xv[i][t];
+        } else {
+          Xs[i * L::ldx + o] = xv[i][t];
+        }
+      }
     wave_lds_sync();
     // one column tile at a time, each a single in-place accumulator chain written to Ys at
     // once (keeping all P/16 tiles live let the compiler rename accumulators between unrolled
@@ -161,8 +181,8 @@ __global__ void __launch_bounds__(256) k_whiten(BasisArgs b, const void* in, voi
       for (int r = 0; r < 4; ++r) Ys[(4 * (lane >> 4) + r) * L::ldx + 16 * c + (lane & 15)] = acc[r];
     }
     wave_lds_sync();
-    s = s0;
-    j = j0;
+    int64_t s = s0;
+    int j = j0;
     for (int i = 0; i < 16; ++i) {
       const int64_t gr = gr0 + i;
 #pragma unroll
