@@ -379,7 +379,6 @@ __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
 // the compiler, so a destination never partly overlaps its accumulator input.
 // ---------------------------------------------------------------------------------------
 typedef float f32x16f __attribute__((ext_vector_type(16)));
-constexpr int kRing = 7;  // weight blocks in registers: six in flight ahead of the one in use
 
 template <int H, int SB>
 struct WsGeo {
@@ -388,7 +387,12 @@ struct WsGeo {
   static constexpr int P3 = H / 256;  // passes of the last product per wave
   static constexpr int KB1 = H / 16, KB2 = HID / 16;
   static constexpr int NBLK = KB1 + KB2 + P3 * KB2;  // weight blocks per wave and layer
-  static constexpr int ldx = H + kPadF, ldh = HID + kPadF;
+  // weight blocks in registers: kRing - 1 in flight ahead of the one in use (fewer at
+  // D = 1024, whose longer unrolled first product needs the registers)
+  static constexpr int kRing = H == 256 ? 7 : 5;
+  static constexpr int XH = 256;      // conditioning inputs staged at a time (H / XH stagings)
+  static constexpr int NXH = H / XH;
+  static constexpr int ldx = XH + kPadF, ldh = HID + kPadF;
   static constexpr int elemsA = SB * ldx, elemsB = SB * ldh;
   static constexpr int nbias = 2 * HID + 2 * H;  // b1, b2, b3
   static constexpr size_t lds_bytes =
@@ -398,7 +402,7 @@ struct WsGeo {
 template <int H, int SB>
 __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
   using G = WsGeo<H, SB>;
-  constexpr int NT = G::NT, KB1 = G::KB1, KB2 = G::KB2, HID = G::HID;
+  constexpr int NT = G::NT, KB1 = G::KB1, KB2 = G::KB2, HID = G::HID, kRing = G::kRing;
   extern __shared__ __align__(16) unsigned short lds[];
   unsigned short* bufA = lds;              // conditioning half, later h2
   unsigned short* bufB = lds + G::elemsA;  // h1
@@ -432,16 +436,20 @@ __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
   for (int b = 0; b < kRing - 1; ++b) fetch_block(b, ring[b]);
 
   for (int i = tid; i < G::nbias; i += 512) bl1[i] = i < HID ? a.b1[i] : (i < 2 * HID ? a.b2[i - HID] : a.b3[i - 2 * HID]);
-  // the conditioning half of the SB samples -> bf16 rows of bufA
-#pragma unroll
-  for (int i = 0; i < SB * H / 4 / 512; ++i) {
-    const int u = tid + 512 * i, row = u / (H / 4), c4 = u - row * (H / 4);
-    const float4 v = *(const float4*)(a.z + (s0 + row) * D + a.a_off + 4 * c4);
-    uint2 pk;
-    pk.x = pack_bf16(v.x, v.y);
-    pk.y = pack_bf16(v.z, v.w);
-    *(uint2*)(bufA + row * G::ldx + 4 * c4) = pk;
-  }
+  // inputs XH x part.. of the conditioning half of the SB samples -> bf16 rows of bufA (at
+  // D = 1024 in two parts, the second staged halfway through the first product)
+  auto stage_x = [&](int part) {
+#pragma unroll 8
+    for (int i = 0; i < SB * G::XH / 4 / 512; ++i) {
+      const int u = tid + 512 * i, row = u / (G::XH / 4), c4 = u - row * (G::XH / 4);
+      const float4 v = *(const float4*)(a.z + (s0 + row) * D + a.a_off + G::XH * part + 4 * c4);
+      uint2 pk;
+      pk.x = pack_bf16(v.x, v.y);
+      pk.y = pack_bf16(v.z, v.w);
+      *(uint2*)(bufA + row * G::ldx + 4 * c4) = pk;
+    }
+  };
+  stage_x(0);
   __syncthreads();
 
   const float ismax = 1.0f / a.smax;
@@ -455,8 +463,9 @@ __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
     const int c = step == 0 ? b : (step == 1 ? b - KB1 : (b - KB1 - KB2) % KB2);
     const unsigned short* src = step == 1 ? bufB : bufA;
     const int ld = step == 1 ? G::ldh : G::ldx;
+    const int cc = step == 0 ? c % (G::XH / 16) : c;  // block within the staged part
 #pragma unroll
-    for (int n = 0; n < NT; ++n) dst[n] = *(const uvec4f*)(src + (32 * n + l32) * ld + 16 * c + 8 * h);
+    for (int n = 0; n < NT; ++n) dst[n] = *(const uvec4f*)(src + (32 * n + l32) * ld + 16 * cc + 8 * h);
   };
   uvec4f bcur[NT], bnxt[NT];
   read_b(0, bcur);
@@ -479,7 +488,8 @@ __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
     const int last = (step == 0 ? KB1 : KB2) - 1;
     // within a step, and across the passes of the last product (h2 stays put), the next
     // block's operands are ready now; after a hidden product they are read after its barrier
-    const bool ahead = b + 1 < G::NBLK && (c != last || step == 2);
+    const bool restage = step == 0 && c % (G::XH / 16) == G::XH / 16 - 1 && c != last;
+    const bool ahead = b + 1 < G::NBLK && (c != last || step == 2) && !restage;
     if (ahead) read_b(b + 1, bnxt);
     if (step == 2 && c == last - kZAhead) load_z(0, G::P3 * w + t3 / KB2, zq);
     // keep the loads here: the scheduler would otherwise sink them next to their use, leaving
@@ -501,6 +511,12 @@ __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
     if (ahead) {
 #pragma unroll
       for (int n = 0; n < NT; ++n) bcur[n] = bnxt[n];
+    }
+    if (restage) {  // the next part of the conditioning half, once every wave is done with this one
+      __syncthreads();
+      stage_x(c / (G::XH / 16) + 1);
+      __syncthreads();
+      read_b(b + 1, bcur);
     }
     if (c != last) continue;
     if (step < 2) {
@@ -612,14 +628,14 @@ hipError_t launch_flow_layer(const FusedLayerArgs& a, hipStream_t s) {
 }
 
 bool frag_layer_supported(int64_t B, int D, int hidden) {
-  return B > 0 && B % 128 == 0 && hidden == 256 && D == 512;
+  return B > 0 && B % 128 == 0 && hidden == 256 && (D == 512 || D == 1024);
 }
 
 hipError_t launch_flow_layer_frag(const FusedLayerArgs& a, hipStream_t s) {
   if (!frag_layer_supported(a.B, a.D, a.hidden)) return hipErrorInvalidValue;
   if ((a.a_off != 0 && a.a_off != a.D / 2) || (a.b_off != 0 && a.b_off != a.D / 2) || a.a_off == a.b_off)
     return hipErrorInvalidValue;
-  return launch_ws<256, 128>(a, s);
+  return a.D == 1024 ? launch_ws<512, 128>(a, s) : launch_ws<256, 128>(a, s);
 }
 
 }  // namespace pib

@@ -85,7 +85,7 @@ class Flow:
         self.draw_precision = "bf16x3"
         self._split_version = -1   # parameter version of the pre-split bf16 weights
         self._fsplit_version = -1  # parameter version of the fused layer's permuted weights
-        self.use_frag = True       # single-bf16 layers at D = 512 on fragment-ordered weights
+        self.use_frag = True       # single-bf16 layers at D = 512, 1024 on fragment-ordered weights
         self._frag_version = -1    # parameter version of those weights
 
     def _set_base(self, rho, sigma, m):
@@ -245,10 +245,10 @@ class Flow:
 
     def _layer_fused(self, inverse, layer, zp, B, logdet, stream, single_bf16=False):
         """The whole coupling layer in one launch (pib_flow_layer_fused); single_bf16: one bf16
-        product per weight block (the hi parts) instead of BF16x3, at D = 512 (with use_frag) on
-        fragment-ordered weights (pib_flow_layer_fused_frag)."""
+        product per weight block (the hi parts) instead of BF16x3, at D = 512 and 1024 (with
+        use_frag) on fragment-ordered weights (pib_flow_layer_fused_frag)."""
         a_off, b_off = self.halves(layer)
-        if single_bf16 and self.use_frag and self.D == 512:
+        if single_bf16 and self.use_frag and self.D in (512, 1024):
             self._split_weights_frag(stream)
             self.ctx.flow_layer_fused_frag(inverse, B, self.D, self.hidden, a_off, b_off, self.smax, zp, logdet,
                                            self._wfrag[layer], self.p(layer, "b1"), self.p(layer, "b2"),

@@ -408,7 +408,7 @@ def test_fused_layer_matches_layer_by_layer(tmp_path, N, P):
     assert dR <= 5e-5 * scale and dld <= 1e-2
 
 
-@pytest.mark.parametrize("N,frag", [(8, True), (8, False), (16, False)])
+@pytest.mark.parametrize("N,frag", [(8, True), (8, False), (16, True), (16, False)])
 def test_single_bf16_layer_is_an_exact_coupling(tmp_path, N, frag):
     """One-launch layers with single bf16 products (lo parts null; at D = 512 with frag the
     fragment-ordered weights of pib_flow_layer_fused_frag): the network differs from
@@ -449,16 +449,18 @@ def test_single_bf16_layer_is_an_exact_coupling(tmp_path, N, frag):
         assert float((z1 - z0).abs().max()) <= 1e-6 * scale
         assert torch.equal(l1, torch.zeros_like(l1))
     # the lo parts are all given or all null
+    fl._split_weights_fused(None)
     w = fl._wfused[0]
     with pytest.raises(_native.NativeError):
         fl.ctx.flow_layer_fused(False, B, D, 256, 0, D // 2, fl.smax, z0, l1,
                                 ((w[0][0], None), w[1], w[2]), fl.p(0, "b1"), fl.p(0, "b2"), fl.p(0, "b3"))
 
 
-@pytest.mark.parametrize("layer,frag", [(0, True), (1, True), (0, False)])
-def test_single_bf16_layer_matches_restatement(tmp_path, layer, frag):
-    """The one-launch single-bf16 layer at D = 512 (both kernels: fragment-ordered weights with
-    frag, pib_flow_layer_fused with null lo parts without) against the network it computes, restated in
+@pytest.mark.parametrize("N,layer,frag", [(8, 0, True), (8, 1, True), (8, 0, False), (16, 0, True), (16, 1, True),
+                                           (16, 0, False)])
+def test_single_bf16_layer_matches_restatement(tmp_path, N, layer, frag):
+    """The one-launch single-bf16 layer at D = 512 and 1024 (both kernels: fragment-ordered weights
+    with frag, pib_flow_layer_fused with null lo parts without) against the network it computes, restated in
     float64 torch: the conditioning half and the weights rounded to bf16 (round to nearest
     even), h1 = bf16(relu(x W1 + b1)), h2 = bf16(relu(h1 W2 + b2)), (raw, t) = h2 W3 + b3,
     s = smax tanh(raw / smax), y = z e^s + t and log-det sum s. The kernel accumulates in FP32,
@@ -466,14 +468,14 @@ def test_single_bf16_layer_matches_restatement(tmp_path, layer, frag):
     other way; such flips are rare and each moves y by about a bf16 ulp of one hidden unit.
     Bounds: 99% of the y elements within 1e-5 of the largest |y| (median within 1e-6), all
     within 2e-2; 95% of the log-dets within 1e-4 relative (or absolute below 1)."""
-    _announce(f"single-bf16 layer {layer} (frag {frag}) vs float64 restatement, D=512")
-    FS = stage_dict_models(tmp_path, synthetic.generate(2, 8, quadratic=False, seed=2))
+    _announce(f"single-bf16 layer {layer} (frag {frag}) vs float64 restatement, D={N * 64}")
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, N, quadratic=False, seed=2))
     data = make_data(FS, 300.0, 64, 64, 64)
     fl = flow.Flow(data, layers=2, hidden=256, gemm="bf16x3")
     fl.set_params(random_params(ref_for(data, 2, 256, 2.0), 7, 0.05))
     fl.use_frag = frag
     B, D, H = 2048, fl.D, fl.H
-    assert D == 512
+    assert D == 64 * N
     fl._split_weights(None)
     z0 = torch.randn((B, D), generator=torch.Generator().manual_seed(3)).to(DEV)
     z1, l1 = z0.clone(), torch.zeros(B, dtype=torch.float64, device=DEV)
