@@ -121,37 +121,55 @@ __global__ void __launch_bounds__(256) k_whiten(BasisArgs b, const void* in, voi
   // each wave owns 16 rows (sample s, mode j) per step: it stages, multiplies and writes only its
   // own rows, so the loop needs no work-group barrier; (s, j) advance without divisions
   const int64_t rows = b.B * (int64_t)N;
-  for (int64_t rw = (int64_t)blockIdx.x * 4 + wave; rw * 16 < rows; rw += (int64_t)gridDim.x * 4) {
+  // Everything a step reads from memory -- its 16 rows and their per-row factors (sigma or m
+  // on the way in, m, 1/sigma or sigma on the way out) -- is loaded one step ahead, before the
+  // current step's products and write-out, and combined only when used: vector loads complete
+  // in order, so a load waited for at once would hold the wave for a memory round trip (per
+  // row, if the factors were read where they are applied).
+  constexpr int NO = P / 64;
+  struct Rows {
+    float x[16][NO];     // input values (whiten: R as float after subtracting m happens at use)
+    double xd[16][NO];   // whiten: R (double)
+    float fin[16][NO];   // unwhiten: sigma of the input element
+    float fout[16][NO];  // whiten: 1/sigma, grad: sigma of the output element
+    double m[16];        // whiten: subtracted on the way in; unwhiten: added on the way out
+  };
+  auto load_rows = [&](int64_t rw, Rows& r) {
     const int64_t gr0 = rw * 16, s0 = gr0 / N;
-    const int j0 = (int)(gr0 - s0 * N);
-    // all 16 rows' loads issued before any is stored (a load waited for at once would hold
-    // the wave for a memory round trip per row)
-    constexpr int NO = P / 64;
-    float xv[16][NO];
-    {
-      int64_t s = s0;
-      int j = j0;
+    int64_t s = s0;
+    int j = (int)(gr0 - s0 * N);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int64_t gr = gr0 + i;
+    for (int i = 0; i < 16; ++i) {
+      const int64_t gr = gr0 + i;
+      if (MODE != FLOW_GRAD) r.m[i] = b.m[j];
 #pragma unroll
-        for (int t = 0; t < NO; ++t) {
-          const int o = lane + 64 * t;
-          if (MODE == FLOW_WHITEN) {
-            xv[i][t] = (float)(((const double*)in)[gr * P + o] - b.m[j]);
-          } else if (MODE == FLOW_GRAD) {
-            xv[i][t] = ((const float*)in)[gr * P + o];
-          } else {  // o runs over (half h, kk); k = 2 kk + h
-            const int h = o / Ph, kk = o - h * Ph, k = 2 * kk + h;
-            xv[i][t] = ((const float*)in)[s * NP + h * H + j * Ph + kk] * b.sigma[j * P + k];
-          }
-        }
-        if (++j == N) {
-          j = 0;
-          ++s;
+      for (int t = 0; t < NO; ++t) {
+        const int o = lane + 64 * t;
+        const int h = o / Ph, kk = o - h * Ph, k = 2 * kk + h;  // permuted position o <-> mode k
+        if (MODE == FLOW_WHITEN) {
+          r.xd[i][t] = ((const double*)in)[gr * P + o];
+          r.fout[i][t] = b.inv_sigma[j * P + k];
+        } else if (MODE == FLOW_GRAD) {
+          r.x[i][t] = ((const float*)in)[gr * P + o];
+          r.fout[i][t] = b.sigma[j * P + k];
+        } else {
+          r.x[i][t] = ((const float*)in)[s * NP + h * H + j * Ph + kk];
+          r.fin[i][t] = b.sigma[j * P + k];
         }
       }
+      if (++j == N) {
+        j = 0;
+        ++s;
+      }
     }
+  };
+  const int64_t rstep = (int64_t)gridDim.x * 4;
+  Rows nxt;
+  if (((int64_t)blockIdx.x * 4 + wave) * 16 < rows) load_rows((int64_t)blockIdx.x * 4 + wave, nxt);
+  for (int64_t rw = (int64_t)blockIdx.x * 4 + wave; rw * 16 < rows; rw += rstep) {
+    const int64_t gr0 = rw * 16, s0 = gr0 / N;
+    const int j0 = (int)(gr0 - s0 * N);
+    const Rows cur = nxt;
 #pragma unroll
     for (int i = 0; i < 16; ++i)
 #pragma unroll
@@ -159,12 +177,15 @@ __global__ void __launch_bounds__(256) k_whiten(BasisArgs b, const void* in, voi
         const int o = lane + 64 * t;
         if (inv) {
           const int h = o / Ph, kk = o - h * Ph;
-          Xs[i * L::ldx + 2 * kk + h] = xv[i][t];
+          Xs[i * L::ldx + 2 * kk + h] = cur.x[i][t] * cur.fin[i][t];
+        } else if (MODE == FLOW_WHITEN) {
+          Xs[i * L::ldx + o] = (float)(cur.xd[i][t] - cur.m[i]);
         } else {
-          Xs[i * L::ldx + o] = xv[i][t];
+          Xs[i * L::ldx + o] = cur.x[i][t];
         }
       }
     wave_lds_sync();
+    if ((rw + rstep) * 16 < rows) load_rows(rw + rstep, nxt);
     // one column tile at a time, each a single in-place accumulator chain written to Ys at
     // once (keeping all P/16 tiles live let the compiler rename accumulators between unrolled
     // steps with a destination that partly overlaps the C operand, which gave wrong sums)
@@ -183,16 +204,17 @@ __global__ void __launch_bounds__(256) k_whiten(BasisArgs b, const void* in, voi
     wave_lds_sync();
     int64_t s = s0;
     int j = j0;
+#pragma unroll
     for (int i = 0; i < 16; ++i) {
       const int64_t gr = gr0 + i;
 #pragma unroll
-      for (int o = lane; o < P; o += 64) {
+      for (int t = 0; t < NO; ++t) {
+        const int o = lane + 64 * t;
         if (inv) {
-          ((double*)out)[gr * P + o] = (double)Ys[i * L::ldx + o] + b.m[j];
+          ((double*)out)[gr * P + o] = (double)Ys[i * L::ldx + o] + cur.m[i];
         } else {
           const int h = o / Ph, kk = o - h * Ph, k = 2 * kk + h;
-          const float f = (MODE == FLOW_WHITEN) ? b.inv_sigma[j * P + k] : b.sigma[j * P + k];
-          ((float*)out)[s * NP + h * H + j * Ph + kk] = Ys[i * L::ldx + k] * f;
+          ((float*)out)[s * NP + h * H + j * Ph + kk] = Ys[i * L::ldx + k] * cur.fout[i][t];
         }
       }
       if (++j == N) {
