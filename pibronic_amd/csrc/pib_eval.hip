@@ -112,12 +112,9 @@ __device__ __forceinline__ void mode_terms(const double* __restrict__ tab, const
                                            double& tm, double (&er)[AR], double (&ev)[A], double (&ep)[A],
                                            double (&em)[A], double (&v)[A][A]) {
   constexpr ModeRec R(A, AR);
-  // the next mode's coordinates are loaded while this mode's arithmetic runs
-  double x0 = qs[p], xn = qs[pn];
-  for (int j = 0; j < N; ++j) {
+  // one mode's terms from its coordinates x0 (bead p) and xn (bead pn)
+  auto mode = [&](int j, double x0, double xn) {
     const double* __restrict__ t = tab + j * R.len();
-    const int jn = (j + 1 < N) ? j + 1 : j;
-    const double x0_next = qs[jn * P + p], xn_next = qs[jn * P + pn];
     const double sx = x0 + xn, dx = x0 - xn;
     const double q2s = fma(x0, x0, xn * xn), d2 = dx * dx;
     tr = fma(t[1], d2, fma(t[0], q2s, tr));
@@ -145,8 +142,33 @@ __device__ __forceinline__ void mode_terms(const double* __restrict__ tab, const
         v[b][c] = fma(t[R.lin_q() + e], x0, v[b][c]);
         if (QD) v[b][c] = fma(t[R.quad() + e], xx, v[b][c]);
       }
-    x0 = x0_next;
-    xn = xn_next;
+  };
+  if constexpr (A <= 4) {
+    // coordinates loaded two modes ahead, the loop unrolled by two so that the rotation needs
+    // no register copy (a copy of a value still in flight would wait for the load just issued;
+    // 4% faster at A = 4; the larger kernels, at the register limit, keep one mode ahead)
+    const int j1 = N > 1 ? 1 : 0;
+    double x0 = qs[p], xn = qs[pn], x0_1 = qs[j1 * P + p], xn_1 = qs[j1 * P + pn];
+#pragma unroll 2
+    for (int j = 0; j < N; ++j) {
+      const int j2 = (j + 2 < N) ? j + 2 : N - 1;
+      const double x0_2 = qs[j2 * P + p], xn_2 = qs[j2 * P + pn];
+      mode(j, x0, xn);
+      x0 = x0_1;
+      xn = xn_1;
+      x0_1 = x0_2;
+      xn_1 = xn_2;
+    }
+  } else {
+    // the next mode's coordinates are loaded while this mode's arithmetic runs
+    double x0 = qs[p], xn = qs[pn];
+    for (int j = 0; j < N; ++j) {
+      const int jn = (j + 1 < N) ? j + 1 : j;
+      const double x0_next = qs[jn * P + p], xn_next = qs[jn * P + pn];
+      mode(j, x0, xn);
+      x0 = x0_next;
+      xn = xn_next;
+    }
   }
 }
 
