@@ -95,11 +95,15 @@ __device__ __forceinline__ void mode_terms_f(const float* __restrict__ tab, cons
                                              float& tm, float (&er)[AR], float (&ev)[A], float (&ep)[A],
                                              float (&em)[A], float (&v)[A][A]) {
   constexpr ModeRec R(A, AR);
-  double x0d = qs[p], xnd = qs[pn];
+  // coordinates loaded two modes ahead, the loop unrolled by two so that the rotation needs
+  // no register copy (a copy of a value still in flight would wait for the load just issued)
+  const int j1 = N > 1 ? 1 : 0;
+  double x0d = qs[p], xnd = qs[pn], x0_1 = qs[j1 * P + p], xn_1 = qs[j1 * P + pn];
+#pragma unroll 2
   for (int j = 0; j < N; ++j) {
     const float* __restrict__ t = tab + j * R.len_f();
-    const int jn = (j + 1 < N) ? j + 1 : j;
-    const double x0_next = qs[jn * P + p], xn_next = qs[jn * P + pn];
+    const int j2 = (j + 2 < N) ? j + 2 : N - 1;
+    const double x0_2 = qs[j2 * P + p], xn_2 = qs[j2 * P + pn];
     const float x0 = (float)x0d, xn = (float)xnd;
     const float sx = x0 + xn, dx = x0 - xn;
     const float q2s = fmaf(x0, x0, xn * xn), d2 = dx * dx;
@@ -127,8 +131,10 @@ __device__ __forceinline__ void mode_terms_f(const float* __restrict__ tab, cons
         v[b][c] = fmaf(t[R.lin_q() + e], x0, v[b][c]);
         if (QD) v[b][c] = fmaf(t[R.quad() + e], xx, v[b][c]);
       }
-    x0d = x0_next;
-    xnd = xn_next;
+    x0d = x0_1;
+    xnd = xn_1;
+    x0_1 = x0_2;
+    xn_1 = xn_2;
   }
 }
 
