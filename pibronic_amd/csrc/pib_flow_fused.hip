@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------
-// k_flow_layer_ws: the single-bf16 layer at D = 512 on fragment-ordered weights
+// k_flow_layer_ws: the single-bf16 layer at D = 512 and 1024 on fragment-ordered weights
 // (pib_flow_layer_fused_frag), weights streamed per wave (v_mfma_f32_32x32x16_bf16: lane l
 // supplies A[l%32][8(l/32)+e] and B[8(l/32)+e][l%32] and holds D[8(r/4)+4(l/32)+r%4][l%32] in
 // register r). Work-group: 8 waves and SB = 128 samples. The layer is a sequence of weight
@@ -370,12 +370,13 @@ __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
 // kRing - 1 blocks ahead of use; the biases sit in LDS and the coupling's z is loaded ahead of
 // its epilogue, because vector loads complete in order: a load issued behind the ring and
 // waited for at once would wait for the whole ring. The activations of all SB samples are
-// staged in LDS as bf16 rows in natural order -- the conditioning half, then h1, then h2 over
-// the conditioning half -- and every wave reads them as B operands one block ahead, each
+// staged in LDS as bf16 rows in natural order -- the conditioning half (at D = 1024 in two
+// parts of 256 inputs, the second behind a barrier pair halfway through the first product),
+// then h1, then h2 over the conditioning half -- and every wave reads them as B operands one block ahead, each
 // 16-byte read feeding one 32x32x16 MFMA (two in the last product) for 32 samples. Against
 // k_flow_layer (each wave reading every staged weight tile for its own 16 samples, a barrier
 // per weight stage) this halves the LDS bytes per flop, removes its bank conflicts and takes
-// four barriers per layer instead of 32. The 16-register MFMA results are early-clobber for
+// four barriers per layer (six at D = 1024) instead of 32. The 16-register MFMA results are early-clobber for
 // the compiler, so a destination never partly overlaps its accumulator input.
 // ---------------------------------------------------------------------------------------
 typedef float f32x16f __attribute__((ext_vector_type(16)));
