@@ -263,7 +263,7 @@ int pib_flow_layer_fused(pib_ctx* ctx, int inverse, int64_t B, int D, int hidden
  * split with PIB_SPLIT_TRANSPOSE | PIB_SPLIT_FRAG32 (all three, inputs in natural order; only
  * the hi parts): each wave streams its weight tiles as contiguous 1 KB blocks. Same network,
  * coupling and exactness of the log-determinant; the FP32 summation order differs. Supported:
- * hidden 256, D = 512, B a multiple of 128; weights and z 16-byte aligned. No counterpart in
+ * hidden 256, D = 512 or 1024, B a multiple of 128; weights and z 16-byte aligned. No counterpart in
  * the reference. */
 int pib_flow_layer_fused_frag(pib_ctx* ctx, int inverse, int64_t B, int D, int hidden, int a_off,
                               int b_off, float smax, float* z_dev, double* logdet_dev,

@@ -652,7 +652,7 @@ int pib_flow_layer_fused_frag(pib_ctx* c, int inverse, int64_t B, int D, int hid
   if (!z || !logdet || !W1_frag || !W2_frag || !W3_frag || !b1 || !b2 || !b3)
     return fail(c, PIB_ERR_ARG, "flow_layer_fused_frag: null pointer");
   if (!frag_layer_supported(B, D, hidden))
-    return fail(c, PIB_ERR_UNSUPPORTED, "flow_layer_fused_frag: needs hidden 256, D = 512 and B a multiple of 128");
+    return fail(c, PIB_ERR_UNSUPPORTED, "flow_layer_fused_frag: needs hidden 256, D = 512 or 1024 and B a multiple of 128");
   const int H = D / 2;
   if (!((a_off == 0 && b_off == H) || (a_off == H && b_off == 0)))
     return fail(c, PIB_ERR_ARG, "flow_layer_fused_frag: the halves must be (0, D/2) or (D/2, 0)");
