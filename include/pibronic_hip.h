@@ -47,14 +47,14 @@ extern "C" {
 #define PIB_EVAL_NOSCALE 2 /* skip the S12 scaling (S_p = 1), as the explicit golden test */
 #define PIB_EVAL_WIDE 4    /* force the one-wave-per-sample kernel (used for A > 8; any
                               A, A_rho <= 16); a tuning/testing switch, same results       */
-#define PIB_EVAL_FP32 8    /* reduced precision (A <= 8): O-factors, V, Jacobi, M and ring
-                              products in FP32 (power-of-two renormalised), FP64 rotation
-                              parameters for near-degenerate pairs (gap < 1e-6 |V|_F), FP64
-                              log-scale and rho log-sums; error vs FP64 is the caller's to
-                              report (the outputs have the same meaning)                  */
-#define PIB_EVAL_BF16X3 16 /* wide kernel with M and the ring products as BF16x3 splits on
-                              the bf16 matrix cores, FP32 accumulation, renormalised every
-                              bead (A, A_rho <= 16); Jacobi and O-factors stay FP64        */
+#define PIB_EVAL_FP32 8    /* reduced precision (A <= 8): O-factors, V, M = exp(-tau V) (a
+                              scaled and squared Taylor polynomial) and the ring products in
+                              FP32 (power-of-two renormalised); FP64 log-scale and rho
+                              log-sums; error vs FP64 is the caller's to report (the
+                              outputs have the same meaning)                               */
+#define PIB_EVAL_BF16X3 16 /* wide kernel with the ring products as BF16x3 splits on the
+                              bf16 matrix cores, FP32 accumulation, renormalised every bead
+                              (A, A_rho <= 16); M = exp(-tau V) and the O-factors stay FP64  */
 
 /* rows of the per-sample output matrix written by pib_eval_block */
 #define PIB_OUT_RHO 0  /* s_rho = rho / prod_p S_p             (BoxResult.scaled_rho)       */
@@ -312,7 +312,10 @@ int pib_adam_f32(pib_ctx* ctx, int64_t n, float* p_dev, const float* g_dev, floa
  * A batch evaluates one block of B samples for each of n problems (contexts: model, O-matrix
  * sets, beads, sampler) in ONE persistent launch; problems may differ in P (the temperatures
  * of a run) and, for the wide family, in A, A_rho and N. The contexts must outlive the batch
- * and keep their constants while it exists. Not re-entrant (one stream at a time).
+ * and keep their constants while it exists: the batch copies their device pointers, so after
+ * any pib_set_coupling / pib_set_omatrix_set / pib_set_beads / pib_set_sampler on one of its
+ * contexts, pib_batch_eval, _eval_sequential, _tune and _run return PIB_ERR_STATE without
+ * launching (create a new batch). Not re-entrant (one stream at a time).
  * ---------------------------------------------------------------------------------------- */
 typedef struct pib_batch pib_batch;
 

@@ -27,8 +27,14 @@ Pinning (what checks each function; the tests under tests/ implement every item)
   energy must reproduce E_harmonic of sos_B80.json within its statistical error.
 * quadratic couplings in V at A > 2: brute-force loops over (j, k) on tiny inputs.
 * eigh / M: M . expm(+tau V) = I (scipy.linalg.expm) and symmetry.
-* A_rho != A sampling path: unbiasedness (<g/rho> identical in expectation to the
-  A_rho == A estimate of the same model) -- statistical only.
+* A_rho != A sampling path (the qTempTensor copy): the oracle's rho against
+  sum_a F_a^P exp(-1/2 sum_j x^T K_j x) with K_j the inverse of the reference's golden
+  sampling covariance (deterministic); and Z_rho * <g/rho> of the reference's Jahn-Teller
+  model with its three alternate sampling models (A_rho = 2, 4, 8) agreeing within
+  3 sigma (tests/test_closed_form.py).
+* log_z_ratio_surface_diagonal (exact Z_g/Z_rho without inter-surface couplings): a
+  Gauss-Hermite quadrature of the oracle's own g and rho at P = 3, the identity g == rho
+  at Q = 0, and the oracle's harmonic Monte Carlo estimate (tests/test_closed_form.py).
 * stats formulas: the reference's known-answer tests (tests/stats/test_stats.py:46-117,
   tests/stats/test_jackknife.py:22-57) re-stated as tests.
 """

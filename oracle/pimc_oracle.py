@@ -18,6 +18,7 @@ import json
 import numpy as np
 from numpy import newaxis as NEW
 import scipy.linalg
+import scipy.special
 
 # pibronic/constants.py:12,24 -- k_B in eV/K, built from the same two NIST numbers
 BOLTZMANN_EV_PER_K = np.float64(1.38064852e-23) / np.float64(1.6021766208e-19)
@@ -352,3 +353,64 @@ def g_of_R_hook(setup, R):
     V, lam, U = coupling_eig(q_all, setup.vib["energy"], setup.vib["linear"], setup.vib["quadratic"])
     M = m_matrix(lam, U, setup.tau)
     return ring_trace(M, O)
+
+
+# --------------------------------------------------------------------------------------
+# exact Z_g / Z_rho for surface-diagonal coupled models (pins the estimators' mean)
+# --------------------------------------------------------------------------------------
+def _log_gauss_integral(coth, csch, tqc, shift, P):
+    """log of int_{R^P} exp(-1/2 x^T K x - (tqc/2) |x + d 1|^2) dx (d = shift), where
+    K = 2 coth I - csch C is the ring precision of one mode (C the adjacency matrix of the
+    ring, eigenvalues 2 cos(2 pi k / P)) and tqc = tau * Q_jj,aa. With A = K + tqc I and
+    K 1 = (2 coth - 2 csch) 1:
+      (P/2) log 2pi - 1/2 sum_k log(2 coth - csch lam_k + tqc)
+        - tqc P d^2 / 2 + (tqc d)^2 P / (2 (2 coth - 2 csch + tqc)).
+    """
+    lam = 2.0 * np.cos(2.0 * np.pi * np.arange(P) / P)
+    ev = 2.0 * coth - csch * lam + tqc
+    if np.any(ev <= 0.0):
+        raise ValueError("K + tau Q is not positive definite: the integral diverges")
+    ones_ev = 2.0 * coth - 2.0 * csch + tqc
+    return (0.5 * P * np.log(2.0 * np.pi) - 0.5 * np.sum(np.log(ev))
+            - 0.5 * tqc * P * shift * shift + 0.5 * (tqc * shift) ** 2 * P / ones_ev)
+
+
+def log_z_ratio_surface_diagonal(setup):
+    """log(Z_g / Z_rho) exactly, for a coupled model whose E, L^j and Q^jk are diagonal in the
+    surfaces with Q^jk = 0 for j != k (no inter-surface coupling) and any sampling model.
+
+    Z_g = int g(R) dR with g = tr prod_p M_p O_p (pibronic/pimc/pimc.py:1177-1213): here V(R)
+    is diagonal, V_aa(R_p) = 1/2 sum_j Q_jj,aa R_jp^2 (the folded E_aa, L^j_aa are zero,
+    :208-218; the quadratic diagonal is not folded, :1147-1160), so M_p = diag(e^{-tau V_aa})
+    and g = sum_a F_a^P prod_j exp(-1/2 x_aj^T K_j x_aj - tau/2 Q_jj,aa |R_j|^2), x = R - d_a
+    (O-matrix exponent summed over the ring, :1087-1129) -- a Gaussian integral per (a, j).
+    Z_rho = int rho(R) dR with rho = sum_a F^rho_a^P prod_j exp(-1/2 x^T K^rho_j x)
+    (:1132-1136). <g/rho> under the harmonic sampler, and <g/q> under any proposal q in
+    rho's normalisation, estimate exactly this ratio at the same P (the Trotter
+    approximation is shared, not an error of the estimator).
+    """
+    v, r = setup.vib, setup.rho
+    E, L, Q = v["energy"], v["linear"], v["quadratic"]
+    A, N, P = setup.A, setup.N, setup.P
+    offdiag = ~np.eye(A, dtype=bool)
+    if np.any(E[offdiag]) or np.any(L[:, offdiag]) or np.any(Q[..., offdiag]):
+        raise ValueError("the closed form needs surface-diagonal couplings")
+    jk = ~np.eye(N, dtype=bool)
+    if np.any(Q[jk]):
+        raise ValueError("the closed form needs Q^jk = 0 for j != k")
+    coth, csch, pref = setup.c_vib
+    terms_g = np.empty(A)
+    for a in range(A):
+        s = P * np.log(pref[a])
+        for j in range(N):
+            s += _log_gauss_integral(coth[a, j], csch[a, j], setup.tau * Q[j, j, a, a],
+                                     v["shift"][a, j], P)
+        terms_g[a] = s
+    cr, sr, pr = setup.c_rho
+    terms_r = np.empty(setup.A_rho)
+    for a in range(setup.A_rho):
+        s = P * np.log(pr[a])
+        for j in range(N):
+            s += _log_gauss_integral(cr[a, j], sr[a, j], 0.0, 0.0, P)
+        terms_r[a] = s
+    return float(scipy.special.logsumexp(terms_g) - scipy.special.logsumexp(terms_r))

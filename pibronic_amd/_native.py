@@ -324,7 +324,8 @@ class Batch:
     """A pib_batch over Contexts (one block of B samples per problem, one persistent launch).
 
     The Contexts must stay alive, with unchanged constants, while the batch exists (the batch
-    keeps references to them)."""
+    keeps references to them); after a setter call on one of them the batch refuses to launch
+    (NativeError PIB_ERR_STATE) and must be created again."""
 
     def __init__(self, contexts, B, pm=False, noscale=False, delta_beta=1.0, max_blocks=1, fp32=False):
         self.lib = load()

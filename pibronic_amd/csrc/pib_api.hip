@@ -225,6 +225,7 @@ const char* pib_last_error(const pib_ctx* c) { return c ? c->err.c_str() : "null
 int pib_set_coupling(pib_ctx* c, int A, int N, const double* E, const double* L,
                      const double* Q) {
   if (!c) return PIB_ERR_ARG;
+  ++c->gen;
   if (A < 1 || N < 1 || !E || !L) return fail(c, PIB_ERR_ARG, "set_coupling: bad A/N or null array");
   const size_t AA = (size_t)A * A;
   if (!finite_all(E, AA) || !finite_all(L, (size_t)N * AA) || (Q && !finite_all(Q, (size_t)N * N * AA)))
@@ -308,6 +309,7 @@ int pib_set_coupling(pib_ctx* c, int A, int N, const double* E, const double* L,
 int pib_set_omatrix_set(pib_ctx* c, int slot, int A_set, int N, const double* coth,
                         const double* csch, const double* pref, const double* shift) {
   if (!c) return PIB_ERR_ARG;
+  ++c->gen;
   if (slot < 0 || slot >= NUM_SETS) return fail(c, PIB_ERR_ARG, "set_omatrix_set: bad slot");
   if (A_set < 1 || N < 1 || !coth || !csch || !pref || !shift)
     return fail(c, PIB_ERR_ARG, "set_omatrix_set: bad size or null array");
@@ -366,6 +368,7 @@ int pib_set_omatrix_set(pib_ctx* c, int slot, int A_set, int N, const double* co
 
 int pib_set_beads(pib_ctx* c, int P, double tau) {
   if (!c) return PIB_ERR_ARG;
+  ++c->gen;
   if (P < 3) return fail(c, PIB_ERR_ARG, "circulant matrix requires 3 or more beads");
   if (!std::isfinite(tau) || tau <= 0.0) return fail(c, PIB_ERR_ARG, "set_beads: tau must be > 0");
   c->P = P;
@@ -414,6 +417,7 @@ int pib_eval_block(pib_ctx* c, const double* q, int64_t B, int flags, double* ou
 int pib_set_sampler(pib_ctx* c, int A_rho, int N, int P, const double* w, const double* coth,
                     const double* csch, const double* shift) {
   if (!c) return PIB_ERR_ARG;
+  ++c->gen;
   if (A_rho < 1 || N < 1 || P < 3 || !w || !coth || !csch || !shift)
     return fail(c, PIB_ERR_ARG, "set_sampler: bad size or null array");
   if (P > 4096) return fail(c, PIB_ERR_ARG, "set_sampler: P > 4096 not supported");

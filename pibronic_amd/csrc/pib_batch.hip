@@ -45,6 +45,7 @@ struct pib_batch {
   int64_t B = 0;
   double delta_beta = 1.0;
   std::vector<pib_ctx*> ctx;
+  std::vector<uint64_t> gens;  // each context's setter generation when the batch copied its pointers
   std::vector<EvalArgs> args;  // per problem; q/out point into the batch buffers
   bool lane_ok = false, wide_ok = false;
   int A = 0, AR = 0;
@@ -161,8 +162,14 @@ hipError_t launch_batch_eval(pib_batch* b, hipStream_t s) {
 }
 
 int check_bound(pib_batch* b, const char* who) {
-  for (int i = 0; i < b->n; ++i)
+  for (int i = 0; i < b->n; ++i) {
     if (!b->bound[i]) return bfail(b, PIB_ERR_STATE, std::string(who) + ": buffers of problem " + std::to_string(i) + " not bound");
+    // the argument blocks hold the context's device pointers as they were at batch_create;
+    // a setter since then may have reallocated them
+    if (b->ctx[i]->gen != b->gens[i])
+      return bfail(b, PIB_ERR_STATE, std::string(who) + ": the constants of problem " + std::to_string(i) +
+                                         " were set again after batch_create; create a new batch");
+  }
   return PIB_OK;
 }
 
@@ -263,6 +270,7 @@ int pib_batch_create(int n, pib_ctx* const* ctxs, int64_t B, int flags, double d
   b->wide_ok = true;
   b->args.resize(n);
   b->lanes.resize(n);
+  b->gens.resize(n);
   for (int i = 0; i < n; ++i) {
     pib_ctx* c = ctxs[i];
     if (!c) return bfail(b, PIB_ERR_ARG, "batch_create: null context");
@@ -270,6 +278,7 @@ int pib_batch_create(int n, pib_ctx* const* ctxs, int64_t B, int flags, double d
     bool wide = false;
     if (make_eval_args(c, flags, B, &b->args[i], &wide))
       return bfail(b, PIB_ERR_STATE, "batch_create: problem " + std::to_string(i) + ": " + c->err);
+    b->gens[i] = c->gen;
     const int A = c->A, AR = b->args[i].AR;
     if (wide || (i > 0 && (A != b->A || AR != b->AR))) b->lane_ok = false;
     if (i == 0) { b->A = A; b->AR = AR; }

@@ -21,6 +21,9 @@ struct pib_ctx {
   int device = 0;
   int num_cu = 256;
   std::string err;
+  // bumped by every setter of constants the evaluation reads (coupling, O-matrix sets, beads,
+  // sampler): a batch records it and refuses to launch with buffers a setter may have freed
+  uint64_t gen = 0;
   // coupling
   int A = 0, N = 0;
   pib::DevBuf energy, linear, qdiag, pair, qpair;

@@ -14,7 +14,12 @@ Differences from the reference, all deliberate:
   * samples are drawn per block on the device (Philox, counter = global sample index) or,
     with data.sampler = "host", by a seeded numpy Generator on the host; the reference's
     legacy global MT19937 stream and its up-front draw of all surface sources (quirk Q5)
-    are not reproduced (the distribution is the same);
+    are not reproduced (the distribution is the same). Like the reference, which reseeds
+    numpy from OS entropy in every job process (pibronic/pimc/pimc.py:55), a run whose
+    data.seed is None takes a fresh 64-bit seed from OS entropy; a job with result.id_job = J
+    starts its counters at sample J * X, so jobs sharing one explicit seed draw disjoint
+    slices of one stream. The seed and the first sample index are written to the result
+    file as the extra keys "seed" and "sample_offset";
   * per-sample standard deviations are not materialised (the reference stores an (X,N,P)
     array); they are looked up per surface on the device;
   * every block also leaves FP64 block sums (result.block_sums) for the statistics, and
@@ -22,6 +27,7 @@ Differences from the reference, all deliberate:
 """
 import json
 import os
+import secrets
 from functools import partial
 
 import numpy as np
@@ -253,6 +259,9 @@ class BoxData:
         # True: the wide kernel with BF16x3 ring products (PIB_EVAL_BF16X3; A <= 16)
         self.ring_bf16x3 = False
         self.seed = None
+        # global index of the run's first sample in the (seed) Philox stream; block_compute
+        # sets it to id_job * samples
+        self.sample_offset = 0
         self.device = 0
         self._ctx = None
         self._dev = None
@@ -406,16 +415,24 @@ class BoxData:
         import torch
         start, B = sample_view.start, sample_view.stop - sample_view.start
         ctx, dev = self.context(), self.device_buffers(B)
-        seed = 0 if self.seed is None else int(self.seed)
+        seed = self.resolve_seed()
+        first = int(self.sample_offset) + start
         if self.sampler == "device":
-            ctx.sample_block(seed, start, B, dev["q"], dev["src"],
+            ctx.sample_block(seed, first, B, dev["q"], dev["src"],
                              torch.cuda.current_stream(self.device))
         elif self.sampler == "host":
-            q, src = self.draw_host_sample(np.random.default_rng([seed, start]), B)
+            q, src = self.draw_host_sample(np.random.default_rng([seed, first]), B)
             dev["q"][:B].copy_(torch.from_numpy(q))
             dev["src"][:B].copy_(torch.from_numpy(src.astype(np.int32)))
         else:
             raise ValueError(f"unknown sampler {self.sampler!r}")
+
+    def resolve_seed(self):
+        """The run's 64-bit sampler seed: data.seed, or, when it is None, a fresh one from OS
+        entropy (kept in data.seed, so every block of the run and the result file use it)."""
+        if self.seed is None:
+            self.seed = secrets.randbits(64)
+        return int(self.seed) & 0xFFFFFFFFFFFFFFFF
 
     def draw_host_sample(self, rng, B):
         """Host draw of R (B, N, P) and surface sources, the reference's algorithm
@@ -498,6 +515,9 @@ class BoxResult:
         else:
             self.samples = 0
         self.block_sums = None
+        # sampler seed and first global sample index of the run (block_compute fills them)
+        self.seed = None
+        self.sample_offset = 0
         self.initialize_arrays()
 
     def compute_path_to_file(self):
@@ -515,8 +535,11 @@ class BoxResult:
         path = self.compute_path_to_file()
         assert self.hash_vib is not None and self.hash_rho is not None, \
             "we save hash values if they don't exist!"
+        extra = {}
+        if self.seed is not None:
+            extra = dict(seed=np.uint64(self.seed), sample_offset=np.int64(self.sample_offset))
         np.savez(path, hash_vib=self.hash_vib, hash_rho=self.hash_rho,
-                 number_of_samples=self.samples, **self._payload())
+                 number_of_samples=self.samples, **self._payload(), **extra)
 
     def _take(self, d, sl, dst):
         dst_map = {"s_g": "scaled_g", "s_rho": "scaled_rho", "s_gP": "scaled_gofr_plus",
@@ -584,6 +607,12 @@ def _run_blocks(data, result, pm):
     assert data.blocks * data.block_size <= result.samples, "result is smaller than the run"
     B = data.block_size
     ctx = data.context()
+    # independent job streams: an unset seed comes from OS entropy, and job J starts at
+    # global sample J * X (pibronic/pimc/pimc.py:55; pibronic/server/job_boss.py:538-611)
+    result.seed = data.resolve_seed()
+    J = 0 if result.id_job is None else int(result.id_job)
+    data.sample_offset = J * int(data.samples)
+    result.sample_offset = data.sample_offset
     stream = torch.cuda.current_stream(data.device)
     dev = data.device_buffers(B)
     result.block_sums = np.zeros((data.blocks, _native.NUM_SUMS))

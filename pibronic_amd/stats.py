@@ -242,7 +242,9 @@ def jackknife_analysis_of_pimc(FS, data_rho, method="basic", delta_beta=constant
     keys = sorted({_parse_pimc_name(f) for f in files})
     written = []
     for P, T in keys:
-        mine = [f for f in files if _parse_pimc_name(f) == (P, T)]
+        # job order (J = 0, 1, ...), so the merged arrays and sums do not depend on glob order
+        mine = sorted((f for f in files if _parse_pimc_name(f) == (P, T)),
+                      key=lambda f: int(os.path.basename(f).split("_J")[1].split("_")[0]))
         res = BoxResultPM()
         res.load_multiple_results(mine)
         analytic = analytic_sampling(data_rho, constants.beta(T), delta_beta)

@@ -16,6 +16,12 @@ below and a divergent partition function; the clip leaves the A=2 and A=4 models
 The sampling model rho is the surface diagonal of the coupled model
 (remove_coupling_from_model semantics, pibronic/vibronic/vibronic_model_io.py:749-761).
 
+surface_diagonal=True draws the same numbers and then zeroes every coupling between
+surfaces (E_ab, L^j_ab, Q^jj_ab for a != b) before the stability clip: the surfaces become
+independent displaced, mode-softened oscillators, g(R) is a sum of Gaussians and Z_g/Z_rho has
+a closed form, while the mode-diagonal Q draws keep rho's tails as light relative to g as in
+the coupled workload.
+
 WORKLOADS names each configuration by its sizes.
 """
 import os
@@ -41,10 +47,15 @@ WORKLOADS = {
     # A=8, N=16, P=64, 300 K (learned-proposal workload)
     "A8_N16_P64_T300": dict(A=8, N=16, P=64, T=300.0, quadratic=True, off=0.05,
                             samples=10_000_000, block=100_000),
+    # the A4 and A8 workloads with every inter-surface coupling removed (exact Z_g/Z_rho)
+    "A4_N8_P64_T300_diag": dict(A=4, N=8, P=64, T=300.0, quadratic=True, off=0.05,
+                                samples=1_000_000, block=100_000, surface_diagonal=True),
+    "A8_N16_P64_T300_diag": dict(A=8, N=16, P=64, T=300.0, quadratic=True, off=0.05,
+                                 samples=10_000_000, block=100_000, surface_diagonal=True),
 }
 
 
-def generate(A, N, quadratic=True, off=0.05, seed=0):
+def generate(A, N, quadratic=True, off=0.05, seed=0, surface_diagonal=False):
     """Return the coupled model as a dict keyed by VibronicModelKeys members."""
     rng = np.random.default_rng(seed)
     w = rng.uniform(0.02, 0.3, N)
@@ -63,6 +74,12 @@ def generate(A, N, quadratic=True, off=0.05, seed=0):
             for a in range(A):
                 for b in range(a, A):
                     Q[j, j, a, b] = Q[j, j, b, a] = rng.normal(0.0, 0.005)
+    if surface_diagonal:
+        keep = np.eye(A, dtype=bool)
+        E = np.where(keep, E, 0.0)
+        L = np.where(keep, L, 0.0)
+        Q = np.where(keep, Q, 0.0)
+    if quadratic:
         for j in range(N):
             lam, vec = np.linalg.eigh(Q[j, j])
             if lam.min() < -0.5 * w[j]:
@@ -86,7 +103,8 @@ def write_workload(root, name, id_data=0, id_rho=0, seed=0):
     from .file_structure import FileStructure
     cfg = WORKLOADS[name]
     FS = FileStructure(root, id_data, id_rho)
-    m = generate(cfg["A"], cfg["N"], cfg["quadratic"], cfg["off"], seed)
+    m = generate(cfg["A"], cfg["N"], cfg["quadratic"], cfg["off"], seed,
+                 surface_diagonal=cfg.get("surface_diagonal", False))
     vIO.save_model_to_JSON(FS.path_vib_model, m)
     vIO.remove_coupling_from_model(FS.path_vib_model, FS.path_rho_model)
     FS.generate_model_hashes()

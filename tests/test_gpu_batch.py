@@ -139,3 +139,23 @@ def test_batch_errors(tmp_path):
     with pytest.raises(_native.NativeError, match="lanes family"):
         b.set_schedule(_native.BATCH_LANES, [4, 4], 4, 256)
     b.close()
+
+
+def test_batch_refuses_contexts_set_again(tmp_path):
+    """A setter on a context after pib_batch_create may reallocate the device buffers whose
+    pointers the batch copied: every launching entry point must then fail with
+    PIB_ERR_STATE, before any kernel is queued."""
+    _announce("batch refuses re-set contexts")
+    FS = stage_dict_models(tmp_path, random_model(3, 4, seed=2, quad="full"))
+    B = 64
+    datas = _datas(FS, B, False, TEMPS_P[:2])
+    tb = TemperatureBatch(datas, B)
+    qs = [d.draw_host_sample(np.random.default_rng(i), B)[0] for i, d in enumerate(datas)]
+    tb.evaluate(qs)  # fine before
+    d = datas[1]
+    d.context().set_beads(d.beads, d.tau)
+    for call in (lambda: tb.batch.eval(), lambda: tb.batch.eval_sequential(), lambda: tb.batch.tune(1),
+                 lambda: tb.batch.run(1, 0, 1)):
+        with pytest.raises(_native.NativeError, match="set again") as ei:
+            call()
+        assert ei.value.code == _native.PIB_ERR_STATE

@@ -7,8 +7,8 @@ Tolerances (FP64 path): per-sample s_rho, s_g, s_gP, s_gM within 1e-10 relative 
 oracle. The remaining differences are the O-matrix exponent (evaluated in a
 cancellation-free form, closer to the exact value than the reference's expression; the
 difference is the reference's own rounding, pinned in test_exponent_form.py), exp()
-implementations (1 ulp), the eigen-solver (Jacobi vs LAPACK) and the association order of
-the bead products.
+implementations (1 ulp), M = exp(-tau V) (a scaled and squared Taylor polynomial against
+LAPACK's eigendecomposition) and the association order of the bead products.
 """
 import os
 

@@ -1,5 +1,6 @@
-"""Parity of the one-wave-per-sample kernel (16x16 padded matrices, one-sided Jacobi, FP64
-MFMA ring products) against the oracle, on the same host-seeded coordinates.
+"""Parity of the one-wave-per-sample kernel (16x16 padded matrices, M = exp(-tau V) by a scaled
+and squared Taylor polynomial and the ring products on the FP64 matrix cores) against the
+oracle, on the same host-seeded coordinates.
 
 It runs automatically for 8 < A <= 16 and can be forced for any A (BoxData.force_wide), which
 lets the same inputs go through both FP64 kernels. Tolerance as for the lanes-per-sample
@@ -78,7 +79,7 @@ def test_wide_large_surface_workload(tmp_path, pm):
     _announce(f"wide A12 N24 P128 pm={pm}")
     cfg = synthetic.WORKLOADS["A12_N24_P128_T250"]
     FS = stage_dict_models(tmp_path, synthetic.generate(12, 24, quadratic=True, off=cfg["off"], seed=0))
-    errs, r, o, q, data = _parity(FS, cfg["T"], cfg["P"], 12, pm, seed=16, force_wide=False, debug=not pm)
+    errs, r, o, q, data = _parity(FS, cfg["T"], cfg["P"], 256, pm, seed=16, force_wide=False, debug=not pm)
     if not pm:
         # the debug records hold O before scaling; the oracle keeps them scaled by S_p
         assert rel_err(r["S"], o["S"]) < 1e-11
@@ -128,15 +129,18 @@ def test_more_than_16_surfaces_is_rejected(tmp_path):
 
 
 # ------------------------------------------------------------------ BF16x3 ring products
-# Stated bounds for PIB_EVAL_BF16X3 (M and the ring chain on the bf16 matrix cores with a
-# hi/lo split and FP32 accumulation; Jacobi and O-factors FP64), against the FP64 wide kernel
-# on the same coordinates: r = g/rho median relative error <= 1e-3 and largest <= 5e-2; the
-# block's Z estimate moves by less than 0.05 of its standard error; rho and the log scale
-# are FP64 paths and agree to 1e-12.
+# Stated bounds for PIB_EVAL_BF16X3 (the ring chain on the bf16 matrix cores with a hi/lo
+# split and FP32 accumulation; M = exp(-tau V) and the O-factors FP64), against the FP64 wide
+# kernel on the same coordinates: r = g/rho median relative error <= 1e-3; the largest
+# |r_bf - r_64| relative to mean |r| (what a block average sees) <= 1e-3; the block's Z estimate
+# moves by less than 0.05 of its standard error; rho and the log scale are FP64 paths and
+# agree to 1e-12. The largest per-sample relative error is reported, not bounded: it comes
+# from samples whose trace cancels (|g| << the terms summed), where any relative bound would be
+# a statement about the cancellation, not about the products.
 def _check_bf(rep):
     print(rep)
     assert rep["r"]["median_rel"] <= 1e-3, rep
-    assert rep["r"]["max_rel"] <= 5e-2, rep
+    assert rep["r"]["max_abs_over_mean_abs"] <= 1e-3, rep
     assert abs(rep["z_shift_sigma"]) < 0.05, rep
     assert rep["s_rho"]["max_rel"] <= 1e-12, rep
     assert rep["log_scale_max_abs"] <= 1e-9, rep

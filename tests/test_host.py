@@ -221,3 +221,34 @@ def test_synthetic_generator():
     assert np.all((a[VMK.w] >= 0.02) & (a[VMK.w] <= 0.3))
     Ts, Ps, tau0 = synthetic.c3_temperatures()
     assert Ps[0] == 256 and Ps[-1] == 52 and np.all(np.diff(Ps) < 0)
+
+
+def test_unset_seed_comes_from_os_entropy(tmp_path):
+    """Two runs without a seed get different 64-bit seeds, each kept for the whole run (the
+    reference reseeds numpy from OS entropy in every job process, pimc.py:55)."""
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, 3, seed=0))
+    a = make_data(FS, 300.0, 12, 10, 10, seed=None)
+    b = make_data(FS, 300.0, 12, 10, 10, seed=None)
+    assert a.seed is None and b.seed is None
+    sa, sb = a.resolve_seed(), b.resolve_seed()
+    assert sa != sb and a.resolve_seed() == sa and a.seed == sa
+    c = make_data(FS, 300.0, 12, 10, 10, seed=77)
+    assert c.resolve_seed() == 77
+
+
+def test_job_split_and_parameter_strings(tmp_path):
+    """PimcSubmissionClass's split (job_boss.py:538-566): 3e5 samples overall in jobs of at
+    most 1e5, blocks of 2.5e4; every job string rebuilds the run's BoxData."""
+    from pibronic_amd.jobs import PimcSubmissionClass
+    FS = stage_dict_models(tmp_path, synthetic.generate(2, 3, seed=0))
+    sub = PimcSubmissionClass(FS, {"number_of_samples": 300_000, "block_size": 25_000,
+                                   "temperature_list": [250.0, 300.0], "bead_list": [12]})
+    jobs = sub.job_parameters()
+    assert sub.n_jobs == 3 and len(jobs) == 6
+    assert [J for _, _, J, _ in jobs] == [0, 1, 2, 0, 1, 2]
+    for T, P, J, s in jobs:
+        assert "," not in s
+        d = pimc.BoxDataPM.from_json_string(s)
+        assert (d.samples, d.blocks, d.block_size) == (100_000, 4, 25_000)
+        assert (d.beads, d.temperature, d.states, d.modes) == (12, T, 2, 3)
+        assert d.path_vib_model == FS.path_vib_model and d.hash_vib == FS.hash_vib
