@@ -160,6 +160,19 @@ int pib_block_sums(pib_ctx* ctx, const double* out_dev, int64_t out_stride, int6
 int pib_block_sums_q(pib_ctx* ctx, const double* out_dev, int64_t out_stride, int64_t B, int pm,
                      double delta_beta, const double* logq_dev, double* sums_dev, void* stream);
 
+/* pib_block_sums (logq_dev NULL) or pib_block_sums_q (logq_dev set) without any device
+ * allocation: the partial rows live in a fixed per-GPU array of the library (1024 rows of
+ * PIB_NUM_SUMS doubles, 114,688 bytes per GPU, held until the process ends) instead of memory
+ * the context allocates on first use. Every B is accepted: the partial rows are capped at
+ * 1024 (the reduction strides over the block). No counterpart in the reference; for callers
+ * that cannot call hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation). All calls on one GPU share its array: calls that overlap on one GPU
+ * (different streams or host threads) race and must be serialised by the caller. Same results,
+ * bit for bit, as pib_block_sums / pib_block_sums_q. */
+int pib_block_sums_noalloc(pib_ctx* ctx, const double* out_dev, int64_t out_stride, int64_t B,
+                           int pm, double delta_beta, const double* logq_dev, double* sums_dev,
+                           void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * Learned proposal (affine-coupling flow over bead normal-mode coordinates). No reference
  * counterpart: the reference's nearest hooks are the ML data generators

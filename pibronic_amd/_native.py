@@ -47,6 +47,7 @@ SIGNATURES = {
     "pib_sample_block": (_c_int, [_vp, _c_u64, _c_i64, _c_i64, _vp, _vp, _vp]),
     "pib_block_sums": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_int, _c_dbl, _vp, _vp]),
     "pib_block_sums_q": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_int, _c_dbl, _vp, _vp, _vp]),
+    "pib_block_sums_noalloc": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_int, _c_dbl, _vp, _vp, _vp]),
     "pib_gemm_f32": (_c_int, [_vp, _c_int, _c_int, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp,
                               _c_i64, _vp, _c_i64, _vp, _c_int, _vp, _c_i64, _c_int, _vp]),
     "pib_gemm_bf16x3": (_c_int, [_vp, _c_int, _c_int, _c_i64, _c_i64, _c_i64, _vp, _c_i64, _vp,
@@ -222,6 +223,13 @@ class Context:
         self._check(self.lib.pib_block_sums_q(self.h, _devptr(out), int(out_stride), int(B),
                                               int(bool(pm)), float(delta_beta), _devptr(logq),
                                               _devptr(sums), _stream(stream)))
+
+    def block_sums_noalloc(self, out, out_stride, B, pm, delta_beta, sums, logq=None, stream=None):
+        """block_sums (logq None) or block_sums_q without device allocation: partial rows in the
+        library's static per-GPU array; calls overlapping on one GPU race."""
+        self._check(self.lib.pib_block_sums_noalloc(self.h, _devptr(out), int(out_stride), int(B),
+                                                    int(bool(pm)), float(delta_beta), _devptr(logq),
+                                                    _devptr(sums), _stream(stream)))
 
     # ------------------------------------------------------------------ learned proposal
     def gemm_f32(self, A, B, C, M, N, K, lda, ldb, ldc, trans_a=False, trans_b=False, bias=None,

@@ -169,6 +169,11 @@ int pib_ctx_create(int device, pib_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     c->num_cu = prop.multiProcessorCount;
+  // the allocation-free block sums' partial rows: resolved here, with the device current
+  if (sum_scratch_address(device, &c->sum_scratch) != hipSuccess) {
+    delete c;
+    return PIB_ERR_HIP;
+  }
   *out = c;
   return PIB_OK;
 }
@@ -522,8 +527,7 @@ static int block_sums_impl(pib_ctx* c, const double* out, int64_t out_stride, in
   if (!out || !sums || B < 1 || out_stride < B)
     return fail(c, PIB_ERR_ARG, "block_sums: null pointer, B < 1 or out_stride < B");
   if (pm && !(delta_beta > 0.0)) return fail(c, PIB_ERR_ARG, "block_sums: delta_beta must be > 0");
-  int nparts = (int)std::min<int64_t>((B + 255) / 256, 1024);
-  if (nparts < 1) nparts = 1;
+  const int nparts = sum_parts(B);
   if (c->nparts < nparts) {
     std::vector<double> zero((size_t)nparts * NUM_SUMS, 0.0);
     int r = upload(c, c->partials, zero.data(), zero.size() * sizeof(double));
@@ -540,6 +544,29 @@ static int block_sums_impl(pib_ctx* c, const double* out, int64_t out_stride, in
   PIB_HIP(c, hipSetDevice(c->device));
   hipError_t e = launch_block_sums(a, (double*)c->partials.p, nparts, sums, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(c, e, "block_sums launch");
+  return PIB_OK;
+}
+
+int pib_block_sums_noalloc(pib_ctx* c, const double* out, int64_t out_stride, int64_t B, int pm,
+                           double delta_beta, const double* logq, double* sums, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (!out || !sums || B < 1 || out_stride < B)
+    return fail(c, PIB_ERR_ARG, "block_sums_noalloc: null pointer, B < 1 or out_stride < B");
+  if (pm && !(delta_beta > 0.0)) return fail(c, PIB_ERR_ARG, "block_sums_noalloc: delta_beta must be > 0");
+  const int nparts = sum_parts(B);
+  if (nparts > kSumMaxParts)
+    return fail(c, PIB_ERR_ARG, "block_sums_noalloc: more partial rows than the static scratch holds");
+  if (!c->sum_scratch) return fail(c, PIB_ERR_STATE, "block_sums_noalloc: static scratch not resolved");
+  SumArgs a{};
+  a.out = out;
+  a.out_stride = out_stride;
+  a.B = B;
+  a.pm = pm ? 1 : 0;
+  a.delta_beta = pm ? delta_beta : 1.0;
+  a.logq = logq;
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_block_sums(a, c->sum_scratch, nparts, sums, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "block_sums_noalloc launch");
   return PIB_OK;
 }
 

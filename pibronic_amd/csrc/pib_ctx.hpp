@@ -55,6 +55,7 @@ struct pib_ctx {
   // reduction scratch
   pib::DevBuf partials;
   int nparts = 0;
+  double* sum_scratch = nullptr;  // this device's static partial rows (pib_block_sums_noalloc)
   // split-K GEMM and column-sum scratch (training calls)
   pib::DevBuf work;
   // flow basis

@@ -150,6 +150,17 @@ constexpr int kWideMaxStates = 16;
 hipError_t launch_sample_fp64(const SampleArgs& a, hipStream_t s);
 hipError_t launch_block_sums(const SumArgs& a, double* partials, int nparts, double* sums,
                              hipStream_t s);
+// Partial rows of the block sums: one per work-group of k_sum_partials, at most this many
+// whatever B is (the work-groups stride over the block).
+constexpr int kSumMaxParts = 1024;
+inline int sum_parts(int64_t B) {
+  const int64_t n = (B + 255) / 256;
+  return (int)(n < 1 ? 1 : n > kSumMaxParts ? kSumMaxParts : n);
+}
+// Device address, on the current device, of the file-scope partial-row array that
+// pib_block_sums_noalloc uses instead of allocated memory (kSumMaxParts * NUM_SUMS doubles per
+// device). Resolved once per device; the current device must be `device`.
+hipError_t sum_scratch_address(int device, double** ptr);
 bool eval_supported(int A, int AR);
 
 }  // namespace pib

@@ -9,12 +9,43 @@
 // fixed order into one partial row; stage 2: one work-group adds the partial rows in index
 // order. No atomics, so the result does not depend on scheduling. With a learned proposal
 // the denominator is its density q (log q given per sample) instead of rho.
+#include <mutex>
+
 #include "pib_common.hpp"
 #include "pib_kernels.hpp"
 
 namespace pib {
 
 constexpr int kSumThreads = 256;
+
+namespace {
+// Partial rows for pib_block_sums_noalloc: callers that cannot allocate (graph capture, tight
+// serving loops) get this fixed array instead of the context's allocated partials. Every GPU
+// holds its own copy for the life of the process; all calls on one GPU share it, so such calls
+// must not overlap on that GPU. k_sum_partials writes every row k_sum_final reads, so nothing
+// needs clearing between calls.
+static __device__ __align__(256) double scratch[kSumMaxParts * NUM_SUMS];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+hipError_t sum_scratch_address(int device, double** ptr) {
+  *ptr = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    // the array's name in host code is only a handle: the device address comes from the
+    // runtime, per device (the caller has made `device` current)
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<double*>(p);
+  }
+  *ptr = g_scratch[device];
+  return hipSuccess;
+}
 
 __device__ __forceinline__ void block_tree(double* sh, double (&v)[NUM_SUMS]) {
   const int t = threadIdx.x;
