@@ -1,16 +1,27 @@
 """Benchmark of the PIMC block estimator on one MI355X.
 
-Workload (default): A=4 surfaces, N=8 modes, P=64 beads, T=300 K, seeded synthetic model
-(linear + mode-diagonal quadratic couplings), blocks of 1e5 samples, FP64. One step is one
-block through the hot path: device bead sampling, the fused evaluation of g and rho
-(block_compute's per-block body) and the on-device block sums. Inputs and outputs stay
-in HBM; the per-sample results are not copied to the host inside the timed region.
+Default workload: A=8 surfaces, N=16 modes, P=64 beads, T=300 K, seeded synthetic model
+(linear + mode-diagonal quadratic couplings). The metric is effective samples per second at a
+fixed Z-ratio standard error, learned proposal against harmonic rho on the same GPU:
 
-Prints one JSON line. `value` is effective samples per second (Kish ESS of w = |g/rho|
-over the timed samples, divided by the timed wall time); samples*beads/s is reported
-beside it. The CPU baseline is the FP64 NumPy restatement of the reference algorithm
-(the oracle), one process per host core, timed on a bounded sample before the GPU is
-touched.
+  * learned leg: the affine-coupling flow (8 layers, hidden 256, over the N*P = 1024 bead
+    normal-mode coordinates) is trained on-policy on the method's own samples and weights
+    (reported as train_seconds, outside the timed region), then one timed step is one draw of
+    65,536 samples (base draw with its log rho, 8 coupling layers with single-bf16 products,
+    unwhitening) + the FP64 evaluation of g + the on-device block sums with q as the
+    denominator; the default K steps evaluate >= 1e7 samples;
+  * harmonic leg: the same number of samples from rho in blocks of 1e5 (device sampler, FP64
+    evaluation, block sums);
+  * value = Kish ESS of w = g/q (|w|) over the timed learned samples / timed wall time.
+
+Also in the line: the same timed draws with FP32-accurate flow products (BF16x3, and FP32
+matrix-core products), tail diagnostics of both proposals' weights (largest weight share,
+Pareto k-hat), and the plain FP64 kernel's samples*beads/s and roofline on the A=4, N=8, P=64
+workload (blocks of 1e5) with the CPU oracle timed beside each. Inputs and outputs stay in HBM;
+nothing is copied to the host inside a timed region.
+
+The CPU baseline is the FP64 NumPy restatement of the reference algorithm (the oracle), one
+process per core, on a bounded sample, before the GPU is touched.
 """
 import argparse
 import json
@@ -26,7 +37,8 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix peaks coincide)
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 matrix cores
 HBM_PEAK_BPS = 8.0e12  # MI355X HBM3E
-HBM_PEAK_GBS = 8000.0
+METRIC = "effective samples/sec at fixed Z-ratio std error; learned ρ vs harmonic ρ"
+CPU_WORKER_CAP = 16  # worker processes per GPU box (its CPU share for one GPU)
 
 
 def algorithmic_flops(A, N, P, n_quad_pairs, pm=False):
@@ -45,8 +57,7 @@ def algorithmic_flops(A, N, P, n_quad_pairs, pm=False):
 def profiled_traffic(path, kernel_prefix):
     """HBM bytes per launch of a kernel from a committed rocprofv3 summary: FETCH_SIZE and
     WRITE_SIZE (KiB per dispatch), each divided by what the same counter reads for a plain
-    256 MiB device copy in the same run (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x on MI355X).
-    Returns (bytes, detail) or (None, reason)."""
+    256 MiB device copy in the same run. Returns (bytes, detail) or (None, reason)."""
     if not os.path.exists(path):
         return None, "no profile"
     rows = {}
@@ -95,8 +106,19 @@ def _cpu_worker(args):
             return n, el, sw, sw2
 
 
+def host_cores():
+    """(cores used, os.cpu_count(), CPUs this process may run on)."""
+    total = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = total
+    return min(avail, CPU_WORKER_CAP), total, avail
+
+
 def cpu_baseline(vib_path, rho_path, T, P, seconds, cores, B=500):
     import multiprocessing as mp
+    import numpy as np
     for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
         os.environ[k] = "1"
     ctx = mp.get_context("spawn")
@@ -108,9 +130,21 @@ def cpu_baseline(vib_path, rho_path, T, P, seconds, cores, B=500):
     n_tot = sum(r[0] for r in res)
     sw = sum(r[2] for r in res)
     sw2 = sum(r[3] for r in res)
-    import numpy as np
     return dict(samples_beads_per_s=sbs, one_core=one, samples=n_tot, ess_frac=(sw * sw / sw2) / n_tot,
-                seconds=max(r[1] for r in res), numpy=np.__version__)
+                seconds=max(r[1] for r in res), numpy=np.__version__, T=T, P=P, cores=cores)
+
+
+def cpu_line(cpu, gpu_sbs, what):
+    used, total, avail = host_cores()
+    return {"value": cpu["samples_beads_per_s"], "unit": "samples*beads/s", "cores": cpu["cores"],
+            "kind": "port",
+            "sample": f"oracle (FP64 NumPy restatement of the reference algorithm, per-(sample, bead) "
+                      f"dot loop, batched eigh) on {what}, {cpu['samples']} samples in blocks of 500 over "
+                      f"{cpu['seconds']:.1f} s, one process per core, numpy {cpu['numpy']}",
+            "host_cpu_count": total, "host_cpus_available": avail,
+            "cores_note": f"one worker per available CPU, capped at {CPU_WORKER_CAP} (the GPU box's CPU "
+                          "share for one GPU)",
+            "one_core_value": cpu["one_core"], "gpu_over_cpu": gpu_sbs / cpu["samples_beads_per_s"]}
 
 
 # --------------------------------------------------------------------------------------
@@ -130,43 +164,122 @@ def weight_summary(s, elapsed):
             "samples_for_1e-3_z_error": n_needed, "time_to_1e-3_z_error_s": n_needed * elapsed / n}
 
 
-def learned_leg(data, args, log):
-    """Train the flow on-policy, then time K steps of: flow draw (device sampler + log rho +
-    flow forward), FP64 evaluation of g, block sums with q as the denominator."""
+def pareto_khat(logw):
+    """Shape of a generalized Pareto fit to the largest weights (Zhang & Stephens 2009 profile
+    estimate, with the weakly informative adjustment of Vehtari et al., "Pareto smoothed
+    importance sampling"): < 0.5 the weights have finite variance, > 0.7 the mean weight is not
+    a reliable estimate. logw: numpy array of log |w|."""
+    import numpy as np
+    lw = np.sort(logw[np.isfinite(logw)])
+    n = lw.size
+    M = int(min(0.2 * n, 3.0 * math.sqrt(n)))
+    if M < 10:
+        return float("nan")
+    top = np.exp(lw[-M:] - lw[-1])
+    x = top - math.exp(lw[-M - 1] - lw[-1])      # exceedances over the (M+1)-th largest, sorted
+    x = x[x > 0]
+    m = x.size
+    if m < 10:
+        return float("nan")
+    grid = 20 + int(math.sqrt(m))
+    theta = 1.0 / x[-1] + (1.0 - np.sqrt(grid / (np.arange(1, grid + 1) - 0.5))) / (3.0 * x[int(m / 4 + 0.5) - 1])
+    k = -np.mean(np.log1p(-theta[:, None] * x[None, :]), axis=1)
+    prof = m * (np.log(theta / k) + k - 1.0)
+    wts = 1.0 / np.exp(np.minimum(prof[None, :] - prof[:, None], 700.0)).sum(axis=1)
+    th = float(np.sum(theta * wts))
+    xi = float(np.mean(np.log1p(-th * x)))
+    return (m * xi + 10 * 0.5) / (m + 10)
+
+
+def tail_diagnostics(draw, n, chunk):
+    """Largest weight share and Pareto k-hat over n fresh samples of a proposal (untimed).
+    draw(B, offset) -> (log|w| tensor, sign tensor)."""
+    import numpy as np
+    lws = []
+    for s in range(0, n, chunk):
+        lw, _ = draw(min(chunk, n - s), s)
+        lws.append(lw.double().cpu().numpy())
+    lw = np.concatenate(lws)
+    lw = lw[np.isfinite(lw)]
+    w = np.exp(lw - lw.max())
+    return {"samples": int(lw.size), "max_weight_share": float(w.max() / w.sum()),
+            "pareto_khat": pareto_khat(lw)}
+
+
+# --------------------------------------------------------------------------------------
+def harmonic_leg(data, B, nblocks, W, seed, flags, pm):
+    """nblocks timed blocks of B samples from rho (device sampler -> evaluation -> block
+    sums), after W warm-up blocks. HIP events on the launch stream time every evaluation."""
     import torch
-    from pibronic_amd import _native, flow, learn
-    t0 = time.perf_counter()
-    fl = flow.Flow(data, layers=args.flow_layers, hidden=args.flow_hidden, seed=args.seed,
-                   gemm=args.flow_gemm)
-    fl.use_fused = bool(args.flow_fused)
-    hist = learn.train(fl, args.train_steps, args.train_batch, args.lr, args.seed + 1,
-                       log=lambda st: log(f"[bench] train step {st['step']} ess {st['ess_frac']:.3e}"),
-                       log_every=max(1, args.train_steps // 5))
+    from pibronic_amd import _native
+    ctx = data.context()
+    dev = data.device_buffers(B)
+    stream = torch.cuda.current_stream()
+    sums = torch.zeros((nblocks + W, _native.NUM_SUMS), dtype=torch.float64, device="cuda")
+    ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(nblocks)]
+    ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(nblocks)]
+
+    def step(i, timed):
+        ctx.sample_block(seed, i * B, B, dev["q"], None, stream)
+        if timed:
+            ev0[i - W].record(stream)
+        ctx.eval_block(dev["q"], B, flags, dev["out"], dev["out"].shape[1], None, stream)
+        if timed:
+            ev1[i - W].record(stream)
+        ctx.block_sums(dev["out"], dev["out"].shape[1], B, pm, data.delta_beta, sums[i], stream)
+
+    for i in range(W):
+        step(i, False)
     torch.cuda.synchronize()
-    train_s = time.perf_counter() - t0
-    # the timed draws: single-bf16 layers only where the one-launch layer runs
-    draw_prec = args.flow_draw if (args.flow_gemm == "bf16x3" and fl._fused_ok(args.flow_block, True)) else "bf16x3"
-    fl.draw_precision = draw_prec
-    B = args.flow_block
-    K, W = args.steps, args.warmup
+    t0 = time.perf_counter()
+    for i in range(W, W + nblocks):
+        step(i, True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    res = weight_summary(sums[W:].sum(dim=0).cpu().numpy(), elapsed)
+    res["ms_per_step"] = elapsed / nblocks * 1e3
+    res["block"] = B
+    res["steps"] = nblocks
+    res["eval_kernel_ms"] = sum(a.elapsed_time(b) for a, b in zip(ev0, ev1)) / nblocks
+    return res
+
+
+def eval_roofline(A, N, P, nq, pm, B, eval_ms, kernel, traffic=None, tdetail=None):
+    fl = algorithmic_flops(A, N, P, nq, pm)
+    achieved = fl["eval"] * B / (eval_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "hbm_GBps": (traffic / (eval_ms * 1e-3) / 1e9) if traffic else None,
+            "hbm_frac": (traffic / (eval_ms * 1e-3) / HBM_PEAK_BPS) if traffic else None,
+            "kernel": kernel, "launch_samples": B, "kernel_ms": eval_ms,
+            "flops_per_sample": fl["eval"], "flops_per_launch": fl["eval"] * B,
+            "algorithmic_bytes": B * N * P * 8 + B * (5 if pm else 3) * 8,
+            "traffic_detail": tdetail or "not profiled for this launch size",
+            "peak_note": "78.6 TFLOP/s: MI355X FP64 vector (= FP64 matrix) peak"}
+
+
+def timed_draws(fl, data, B, K, W, seed, variant, eval_events=True):
+    """K timed steps: flow draw + FP64 g + block sums with q as the denominator."""
+    import torch
+    from pibronic_amd import _native
     out = torch.empty((_native.NUM_OUT, B), dtype=torch.float64, device="cuda")
     sums = torch.zeros((K + W, _native.NUM_SUMS), dtype=torch.float64, device="cuda")
     ctx = data.context()
     stream = torch.cuda.current_stream()
-
-    e0 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    e1 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    l0 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    l1 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    mk = lambda: [torch.cuda.Event(enable_timing=True) for _ in range(K)]  # noqa: E731
+    d0, d1, l0, l1, g0, g1 = mk(), mk(), mk(), mk(), mk(), mk()
 
     def step(i, timed):
+        j = i - W
         if timed:
-            e0[i - W].record(stream)
-        R, logq = fl.draw(B, args.seed + 2, (1 << 40) + i * B, stream,
-                          layer_events=(l0[i - W], l1[i - W]) if timed else None)
+            d0[j].record(stream)
+        R, logq = fl.draw(B, seed, (1 << 40) + i * B, stream, layer_events=(l0[j], l1[j]) if timed else None)
         if timed:
-            e1[i - W].record(stream)
+            d1[j].record(stream)
+            g0[j].record(stream)
         data.evaluate_device(R, B, False, out, B, stream=stream)
+        if timed:
+            g1[j].record(stream)
         ctx.block_sums_q(out, B, B, False, 1.0, logq, sums[i], stream)
 
     for i in range(W):
@@ -177,42 +290,141 @@ def learned_leg(data, args, log):
         step(i, True)
     torch.cuda.synchronize()
     el = time.perf_counter() - t1
-    draw_ms = sum(a.elapsed_time(b) for a, b in zip(e0, e1)) / K
-    layers_ms = sum(a.elapsed_time(b) for a, b in zip(l0, l1)) / K
+    avg = lambda a, b: sum(x.elapsed_time(y) for x, y in zip(a, b)) / K  # noqa: E731
+    res = weight_summary(sums[W:].sum(dim=0).cpu().numpy(), el)
+    res.update(variant=variant, steps=K, samples_per_step=B, ms_per_step=el / K * 1e3,
+               flow_draw_ms=avg(d0, d1), layers_ms=avg(l0, l1), eval_kernel_ms=avg(g0, g1))
+    return res
+
+
+def learned_leg(data, args, log):
+    """Train the flow on-policy, then time K steps of the single-bf16 draws; the same timed
+    steps with FP32-accurate flow products are reported beside them."""
+    import torch
+    from pibronic_amd import flow, learn
+    t0 = time.perf_counter()
+    fl = flow.Flow(data, layers=args.flow_layers, hidden=args.flow_hidden, seed=args.seed, gemm="bf16x3")
+    hist = learn.train(fl, args.train_steps, args.train_batch, args.lr, args.seed + 1,
+                       log=lambda st: log(f"[bench] train step {st['step']} ess {st['ess_frac']:.3e}"),
+                       log_every=max(1, args.train_steps // 5))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    B, K, W = args.flow_block, args.steps, args.warmup
+    fl.draw_precision = "bf16" if fl._fused_ok(B, True) else "bf16x3"
+    main = timed_draws(fl, data, B, K, W, args.seed + 2, f"{fl.draw_precision} draws")
+    log(f"[bench] learned ({fl.draw_precision}): {K} steps, ess fraction {main['ess_fraction']:.4f}")
     H, hdn, D = fl.H, fl.hidden, fl.D
     mlp_flops = 2.0 * B * fl.layers * (H * hdn + hdn * hdn + hdn * D)
-    fused = fl.use_fused and fl._fused_ok(B, draw_prec == "bf16")
-    # the coupling layers (the learned step's largest cost) against the bf16 matrix cores:
-    # algorithmic flops are the dense products' 2 m n k; BF16x3 issues three bf16 products
-    # for each (single-bf16 draws one), so the matrix cores execute that multiple
-    layer_tflops = mlp_flops / (layers_ms * 1e-3) / 1e12
-    products = 1 if draw_prec == "bf16" else 3
-    layer_roofline = {"kernel": ("k_flow_layer (one launch per coupling layer)" if fused
-                                 else "k_gemm_bf16x3_pre + k_coupling per layer"),
-                      "bound": "mfma", "achieved": layer_tflops, "peak": BF16_PEAK_TFLOPS,
-                      "unit": "TFLOP/s", "frac": layer_tflops / BF16_PEAK_TFLOPS,
-                      "matrix_core_frac": products * layer_tflops / BF16_PEAK_TFLOPS,
-                      "layers_ms": layers_ms, "flops_per_draw": mlp_flops,
-                      "note": "HIP events around the coupling layers of each timed draw"}
-    res = weight_summary(sums[W:].sum(dim=0).cpu().numpy(), el)
-    res.update(train_seconds=train_s, train_steps=args.train_steps, train_batch=args.train_batch,
-               samples_per_step=B, layers=args.flow_layers, hidden=args.flow_hidden,
-               gemm=args.flow_gemm, draw_precision=draw_prec, flow_draw_ms=draw_ms,
-               flow_mlp_tflops=mlp_flops / (draw_ms * 1e-3) / 1e12,
-               flow_mlp_flops_per_draw=mlp_flops, roofline=layer_roofline,
-               train_final_batch_ess=hist[-1]["ess_frac"] if hist else None)
-    return res
+    tf = mlp_flops / (main["layers_ms"] * 1e-3) / 1e12
+    single = fl.draw_precision == "bf16"
+    main["roofline_layers"] = {
+        "kernel": "k_flow_layer_ws (one launch per coupling layer)" if single else "BF16x3 layer-by-layer",
+        "bound": "mfma", "achieved": tf, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": tf / BF16_PEAK_TFLOPS, "matrix_core_frac": (1 if single else 3) * tf / BF16_PEAK_TFLOPS,
+        "layers_ms": main["layers_ms"], "flops_per_draw": mlp_flops,
+        "note": "HIP events around the coupling layers of each timed draw"}
+    variants = []
+    if args.variants:
+        Kv = max(1, min(K, args.variant_steps))
+        fl.draw_precision = "bf16x3"
+        variants.append(timed_draws(fl, data, B, Kv, 1, args.seed + 3, "bf16x3 draws (FP32-accurate products)"))
+        fl.gemm = "f32"
+        variants.append(timed_draws(fl, data, B, Kv, 1, args.seed + 4, "fp32 draws (FP32 matrix-core products)"))
+        fl.gemm = "bf16x3"
+        for v in variants:
+            log(f"[bench] learned {v['variant']}: ess fraction {v['ess_fraction']:.4f}, "
+                f"{v['ess_per_s']:.3e} ESS/s")
+    fl.draw_precision = "bf16" if single else "bf16x3"
+    main.update(train_seconds=train_s, train_steps=args.train_steps, train_batch=args.train_batch,
+                layers=args.flow_layers, hidden=args.flow_hidden,
+                train_final_batch_ess=hist[-1]["ess_frac"] if hist else None,
+                variants=variants)
+    main["time_to_1e-3_z_error_incl_training_s"] = train_s + main["time_to_1e-3_z_error_s"]
+    return fl, main
+
+
+def flow_tails(fl, data, n, seed):
+    from pibronic_amd import flow as flow_mod
+
+    def draw(B, off):
+        R, lq = fl.draw(B, seed, (1 << 44) + off)
+        return flow_mod.log_weights(data, R, lq)
+    return tail_diagnostics(draw, n, 65536)
+
+
+def rho_tails(data, n, seed):
+    import torch
+    from pibronic_amd import flow as flow_mod
+
+    def draw(B, off):
+        R = torch.empty((B, data.modes, data.beads), dtype=torch.float64, device="cuda")
+        data.context().sample_block(seed, (1 << 44) + off, B, R)
+        return flow_mod.harmonic_log_weights(data, R)
+    return tail_diagnostics(draw, n, 65536)
+
+
+def setup_data(FS, cfg, B, blocks, seed, pm=False):
+    from pibronic_amd import pimc
+    data = (pimc.BoxDataPM if pm else pimc.BoxData).from_FileStructure(FS)
+    data.samples, data.beads, data.temperature = B * blocks, cfg["P"], cfg["T"]
+    data.block_size, data.blocks = B, blocks
+    data.seed = seed
+    data.preprocess()
+    return data
+
+
+def n_quad_pairs(data):
+    import numpy as np
+    return int(np.count_nonzero(np.abs(data.vib.quadratic).sum(axis=(2, 3))))
+
+
+def fp64_line(FS, name, cfg, args, cpu, log):
+    """The plain FP64 kernel on one workload: harmonic blocks of 1e5, samples*beads/s and the
+    evaluation kernel's roofline (traffic from the committed counter profile when there is one)."""
+    import torch
+    from pibronic_amd import pimc
+    A, N, P = cfg["A"], cfg["N"], cfg["P"]
+    B = args.block
+    data = setup_data(FS, cfg, B, args.fp64_steps + args.warmup, args.seed)
+    lanes = data.context().lanes_per_sample(B)
+    h = harmonic_leg(data, B, args.fp64_steps, args.warmup, args.seed, 0, False)
+    sbs = h["samples"] * P / h["seconds"]
+    kname = f"k_eval_fp64<{A}, {A}, false, false>" if A <= 8 else "k_eval_wide"
+    traffic, tdetail = profiled_traffic(os.path.join(ROOT, "profiles", f"k_eval_fp64_A{A}_N{N}_P{P}_B{B}.md"),
+                                        f"void pib::k_eval_fp64<{A}, {A}, false, false>")
+    log(f"[bench] FP64 {name}: {sbs:.3e} samples*beads/s, eval {h['eval_kernel_ms']:.3f} ms")
+    line = {"workload": f"{name}: A={A} N={N} P={P} T={cfg['T']:g}K block={B}, FP64, harmonic rho",
+            "samples_beads_per_s": sbs, "samples_per_s": h["samples_per_s"], "ms_per_step": h["ms_per_step"],
+            "steps": h["steps"], "lanes_per_sample": lanes, "eval_kernel_ms": h["eval_kernel_ms"],
+            "ess_fraction": h["ess_fraction"],
+            "roofline": eval_roofline(A, N, P, n_quad_pairs(data), False, B, h["eval_kernel_ms"], kname,
+                                      traffic, tdetail),
+            "cpu_baseline": cpu_line(cpu, sbs, f"the {name} model") if cpu else None}
+    # the reference-API path: block_compute (device sampling, evaluation, sums, and the copy of
+    # every block's per-sample results to the host arrays of BoxResult)
+    api = setup_data(FS, cfg, B, args.fp64_steps, args.seed + 1)
+    res = pimc.BoxResult(data=api)
+    pimc.block_compute(api, res)  # warm-up: context, buffers
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pimc.block_compute(api, res)
+    el = time.perf_counter() - t0
+    line["block_compute_api"] = {"samples_beads_per_s": B * args.fp64_steps * P / el,
+                                 "ms_per_block": el / args.fp64_steps * 1e3,
+                                 "note": "pimc.block_compute end to end, incl. the per-block device-to-host "
+                                         "copies into BoxResult"}
+    return line
 
 
 def batched_leg(FS, cfg, args, cpu, log):
     """Sixteen temperatures of one model (T_i = 100 * 5^(i/15) K, P_i = ceil(beta_i / tau0))
-    in one persistent launch per block, the block step replayed as a hipGraph."""
+    in one block step per call, the block step replayed as a hipGraph."""
     import numpy as np
     import torch
-    from pibronic_amd import _native, pimc, synthetic
+    from pibronic_amd import pimc, synthetic
     from pibronic_amd.batch import TemperatureBatch
     Ts, Ps, tau0 = synthetic.c3_temperatures()
-    B, K, W = args.block, args.steps, args.warmup
+    B, K, W = args.block, args.fp64_steps, args.warmup
     datas = []
     for T, P in zip(Ts, Ps):
         d = pimc.BoxData.from_FileStructure(FS)
@@ -230,7 +442,6 @@ def batched_leg(FS, cfg, args, cpu, log):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     sums = tb.block_sums()[W:W + K]
-    # evaluation launch alone (HIP events on the launch stream), for the roofline
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
@@ -239,30 +450,28 @@ def batched_leg(FS, cfg, args, cpu, log):
     e1.record(stream)
     torch.cuda.synchronize()
     eval_ms = e0.elapsed_time(e1) / 3
-    nq = int(np.count_nonzero(np.abs(datas[0].vib.quadratic).sum(axis=(2, 3))))
+    nq = n_quad_pairs(datas[0])
     flops = sum(algorithmic_flops(cfg["A"], cfg["N"], int(P), nq, False)["eval"] for P in Ps) * B
     errors = None
-    if args.fp32:  # FP32 vs FP64 on the last block's samples, per temperature
+    if args.fp32:
         from pibronic_amd.precision import precision_report
         errors = []
         for i, d in enumerate(datas):
             rep = precision_report(d, tb.q[i], B, pm=False)
             errors.append({"T": float(Ts[i]), "P": int(Ps[i]), "r_median_rel": rep["r"]["median_rel"],
-                           "r_max_rel": rep["r"]["max_rel"], "s_rho_max_rel": rep["s_rho"]["max_rel"],
-                           "z_shift_sigma": rep["z_shift_sigma"]})
+                           "r_max_abs_over_mean_abs": rep["r"]["max_abs_over_mean_abs"],
+                           "s_rho_max_rel": rep["s_rho"]["max_rel"], "z_shift_sigma": rep["z_shift_sigma"]})
     achieved = flops / (eval_ms * 1e-3) / 1e12
-    per_t = []
-    ess_tot = 0.0
+    per_t, ess_tot = [], 0.0
     for i in range(len(Ts)):
-        s = sums[:, i].sum(axis=0)
-        w = weight_summary(s, el)
+        w = weight_summary(sums[:, i].sum(axis=0), el)
         ess_tot += w["ess_per_s"] * el
         per_t.append({"T": float(Ts[i]), "P": int(Ps[i]), "ess_fraction": w["ess_fraction"],
                       "z_ratio": w["z_ratio"], "z_ratio_err": w["z_ratio_err"]})
     beads = int(np.sum(Ps))
-    line = {
-        "metric": "effective samples/sec at fixed Z-ratio std error; learned ρ vs harmonic ρ",
-        "value": ess_tot / el, "unit": "effective samples/s (harmonic rho, all temperatures)",
+    sbs = B * beads * K / el
+    return {
+        "metric": METRIC, "value": ess_tot / el, "unit": "effective samples/s (harmonic rho, all temperatures)",
         "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": el / K * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.fp32 else "f64", "data": "synthetic (seeded model, device Philox samples)",
@@ -272,148 +481,115 @@ def batched_leg(FS, cfg, args, cpu, log):
                    "samples_per_step": B * len(Ts), "schedule": tune["schedule"],
                    "value_from": "harmonic rho; block step (16 samplers, evaluation, 16 reductions) "
                                  "replayed as one hipGraph, evaluation schedule chosen by the autotuner"},
-        "samples_beads_per_s": B * beads * K / el,
-        "samples_per_s": B * len(Ts) * K / el,
-        "batched_eval_ms": eval_ms,
-        "tuned_batched_ms": tune["batched_ms"],
+        "samples_beads_per_s": sbs, "samples_per_s": B * len(Ts) * K / el,
+        "batched_eval_ms": eval_ms, "tuned_batched_ms": tune["batched_ms"],
         "sequential_eval_ms": tune["sequential_ms"],
         "batched_speedup_vs_sequential": tune["sequential_ms"] / tune["batched_ms"],
-        "temperatures": per_t,
-        "fp32_error_vs_fp64": errors,
+        "temperatures": per_t, "fp32_error_vs_fp64": errors,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "peak_note": "78.6 TFLOP/s is also the non-packed FP32 VALU rate (packed FP32 "
-                                  "is disabled in this build)",
                      "kernel": {"lanes": "k_eval_batched (persistent)", "wide": "k_eval_wide_batched (persistent)",
                                 "sequential": "k_eval_fp64, one launch per temperature in the graph step"
                                 }[tune["schedule"]["family"]],
                      "flops_per_launch": flops, "traffic_detail": "not profiled for this workload"},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu_line(cpu, sbs, f"the middle temperature (T={cpu['T']:.1f} K, P={cpu['P']})")
+        if cpu else None,
     }
-    if cpu is not None:
-        line["cpu_baseline"] = {
-            "value": cpu["samples_beads_per_s"], "unit": "samples*beads/s", "cores": cpu["cores"],
-            "kind": "port",
-            "sample": f"oracle at the middle temperature only (T={cpu['T']:.1f} K, P={cpu['P']}), "
-                      f"{cpu['samples']} samples in blocks of 500 over {cpu['seconds']:.1f} s, "
-                      f"one process per core, numpy {cpu['numpy']}",
-            "one_core_value": cpu["one_core"],
-            "gpu_over_cpu": (B * beads * K / el) / cpu["samples_beads_per_s"],
-        }
-    return line
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=153,
+                    help="timed learned steps of --flow-block samples (153 x 65536 >= 1e7 samples)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--block", type=int, default=100_000)
-    ap.add_argument("--lanes", type=int, default=0, help="lanes per sample (0 = automatic)")
-    ap.add_argument("--pm", type=int, default=0, help="1: block_compute_pm (g at beta +- dbeta)")
+    ap.add_argument("--block", type=int, default=100_000, help="harmonic block size")
+    ap.add_argument("--workload", default="A8_N16_P64_T300")
+    ap.add_argument("--fp64-workload", default="A4_N8_P64_T300",
+                    help="workload of the plain FP64 kernel's samples*beads/s line ('none' to skip)")
+    ap.add_argument("--fp64-steps", type=int, default=10, help="timed blocks of the FP64 line")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-cores", type=int, default=0, help="0 = min(16, os.cpu_count())")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
-    ap.add_argument("--ring-bf16x3", action="store_true",
-                    help="wide kernel with M and ring products as BF16x3 on the bf16 matrix cores "
-                         "(PIB_EVAL_BF16X3); the error against FP64 on the same samples is reported")
-    ap.add_argument("--wide", action="store_true",
-                    help="harmonic leg: the one-wave-per-sample FP64 kernel (PIB_EVAL_WIDE) at any A <= 16")
-    ap.add_argument("--fp32", action="store_true",
-                    help="reduced-precision kernel (PIB_EVAL_FP32); the error against FP64 on the "
-                         "same samples is reported")
-    ap.add_argument("--workload", default="A4_N8_P64_T300")
+    ap.add_argument("--no-variants", dest="variants", action="store_false",
+                    help="skip the FP32-accurate flow variants")
+    ap.add_argument("--variant-steps", type=int, default=50)
+    ap.add_argument("--tail-samples", type=int, default=1 << 22,
+                    help="fresh samples per proposal for the tail diagnostics (0 to skip)")
+    ap.add_argument("--fp32", action="store_true", help="batched workload: the FP32 kernel")
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--train-batch", type=int, default=2048)
     ap.add_argument("--flow-block", type=int, default=65536)
     ap.add_argument("--flow-layers", type=int, default=8)
     ap.add_argument("--flow-hidden", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--flow-gemm", default="bf16x3", choices=["f32", "bf16x3"],
-                    help="precision of the flow's dense layers (FP32 MFMA or BF16x3 split)")
-    ap.add_argument("--flow-fused", type=int, default=1,
-                    help="0: layer-by-layer products in the draws instead of one launch per coupling layer")
-    ap.add_argument("--flow-draw", default="bf16", choices=["bf16x3", "bf16"],
-                    help="precision of the timed draws' one-launch layers (bf16: single bf16 products; "
-                         "log q stays exact for the map applied); training is BF16x3 either way")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
-                    help="after the timed region, copy a buffer of this many MiB 3 times "
+                    help="after the timed regions, copy a buffer of this many MiB 3 times "
                          "(calibrates profiler byte counters against a known transfer)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not "
-                                   "supported (samples shard trivially; see DESIGN.md)",
+                                   "supported (samples shard trivially, one process per GPU)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    import numpy as np
-    from pibronic_amd import pimc, synthetic, _native
+    from pibronic_amd import synthetic
     name = args.workload
     cfg = synthetic.WORKLOADS[name]
-    A, N, P, T = cfg["A"], cfg["N"], cfg["P"], cfg["T"]
-    B = args.block
-    pm = bool(args.pm)
     tmp = tempfile.mkdtemp(prefix="pib_bench_")
     FS = synthetic.write_workload(tmp, name)
+    cores = host_cores()[0]
 
-    batched = P is None
-    cpu = None
-    if not args.no_cpu:
-        cores = args.cpu_cores or min(16, os.cpu_count() or 1)
-        if batched:
+    if cfg["P"] is None:  # the sixteen-temperature batch
+        cpu = None
+        if not args.no_cpu:
             Ts, Ps, _ = synthetic.c3_temperatures()
-            T, P = float(Ts[8]), int(Ps[8])
-        cpu = cpu_baseline(FS.path_vib_model, FS.path_rho_model, T, P, args.cpu_seconds, cores)
-        cpu.update(cores=cores, T=T, P=P)
-        log(f"[bench] cpu baseline {cpu['samples_beads_per_s']:.3e} samples*beads/s on {cores} cores")
-
-    if batched:
+            cpu = cpu_baseline(FS.path_vib_model, FS.path_rho_model, float(Ts[8]), int(Ps[8]),
+                               args.cpu_seconds, cores)
         print(json.dumps(batched_leg(FS, cfg, args, cpu, log)))
         return
 
-    import torch
-    data = (pimc.BoxDataPM if pm else pimc.BoxData).from_FileStructure(FS)
-    data.samples, data.beads, data.temperature = B * (args.steps + args.warmup), P, T
-    data.block_size, data.blocks = B, args.steps + args.warmup
-    data.seed = args.seed
-    data.preprocess()
-    ctx = data.context()
-    if args.lanes:
-        ctx.set_lanes_per_sample(args.lanes)
-    lanes = ctx.lanes_per_sample(B)
-    dev = data.device_buffers(B)
-    stream = torch.cuda.current_stream()
-    K, W = args.steps, args.warmup
-    sums = torch.zeros((K + W, _native.NUM_SUMS), dtype=torch.float64, device="cuda")
-    flags = ((_native.EVAL_PM if pm else 0) | (_native.EVAL_FP32 if args.fp32 else 0)
-             | (_native.EVAL_BF16X3 if args.ring_bf16x3 else 0) | (_native.EVAL_WIDE if args.wide else 0))
-    ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    fp64_name = None if args.fp64_workload == "none" else args.fp64_workload
+    FS64 = synthetic.write_workload(tempfile.mkdtemp(prefix="pib_bench64_"), fp64_name) if fp64_name else None
+    cpu = cpu64 = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(FS.path_vib_model, FS.path_rho_model, cfg["T"], cfg["P"], args.cpu_seconds, cores)
+        log(f"[bench] cpu baseline {name}: {cpu['samples_beads_per_s']:.3e} samples*beads/s on {cores} cores")
+        if FS64:
+            c64 = synthetic.WORKLOADS[fp64_name]
+            cpu64 = cpu_baseline(FS64.path_vib_model, FS64.path_rho_model, c64["T"], c64["P"],
+                                 args.cpu_seconds, cores)
+            log(f"[bench] cpu baseline {fp64_name}: {cpu64['samples_beads_per_s']:.3e} samples*beads/s")
 
-    def step(i, timed):
-        ctx.sample_block(args.seed, i * B, B, dev["q"], None, stream)
-        if timed:
-            ev0[i - W].record(stream)
-        ctx.eval_block(dev["q"], B, flags, dev["out"], dev["out"].shape[1], None, stream)
-        if timed:
-            ev1[i - W].record(stream)
-        ctx.block_sums(dev["out"], dev["out"].shape[1], B, pm, data.delta_beta, sums[i], stream)
+    import torch  # noqa: F401  (the GPU is first touched below)
+    A, N, P, T = cfg["A"], cfg["N"], cfg["P"], cfg["T"]
+    B_l, K, W = args.flow_block, args.steps, args.warmup
+    # harmonic rho on the same model and at least as many samples as the timed learned steps
+    nblocks = max(1, -(-K * B_l // args.block))
+    data = setup_data(FS, cfg, args.block, nblocks + W, args.seed)
+    harm = harmonic_leg(data, args.block, nblocks, W, args.seed, 0, False)
+    log(f"[bench] harmonic {name}: {nblocks} blocks, ess fraction {harm['ess_fraction']:.3e}")
+    nq = n_quad_pairs(data)
+    kname = f"k_eval_fp64<{A}, {A}, false, false>" if A <= 8 else "k_eval_wide"
 
-    for i in range(W):
-        step(i, False)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(W, W + K):
-        step(i, True)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    log(f"[bench] harmonic: {K} steps of {B} samples in {elapsed:.4f} s")
+    learned = fl = None
+    if not args.no_learned and P % 64 == 0:
+        fl, learned = learned_leg(data, args, log)
+    elif not args.no_learned:
+        log(f"[bench] learned leg skipped: P = {P} is not a multiple of 64 (the flow's whitening tiles)")
+
+    tails = None
+    if args.tail_samples:
+        tails = {"harmonic": rho_tails(data, args.tail_samples, args.seed + 7)}
+        if fl is not None:
+            tails["learned"] = flow_tails(fl, data, args.tail_samples, args.seed + 8)
+
+    fp64 = fp64_line(FS64, fp64_name, synthetic.WORKLOADS[fp64_name], args, cpu64, log) if FS64 else None
 
     if args.calib_copy_mib:
         nel = args.calib_copy_mib * (1 << 20) // 8
@@ -424,82 +600,50 @@ def main():
         torch.cuda.synchronize()
         del src_t, dst_t
 
-    harm = weight_summary(sums[W:].sum(dim=0).cpu().numpy(), elapsed)
-    harm["ms_per_step"] = elapsed / K * 1e3
-    n = harm["samples"]
-    eval_ms = sum(a.elapsed_time(b) for a, b in zip(ev0, ev1)) / K
-    nq = int(np.count_nonzero(np.abs(data.vib.quadratic).sum(axis=(2, 3))))
-    fl = algorithmic_flops(A, N, P, nq, pm)
-    achieved = fl["eval"] * B / (eval_ms * 1e-3) / 1e12
-    traffic, tdetail = None, "profile only for the default workload"
-    fp32_err = None
-    if args.fp32 or args.ring_bf16x3:
-        from pibronic_amd.precision import precision_report
-        fp32_err = precision_report(data, dev["q"], B, pm=pm,
-                                    variant="fp32" if args.fp32 else "bf16x3")
-    if not pm and B == 100_000 and not (args.fp32 or args.ring_bf16x3):
-        traffic, tdetail = profiled_traffic(
-            os.path.join(ROOT, "profiles", f"k_eval_fp64_A{A}_N{N}_P{P}_B{B}.md"),
-            f"void pib::k_eval_fp64<{A}, {A}, false, false>")
-
-    learned = None
-    if not args.no_learned and not pm and P % 64:
-        # the flow's normal-mode whitening works in MFMA tiles of 64 beads
-        log(f"[bench] learned leg skipped: P = {P} is not a multiple of 64")
-    elif not args.no_learned and not pm:
-        learned = learned_leg(data, args, log)
-        log(f"[bench] learned: ess fraction {learned['ess_fraction']:.3e}, "
-            f"{learned['ess_per_s']:.3e} ESS/s (harmonic {harm['ess_per_s']:.3e})")
-
-    value = learned["ess_per_s"] if learned else harm["ess_per_s"]
+    harm_sbs = harm["samples"] * P / harm["seconds"]
+    if learned:
+        roof = eval_roofline(A, N, P, nq, False, B_l, learned["eval_kernel_ms"], kname,
+                             *profiled_traffic(os.path.join(ROOT, "profiles", f"k_eval_fp64_A{A}_N{N}_P{P}_B{B_l}.md"),
+                                               f"void pib::k_eval_fp64<{A}, {A}, false, false>"))
+        roof["step_note"] = "the FP64 evaluation of g inside the timed learned step (its largest kernel)"
+        value = learned["ess_per_s"]
+    else:
+        roof = eval_roofline(A, N, P, nq, False, args.block, harm["eval_kernel_ms"], kname)
+        value = harm["ess_per_s"]
     line = {
-        "metric": "effective samples/sec at fixed Z-ratio std error; learned ρ vs harmonic ρ",
+        "metric": METRIC,
         "value": value,
         "unit": "effective samples/s" + ("" if learned else " (harmonic rho)"),
-        "n_gpus": 1, "steps": K, "warmup": W,
-        # the timed step of `value`: a learned draw + FP64 g + sums (or the harmonic step)
-        "ms_per_step": (learned["seconds"] if learned else elapsed) / K * 1e3,
+        "n_gpus": 1, "steps": learned["steps"] if learned else harm["steps"], "warmup": W,
+        "ms_per_step": learned["ms_per_step"] if learned else harm["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if args.fp32 else ("f64 M, bf16x3/f32 ring" if args.ring_bf16x3 else "f64"),
+        "dtype": ("bf16 flow draws (single-bf16 matrix-core products, FP32 accumulation and flow "
+                  "state), f64 g and weights") if learned else "f64",
         "data": "synthetic (seeded model, device Philox samples)",
-        "config": {"workload": f"{name}: A={A} N={N} P={P} T={T:g}K block={B}"
-                               + (" +-dbeta" if pm else ""),
-                   "samples_per_step": learned["samples_per_step"] if learned else B,
-                   "harmonic_block": B, "lanes_per_sample": lanes,
-                   "value_from": ("learned proposal (ms_per_step: one draw of samples_per_step, FP64 g and "
-                                  "sums); samples_beads_per_s, samples_per_s: the harmonic FP64 step")
-                                 if learned else "harmonic rho",
-                   "flow_draw": (f"{learned['draw_precision']} coupling-layer products in the timed draws "
-                                 "(trained with BF16x3); g in FP64; log q exact for the map applied")
-                                if learned else None},
-        "samples_beads_per_s": n * P / elapsed,
-        "samples_per_s": harm["samples_per_s"],
-        "eval_kernel_ms": eval_ms,
-        "harmonic": harm,
+        "config": {"workload": f"{name}: A={A} N={N} P={P} T={T:g}K",
+                   "samples_per_step": learned["samples_per_step"] if learned else args.block,
+                   "flow": (f"{args.flow_layers} affine-coupling layers, hidden {args.flow_hidden}, "
+                            f"D = N*P = {N * P}, trained on-policy with BF16x3 products "
+                            f"({args.train_steps} steps of {args.train_batch})") if learned else None,
+                   "harmonic_block": args.block,
+                   "value_from": ("learned proposal: Kish ESS over the timed samples / timed wall time "
+                                  "(ms_per_step: one draw of samples_per_step + FP64 g + sums)")
+                                 if learned else "harmonic rho"},
+        "evaluated_samples": learned["samples"] if learned else harm["samples"],
         "learned": learned,
+        "harmonic": harm,
         "learned_over_harmonic_ess_per_s": (learned["ess_per_s"] / harm["ess_per_s"]) if learned else None,
-        "reduced_precision_error_vs_fp64": fp32_err,
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "hbm_GBps": (traffic / (eval_ms * 1e-3) / 1e9) if (traffic and eval_ms) else None,
-                     "hbm_frac": (traffic / (eval_ms * 1e-3) / HBM_PEAK_BPS) if (traffic and eval_ms) else None,
-                     "kernel": ("k_eval_fp32" if args.fp32 else "k_eval_wide" if (A > 8 or args.ring_bf16x3 or args.wide)
-                                else "k_eval_fp64"), "flops_per_sample": fl["eval"],
-                     "algorithmic_bytes": B * N * P * 8 + B * (5 if pm else 3) * 8,
-                     "traffic_detail": tdetail},
-        "cpu_baseline": None,
+        "learned_over_harmonic_incl_training": (
+            harm["time_to_1e-3_z_error_s"] / learned["time_to_1e-3_z_error_incl_training_s"]) if learned else None,
+        "tail_diagnostics": tails,
+        "samples_beads_per_s": harm_sbs,
+        "samples_beads_per_s_note": f"harmonic FP64 step on {name} (blocks of {args.block})",
+        "roofline": roof,
+        "cpu_baseline": cpu_line(cpu, harm_sbs, f"the {name} model") if cpu else None,
+        "fp64_kernel": fp64,
     }
-    if cpu is not None:
-        line["cpu_baseline"] = {
-            "value": cpu["samples_beads_per_s"], "unit": "samples*beads/s",
-            "cores": cpu["cores"], "kind": "port",
-            "sample": f"oracle (FP64 NumPy restatement, per-(sample,bead) dot loop) on the same "
-                      f"model, {cpu['samples']} samples in blocks of 500 over {cpu['seconds']:.1f} s, "
-                      f"one process per core, numpy {cpu['numpy']}",
-            "one_core_value": cpu["one_core"],
-            "effective_samples_per_s": cpu["samples_beads_per_s"] / P * cpu["ess_frac"],
-            "gpu_over_cpu": (n * P / elapsed) / cpu["samples_beads_per_s"],
-        }
+    if cpu and learned:
+        line["cpu_baseline"]["effective_samples_per_s"] = cpu["samples_beads_per_s"] / P * cpu["ess_frac"]
     print(json.dumps(line))
 
 

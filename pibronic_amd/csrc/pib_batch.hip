@@ -47,7 +47,9 @@ struct pib_batch {
   std::vector<pib_ctx*> ctx;
   std::vector<uint64_t> gens;  // each context's setter generation when the batch copied its pointers
   std::vector<EvalArgs> args;  // per problem; q/out point into the batch buffers
+  LaneProbs lprobs{};          // the lanes family's kernel-argument copy of args
   bool lane_ok = false, wide_ok = false;
+  bool seq_lanes = true;  // every problem has a compiled lanes kernel (one launch per problem)
   int A = 0, AR = 0;
   // schedule
   int family = 0;  // PIB_BATCH_LANES or PIB_BATCH_WIDE
@@ -125,6 +127,10 @@ int build_schedule(pib_batch* b) {
     }
   }
   std::stable_sort(tiles.begin(), tiles.end(), [](const T& x, const T& y) { return x.cost > y.cost; });
+  if (b->lane_ok) {
+    b->lprobs.n = b->n;
+    for (int i = 0; i < b->n; ++i) b->lprobs.p[i] = lane_prob(b->args[i]);
+  }
   std::vector<BatchTile> ht(tiles.size());
   for (size_t k = 0; k < tiles.size(); ++k) ht[k] = tiles[k].t;
   if (ht.size() > (size_t)0x7fffffff) return bfail(b, PIB_ERR_ARG, "batch: too many tiles");
@@ -144,7 +150,7 @@ hipError_t launch_batch_eval(pib_batch* b, hipStream_t s) {
     for (int i = 0; i < b->n; ++i) {
       const EvalArgs& a = b->args[i];
       hipError_t e = (b->flags & PIB_EVAL_FP32) ? launch_eval_fp32(a.A, a.AR, b->pm, a, s)
-                     : b->lane_ok               ? launch_eval_fp64(a.A, a.AR, b->pm, a, s)
+                     : b->seq_lanes             ? launch_eval_fp64(a.A, a.AR, b->pm, a, s)
                                                 : launch_eval_wide(b->pm, a, s);
       if (e != hipSuccess) return e;
     }
@@ -157,7 +163,7 @@ hipError_t launch_batch_eval(pib_batch* b, hipStream_t s) {
   const BatchTile* pt = (const BatchTile*)b->d_tiles.p;
   int* cnt = (int*)b->d_counter.p;
   if (b->family == PIB_BATCH_LANES)
-    return launch_eval_batched(b->A, b->AR, b->pm, pa, pt, b->ntiles, cnt, b->grid_waves, s);
+    return launch_eval_batched(b->A, b->AR, b->pm, b->lprobs, pt, b->ntiles, cnt, b->grid_waves, s);
   return launch_eval_wide_batched(b->pm, pa, pt, b->ntiles, cnt, b->grid_waves, s);
 }
 
@@ -280,13 +286,16 @@ int pib_batch_create(int n, pib_ctx* const* ctxs, int64_t B, int flags, double d
       return bfail(b, PIB_ERR_STATE, "batch_create: problem " + std::to_string(i) + ": " + c->err);
     b->gens[i] = c->gen;
     const int A = c->A, AR = b->args[i].AR;
+    if (wide) b->seq_lanes = false;
     if (wide || (i > 0 && (A != b->A || AR != b->AR))) b->lane_ok = false;
     if (i == 0) { b->A = A; b->AR = AR; }
     if (A > kWideMaxStates || AR > kWideMaxStates) b->wide_ok = false;
     b->lanes[i] = auto_lanes(c, B);
   }
   if (flags & PIB_EVAL_FP32) b->wide_ok = false;  // FP32 runs as one launch per problem
-  if (!b->lane_ok && !b->wide_ok) return bfail(b, PIB_ERR_UNSUPPORTED, "batch_create: no kernel family fits every problem");
+  if (n > kBatchLaunchProbs) b->lane_ok = false;   // the lanes kernel takes its problems by value
+  if (!b->lane_ok && !b->wide_ok && !b->seq_lanes)
+    return bfail(b, PIB_ERR_UNSUPPORTED, "batch_create: no kernel family fits every problem");
   if (hipSetDevice(b->device) != hipSuccess) return bfail(b, PIB_ERR_HIP, "batch_create: set device");
   b->nparts = (int)std::max<int64_t>(1, std::min<int64_t>((B + 255) / 256, 1024));
   int r;
@@ -301,7 +310,10 @@ int pib_batch_create(int n, pib_ctx* const* ctxs, int64_t B, int flags, double d
     b->args[i].out_stride = B;
     b->args[i].dbg = nullptr;
   }
-  b->family = (flags & PIB_EVAL_FP32) ? PIB_BATCH_SEQUENTIAL : b->lane_ok ? PIB_BATCH_LANES : PIB_BATCH_WIDE;
+  b->family = (flags & PIB_EVAL_FP32) ? PIB_BATCH_SEQUENTIAL
+              : b->lane_ok            ? PIB_BATCH_LANES
+              : b->wide_ok            ? PIB_BATCH_WIDE
+                                      : PIB_BATCH_SEQUENTIAL;
   b->tile_waves = 4;
   b->grid_waves = b->num_cu * 8;
   PIB_BHIP(b, hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
@@ -471,7 +483,7 @@ int pib_batch_tune(pib_batch* b, uint64_t seed, double* best_ms, double* sequent
       if (probe.ms > 1.5 * best.ms) continue;
     }
     if (fam == PIB_BATCH_SEQUENTIAL) {  // per-problem launches: only the lanes matter
-      if (!b->lane_ok) {
+      if (!b->seq_lanes) {
         if ((rc = run(cur))) return rc;
       } else {
         for (int kt : {2, 4, 8, 16}) {

@@ -42,6 +42,17 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 ctr, uint32_t k0, uint32_t 
   return ctr;
 }
 
+// Read-only tables through the constant address space: a wave-uniform address then becomes a
+// scalar load even where the kernel also stores to global memory (the compiler cannot prove
+// that a store does not clobber a generic pointer's target, and a persistent loop stores
+// between reads). Only for data no kernel writes during the launch.
+template <class T>
+using const_as = const T __attribute__((address_space(4)));
+template <class T>
+__device__ __forceinline__ const_as<T>* as_const(const T* p) {
+  return (const_as<T>*)p;
+}
+
 // 53-bit uniform in [0, 1) from two 32-bit words.
 __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
   const uint64_t m = ((uint64_t)(hi >> 5) << 26) | (uint64_t)(lo >> 6);

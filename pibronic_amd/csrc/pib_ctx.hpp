@@ -132,10 +132,13 @@ inline bool finite_all(const double* v, size_t n) {
 }
 
 inline int auto_lanes(const pib_ctx* c, int64_t B) {
-  // enough lanes in flight for ~24 waves per SIMD over the launch (measured on MI355X at
-  // A=4, N=8, P=64: 16 lanes per sample for blocks of 1e5 samples is the fastest split);
-  // never more lanes per sample than beads, so that each lane owns at least one bead
-  const int64_t target = (int64_t)c->num_cu * 4 * kWave * 24;
+  // the fewest lanes per sample that give ~16 waves per SIMD over the launch; each doubling
+  // adds a level to the cross-lane product tree. Measured on MI355X (eval ms at 4/8/16/32 lanes):
+  // A=4, N=8, P=64, 1e5 samples 0.95/0.48/0.31/0.34, 65,536 samples 0.61/0.31/0.21/0.22;
+  // A=8, N=16, P=64, 1e5 samples 2.70/2.44/2.33/2.88, 65,536 samples 1.74/1.56/1.51/1.90 (16 in
+  // every case; a target of 24 waves per SIMD picked 32 at 65,536). Never more lanes per sample
+  // than beads, so that each lane owns at least one bead.
+  const int64_t target = (int64_t)c->num_cu * 4 * kWave * 16;
   int maxg = 1;
   while (maxg < 64 && maxg * 2 <= c->P) maxg *= 2;
   int g = 1;

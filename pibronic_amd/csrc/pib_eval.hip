@@ -34,18 +34,18 @@ namespace pib {
 //
 // The reference forms M = U diag(exp(-tau lambda)) U^T from eigh(V) (pimc.py:1167-1187); M is
 // the only consumer of the decomposition, so it is computed directly: scaling and squaring of
-// the degree-8 Taylor polynomial T_8, evaluated by Horner's rule. X is scaled by 2^-s so that
-// |X|_F 2^-s <= theta_8 = 0.049, below the root of |y|^8 / 9! = u (the truncation term
-// relative to y, i.e. a backward error under the unit roundoff); the Frobenius norm bounds
-// the spectral radius, and for a symmetric X the 2-norm error of a polynomial in X is its
-// largest error over the eigenvalues. T_8 is then squared s times. Every intermediate is a
-// polynomial in X, hence symmetric, so products form the upper triangle only. s differs
-// between lanes; the squaring loop runs the wave's largest count. Degree 8 minimises
-// products plus squarings for Horner evaluation (one product per degree against one squaring
-// per halving of theta).
-// ---------------------------------------------------------------------------------------
-constexpr double kLog2Theta8 = -4.3511;  // log2(0.049)
-
+// the degree-12 Taylor polynomial T_12, evaluated by Paterson-Stockmeyer in X^3:
+//   T_12 = ((B3 X^3 + B2) X^3 + B1) X^3 + B0,  B_n = c_3n I + c_3n+1 X + c_3n+2 X^2 (n < 3),
+//   B3 = c9 I + c10 X + c11 X^2 + c12 X^3,
+// five products (X^2, X^3, three Horner steps) against seven for Horner's rule at degree 8.
+// X is scaled by 2^-s so that alpha 2^-s <= theta_12 = 0.2996 (Sastre et al. 2015: the backward
+// error of T_12 below the unit roundoff), alpha = min(|X|_F, |X^3|_F^(1/3)) >= the spectral
+// radius (X^2 and X^3 are formed before scaling; scaling by powers of two is exact). For a
+// symmetric X the 2-norm error of a polynomial in X is its largest error over the eigenvalues.
+// T_12 is then squared s times: on the workload matrices 2.0 squarings per wave at A = 4
+// instead of 5.0 at degree 8 with the Frobenius bound (7.0 instead of 12.0 products per bead).
+// Every intermediate is a polynomial in X, hence symmetric, so products form the upper
+// triangle only. s differs between lanes; the squaring loop runs the wave's largest count.
 // Z = X Y + d I for commuting symmetric X, Y (Z symmetric; Z must not alias X or Y)
 template <int A>
 __device__ __forceinline__ void sym_prod_lane(const double (&X)[A][A], const double (&Y)[A][A], double d,
@@ -62,8 +62,91 @@ __device__ __forceinline__ void sym_prod_lane(const double (&X)[A][A], const dou
     }
 }
 
+// Z = T Y + c0 I + c1 X + c2 X2 for commuting symmetric T, Y, X, X2 (Z symmetric, no aliasing)
 template <int A>
-__device__ __forceinline__ void expm_lane(double (&X)[A][A], double (&T)[A][A]) {
+__device__ __forceinline__ void sym_prod_add(const double (&T)[A][A], const double (&Y)[A][A], double c0,
+                                             double c1, const double (&X)[A][A], double c2,
+                                             const double (&X2)[A][A], double (&Z)[A][A]) {
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = i; k < A; ++k) {
+      double s = fma(c2, X2[i][k], c1 * X[i][k]);
+      if (i == k) s += c0;
+#pragma unroll
+      for (int c = 0; c < A; ++c) s = fma(T[i][c], Y[c][k], s);
+      Z[i][k] = s;
+      Z[k][i] = s;
+    }
+}
+
+constexpr double kLog2Theta12 = -1.738865;  // log2(0.2996)
+
+template <int A>
+__device__ __forceinline__ void expm_lane_ps12(double (&X)[A][A], double (&T)[A][A]) {
+  double X2[A][A], X3[A][A];
+  sym_prod_lane<A>(X, X, 0.0, X2);
+  sym_prod_lane<A>(X2, X, 0.0, X3);
+  double nf = 0.0, n3 = 0.0;
+#pragma unroll
+  for (int i = 0; i < A; ++i) {
+    nf = fma(X[i][i], X[i][i], nf);
+    n3 = fma(X3[i][i], X3[i][i], n3);
+#pragma unroll
+    for (int k = i + 1; k < A; ++k) {
+      nf = fma(2.0 * X[i][k], X[i][k], nf);
+      n3 = fma(2.0 * X3[i][k], X3[i][k], n3);
+    }
+  }
+  const double la = fmin(0.5 * log2_coarse(nf), log2_coarse(n3) / 6.0);
+  const double sd = la - kLog2Theta12;
+  int s = 0;
+  if (sd > 0.0) s = sd < 64.0 ? (int)ceil(sd) : 64;
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = i; k < A; ++k) {
+      X[i][k] = ldexp(X[i][k], -s);
+      X[k][i] = X[i][k];
+      X2[i][k] = ldexp(X2[i][k], -2 * s);
+      X2[k][i] = X2[i][k];
+      X3[i][k] = ldexp(X3[i][k], -3 * s);
+      X3[k][i] = X3[i][k];
+    }
+  constexpr double c[13] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040,
+                            1.0 / 40320, 1.0 / 362880, 1.0 / 3628800, 1.0 / 39916800, 1.0 / 479001600};
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = 0; k < A; ++k) {
+      double v = fma(c[12], X3[i][k], fma(c[11], X2[i][k], c[10] * X[i][k]));
+      T[i][k] = (i == k) ? v + c[9] : v;
+    }
+#pragma unroll
+  for (int n = 2; n >= 0; --n) {
+    double Z[A][A];
+    sym_prod_add<A>(T, X3, c[3 * n], c[3 * n + 1], X, c[3 * n + 2], X2, Z);
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int k = 0; k < A; ++k) T[i][k] = Z[i][k];
+  }
+  for (int k = 0; k < s; ++k) {
+    double Z[A][A];
+    sym_prod_lane<A>(T, T, 0.0, Z);
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int j = 0; j < A; ++j) T[i][j] = Z[i][j];
+  }
+}
+
+// The degree-8 Taylor polynomial by Horner's rule (seven products), alpha = |X|_F and
+// theta_8 = 0.049 (below the root of |y|^8 / 9! = u): three matrices live instead of five.
+constexpr double kLog2Theta8 = -4.3511;  // log2(0.049)
+
+template <int A>
+__device__ __forceinline__ void expm_lane_h8(double (&X)[A][A], double (&T)[A][A]) {
   double nf = 0.0;
 #pragma unroll
   for (int i = 0; i < A; ++i) {
@@ -106,15 +189,31 @@ __device__ __forceinline__ void expm_lane(double (&X)[A][A], double (&T)[A][A]) 
   }
 }
 
+// Which scheme each surface count uses (index A): Paterson-Stockmeyer degree 12 where its two
+// extra live matrices cost no occupancy that the saved products do not buy back. Measured on
+// MI355X (eval ms, Horner-8 against PS-12, same GPU): A=4, N=8, P=64, 65,536 samples 0.209
+// against 0.197; A=6, N=12, P=128, 1e5 samples 1.70 against 2.16 (PS-12 needs 318 VGPRs there:
+// one wave per SIMD instead of two); A=8, N=16, P=64, 65,536 samples 1.51 against 1.57 (both
+// at one wave per SIMD, PS-12 spilling).
+constexpr bool kExpmPS12[9] = {true, true, true, true, true, false, false, false, false};
+
+template <int A>
+__device__ __forceinline__ void expm_lane(double (&X)[A][A], double (&T)[A][A]) {
+  if constexpr (kExpmPS12[A])
+    expm_lane_ps12<A>(X, T);
+  else
+    expm_lane_h8<A>(X, T);
+}
+
 template <int A, int AR, bool PM, bool QD>
-__device__ __forceinline__ void mode_terms(const double* __restrict__ tab, const double* __restrict__ qs, int N,
+__device__ __forceinline__ void mode_terms(const_as<double>* tab, const double* __restrict__ qs, int N,
                                            int P, int p, int pn, double& tr, double& tv, double& tp,
                                            double& tm, double (&er)[AR], double (&ev)[A], double (&ep)[A],
                                            double (&em)[A], double (&v)[A][A]) {
   constexpr ModeRec R(A, AR);
   // one mode's terms from its coordinates x0 (bead p) and xn (bead pn)
   auto mode = [&](int j, double x0, double xn) {
-    const double* __restrict__ t = tab + j * R.len();
+    const_as<double>* t = tab + j * R.len();
     const double sx = x0 + xn, dx = x0 - xn;
     const double q2s = fma(x0, x0, xn * xn), d2 = dx * dx;
     tr = fma(t[1], d2, fma(t[0], q2s, tr));
@@ -193,19 +292,19 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
   const int64_t NP = (int64_t)N * P;
   const double* __restrict__ qs = args.q + (live ? sample : 0) * NP;
 
-  // read-only constants: restrict-qualified so the compiler can keep them in scalar loads
+  // read-only constants through the constant address space: scalar loads
   struct RO {
-    const double* __restrict__ k2;
-    const double* __restrict__ lpref;
+    const_as<double>* k2;
+    const_as<double>* lpref;
   };
-  const RO sr{args.set[SET_RHO].k2, args.set[SET_RHO].lpref};
-  const RO sv{args.set[SET_VIB].k2, args.set[SET_VIB].lpref};
-  const RO sp{args.set[SET_VIB_PLUS].k2, args.set[SET_VIB_PLUS].lpref};
-  const RO sm{args.set[SET_VIB_MINUS].k2, args.set[SET_VIB_MINUS].lpref};
-  const double* __restrict__ Ev = args.energy;
+  const RO sr{as_const(args.set[SET_RHO].k2), as_const(args.set[SET_RHO].lpref)};
+  const RO sv{as_const(args.set[SET_VIB].k2), as_const(args.set[SET_VIB].lpref)};
+  const RO sp{as_const(args.set[SET_VIB_PLUS].k2), as_const(args.set[SET_VIB_PLUS].lpref)};
+  const RO sm{as_const(args.set[SET_VIB_MINUS].k2), as_const(args.set[SET_VIB_MINUS].lpref)};
+  const_as<double>* Ev = as_const(args.energy);
   const double* __restrict__ Qd = args.qdiag;
-  const int* __restrict__ pairs = args.pair;
-  const double* __restrict__ qpair = args.qpair;
+  const_as<int>* pairs = as_const(args.pair);
+  const_as<double>* qpair = as_const(args.qpair);
 
   double Ng[A][A], Np[A][A], Nm[A][A];
 #pragma unroll
@@ -240,14 +339,14 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
       for (int c = 0; c <= a; ++c) v[a][c] = 0.0;
     }
     if (Qd)
-      mode_terms<A, AR, PM, true>(args.mode_tab, qs, N, P, p, pn, tr, tv, tp, tm, er, ev, ep, em, v);
+      mode_terms<A, AR, PM, true>(as_const(args.mode_tab), qs, N, P, p, pn, tr, tv, tp, tm, er, ev, ep, em, v);
     else
-      mode_terms<A, AR, PM, false>(args.mode_tab, qs, N, P, p, pn, tr, tv, tp, tm, er, ev, ep, em, v);
+      mode_terms<A, AR, PM, false>(as_const(args.mode_tab), qs, N, P, p, pn, tr, tv, tp, tm, er, ev, ep, em, v);
     // off-diagonal quadratic mode pairs (j != k)
     for (int t = 0; t < args.npair; ++t) {
       const int jj = pairs[2 * t], kk = pairs[2 * t + 1];
       const double xj = qs[jj * P + p], xk = qs[kk * P + p];
-      const double* Qh = qpair + (int64_t)t * A * A;
+      const_as<double>* Qh = qpair + (int64_t)t * A * A;
 #pragma unroll
       for (int b = 0; b < A; ++b)
 #pragma unroll
@@ -429,24 +528,74 @@ k_eval_fp64(const EvalArgs args) {
   eval_wave<A, AR, PM, DBG>(args, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
 }
 
+LaneProb lane_prob(const EvalArgs& a) {
+  LaneProb p{};
+  p.q = a.q;
+  p.out = a.out;
+  p.out_stride = a.out_stride;
+  p.B = a.B;
+  p.N = a.N;
+  p.P = a.P;
+  p.G = a.G;
+  p.K = a.K;
+  p.npair = a.npair;
+  p.noscale = a.noscale;
+  p.tau = a.tau;
+  p.mode_tab = a.mode_tab;
+  p.energy = a.energy;
+  p.qdiag = a.qdiag;
+  p.pair = a.pair;
+  p.qpair = a.qpair;
+  for (int k = 0; k < NUM_SETS; ++k) {
+    p.k2[k] = a.set[k].k2;
+    p.lpref[k] = a.set[k].lpref;
+  }
+  return p;
+}
+
 // Persistent batched launch: many blocks of samples (temperatures with different P, models
 // with the same surface counts) in one grid. Each wave repeatedly takes the next tile
 // (problem, first wave, wave count) from a device counter until the tile list is exhausted;
 // the host orders tiles by decreasing cost, so the long ones start first. Every wave reaches
-// the exit once the counter passes the tile count.
+// the exit once the counter passes the tile count. The problems' arguments come by value in
+// the kernel arguments (scalar registers), as in the stand-alone launch.
 template <int A, int AR, bool PM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A>::waves)))
-k_eval_batched(const EvalArgs* __restrict__ probs, const BatchTile* __restrict__ tiles, int ntiles,
-               int* counter) {
+k_eval_batched(const LaneProbs probs, const BatchTile* __restrict__ tiles, int ntiles, int* counter) {
   const int lane = threadIdx.x & (kWave - 1);
   for (;;) {
     int t = 0;
     if (lane == 0) t = atomicAdd(counter, 1);
     t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, kWave));
     if (t >= ntiles) break;
-    const BatchTile tile = tiles[t];
-    const EvalArgs& a = probs[tile.prob];
-    for (int64_t w = tile.wave0; w < tile.wave0 + tile.nwaves; ++w) eval_wave<A, AR, PM, false>(a, w);
+    const int pi = __builtin_amdgcn_readfirstlane(tiles[t].prob);
+    const int64_t w0 = __builtin_amdgcn_readfirstlane(tiles[t].wave0);
+    const int nw = __builtin_amdgcn_readfirstlane(tiles[t].nwaves);
+    const LaneProb& d = probs.p[pi];
+    EvalArgs a{};
+    a.q = d.q;
+    a.out = d.out;
+    a.out_stride = d.out_stride;
+    a.B = d.B;
+    a.N = d.N;
+    a.P = d.P;
+    a.G = d.G;
+    a.K = d.K;
+    a.npair = d.npair;
+    a.noscale = d.noscale;
+    a.tau = d.tau;
+    a.mode_tab = d.mode_tab;
+    a.energy = d.energy;
+    a.qdiag = d.qdiag;
+    a.pair = d.pair;
+    a.qpair = d.qpair;
+    a.dbg = nullptr;
+#pragma unroll
+    for (int k = 0; k < NUM_SETS; ++k) {
+      a.set[k].k2 = d.k2[k];
+      a.set[k].lpref = d.lpref[k];
+    }
+    for (int64_t w = w0; w < w0 + nw; ++w) eval_wave<A, AR, PM, false>(a, w);
   }
 }
 
@@ -476,7 +625,7 @@ static hipError_t launch_a(int pm, const EvalArgs& a, hipStream_t s) {
 }
 
 template <int A, int AR>
-static hipError_t launch_batched_a(int pm, const EvalArgs* probs, const BatchTile* tiles, int ntiles,
+static hipError_t launch_batched_a(int pm, const LaneProbs& probs, const BatchTile* tiles, int ntiles,
                                    int* counter, int waves, hipStream_t s) {
   const dim3 grid((unsigned)((waves + 3) / 4));
   if (pm)
@@ -504,7 +653,7 @@ hipError_t launch_eval_fp64(int A, int AR, int pm, const EvalArgs& a, hipStream_
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_eval_batched(int A, int AR, int pm, const EvalArgs* probs, const BatchTile* tiles,
+hipError_t launch_eval_batched(int A, int AR, int pm, const LaneProbs& probs, const BatchTile* tiles,
                                int ntiles, int* counter, int waves, hipStream_t s) {
 #define PIB_CASE(a_, ar_) \
   if (A == a_ && AR == ar_) return launch_batched_a<a_, ar_>(pm, probs, tiles, ntiles, counter, waves, s);

@@ -131,6 +131,32 @@ struct SumArgs {
   int64_t max_steps;
 };
 
+// Everything the lanes evaluation reads of one problem, passed by value in the kernel
+// arguments of the batched lanes kernel (at most kBatchLaunchProbs problems per launch), so
+// that every constant sits in scalar registers as in the stand-alone launch. Read from global
+// memory instead, the pointers lose their address space and the constants come through
+// vector loads (about 90 more VGPRs at A = 6, half the occupancy).
+constexpr int kBatchLaunchProbs = 16;
+struct LaneProb {
+  const double* q;
+  double* out;
+  int64_t out_stride;
+  int64_t B;
+  int N, P, G, K, npair, noscale;
+  double tau;
+  const double* mode_tab;
+  const double* energy;
+  const double* qdiag;
+  const int* pair;
+  const double* qpair;
+  const double* k2[NUM_SETS];
+  const double* lpref[NUM_SETS];
+};
+struct LaneProbs {
+  int n;
+  LaneProb p[kBatchLaunchProbs];
+};
+
 // One tile of a batched launch: waves [wave0, wave0 + nwaves) of problem `prob`.
 struct BatchTile {
   int prob;
@@ -142,8 +168,9 @@ struct BatchTile {
 hipError_t launch_eval_fp64(int A, int AR, int pm, const EvalArgs& a, hipStream_t s);
 hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s, bool bf16x3 = false);
 hipError_t launch_eval_fp32(int A, int AR, int pm, const EvalArgs& a, hipStream_t s);
-hipError_t launch_eval_batched(int A, int AR, int pm, const EvalArgs* probs, const BatchTile* tiles,
+hipError_t launch_eval_batched(int A, int AR, int pm, const LaneProbs& probs, const BatchTile* tiles,
                                int ntiles, int* counter, int waves, hipStream_t s);
+LaneProb lane_prob(const EvalArgs& a);
 hipError_t launch_eval_wide_batched(int pm, const EvalArgs* probs, const BatchTile* tiles, int ntiles,
                                     int* counter, int waves, hipStream_t s);
 constexpr int kWideMaxStates = 16;
