@@ -497,6 +497,20 @@ def batched_leg(FS, cfg, args, cpu, log):
     }
 
 
+def calibration_copy(mib):
+    """Three device copies of a `mib` MiB buffer (the known transfer that calibrates the
+    profiler's FETCH_SIZE / WRITE_SIZE counters); nothing when mib is 0."""
+    if not mib:
+        return
+    import torch
+    nel = mib * (1 << 20) // 8
+    src_t = torch.ones(nel, dtype=torch.float64, device="cuda")
+    dst_t = torch.empty_like(src_t)
+    for _ in range(3):
+        dst_t.copy_(src_t)
+    torch.cuda.synchronize()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -551,7 +565,9 @@ def main():
             Ts, Ps, _ = synthetic.c3_temperatures()
             cpu = cpu_baseline(FS.path_vib_model, FS.path_rho_model, float(Ts[8]), int(Ps[8]),
                                args.cpu_seconds, cores)
-        print(json.dumps(batched_leg(FS, cfg, args, cpu, log)))
+        line = batched_leg(FS, cfg, args, cpu, log)
+        calibration_copy(args.calib_copy_mib)
+        print(json.dumps(line))
         return
 
     fp64_name = None if args.fp64_workload == "none" else args.fp64_workload
@@ -591,14 +607,7 @@ def main():
 
     fp64 = fp64_line(FS64, fp64_name, synthetic.WORKLOADS[fp64_name], args, cpu64, log) if FS64 else None
 
-    if args.calib_copy_mib:
-        nel = args.calib_copy_mib * (1 << 20) // 8
-        src_t = torch.ones(nel, dtype=torch.float64, device="cuda")
-        dst_t = torch.empty_like(src_t)
-        for _ in range(3):
-            dst_t.copy_(src_t)
-        torch.cuda.synchronize()
-        del src_t, dst_t
+    calibration_copy(args.calib_copy_mib)
 
     harm_sbs = harm["samples"] * P / harm["seconds"]
     if learned:
