@@ -297,6 +297,11 @@ def timed_draws(fl, data, B, K, W, seed, variant, eval_events=True):
     return res
 
 
+def fp32_supported(data):
+    """The FP32 kernel covers A <= 8 (lanes family)."""
+    return data.states <= 8 and data.rho.states <= 8
+
+
 def learned_leg(data, args, log):
     """Train the flow on-policy, then time K steps of the single-bf16 draws; the same timed
     steps with FP32-accurate flow products are reported beside them."""
@@ -331,6 +336,25 @@ def learned_leg(data, args, log):
         fl.gemm = "f32"
         variants.append(timed_draws(fl, data, B, Kv, 1, args.seed + 4, "fp32 draws (FP32 matrix-core products)"))
         fl.gemm = "bf16x3"
+        # the precision pattern on the learned step: g by the FP32 kernel (PIB_EVAL_FP32), with
+        # its error against the FP64 kernel on one draw's samples
+        if fp32_supported(data):
+            import torch
+            from pibronic_amd.precision import precision_report
+            fl.draw_precision = "bf16" if single else "bf16x3"
+            data.fp32 = True
+            try:
+                v = timed_draws(fl, data, B, Kv, 1, args.seed + 5, f"{fl.draw_precision} draws, g by the FP32 kernel")
+            finally:
+                data.fp32 = False
+            R, _ = fl.draw(B, args.seed + 6, 1 << 42)
+            rep = precision_report(data, R, B, pm=False, variant="fp32")
+            v["fp32_error_vs_fp64"] = {"r_median_rel": rep["r"]["median_rel"], "r_max_rel": rep["r"]["max_rel"],
+                                       "r_max_abs_over_mean_abs": rep["r"]["max_abs_over_mean_abs"],
+                                       "z_shift_sigma": rep["z_shift_sigma"]}
+            variants.append(v)
+            del R
+            torch.cuda.synchronize()
         for v in variants:
             log(f"[bench] learned {v['variant']}: ess fraction {v['ess_fraction']:.4f}, "
                 f"{v['ess_per_s']:.3e} ESS/s")
@@ -583,6 +607,8 @@ def main():
             log(f"[bench] cpu baseline {fp64_name}: {cpu64['samples_beads_per_s']:.3e} samples*beads/s")
 
     import torch  # noqa: F401  (the GPU is first touched below)
+    # the plain FP64 kernel's line first, on a GPU that has not yet run the flow's matrix-core work
+    fp64 = fp64_line(FS64, fp64_name, synthetic.WORKLOADS[fp64_name], args, cpu64, log) if FS64 else None
     A, N, P, T = cfg["A"], cfg["N"], cfg["P"], cfg["T"]
     B_l, K, W = args.flow_block, args.steps, args.warmup
     # harmonic rho on the same model and at least as many samples as the timed learned steps
@@ -604,8 +630,6 @@ def main():
         tails = {"harmonic": rho_tails(data, args.tail_samples, args.seed + 7)}
         if fl is not None:
             tails["learned"] = flow_tails(fl, data, args.tail_samples, args.seed + 8)
-
-    fp64 = fp64_line(FS64, fp64_name, synthetic.WORKLOADS[fp64_name], args, cpu64, log) if FS64 else None
 
     calibration_copy(args.calib_copy_mib)
 

@@ -211,6 +211,7 @@ int pib_ctx_destroy(pib_ctx* c) {
   release(c->mode_tab);
   release(c->fb_mu);
   release(c->fb_c);
+  release(c->fb_qidx);
   release(c->mode_tab_f);
   release(c->s_cdf);
   release(c->s_chol);
@@ -798,11 +799,30 @@ int pib_flow_set_base(pib_ctx* c, int A_rho, int D, const double* mu, const doub
     return fail(c, PIB_ERR_ARG, "flow_set_base: need 1 <= A_rho <= 16, D even and non-null arrays");
   if (!finite_all(mu, (size_t)A_rho * D) || !finite_all(cc, A_rho))
     return fail(c, PIB_ERR_ARG, "flow_set_base: non-finite value");
+  if (D % 4) return fail(c, PIB_ERR_ARG, "flow_set_base: D must be a multiple of 4");
+  // only the feature quads where some mu_a is non-zero are kept (the displacements live in the
+  // constant normal mode of each bead mode: N of the D / 4 quads); the draw adds and dots
+  // exactly those
+  std::vector<int> qidx(D / 4, -1);
+  std::vector<double> mq;
+  int nq = 0;
+  for (int m = 0; m < D / 4; ++m) {
+    bool nz = false;
+    for (int b = 0; b < A_rho && !nz; ++b)
+      for (int e = 0; e < 4; ++e) nz = nz || mu[(size_t)b * D + 4 * m + e] != 0.0;
+    if (!nz) continue;
+    qidx[m] = nq++;
+    for (int b = 0; b < A_rho; ++b)
+      for (int e = 0; e < 4; ++e) mq.push_back(mu[(size_t)b * D + 4 * m + e]);
+  }
+  if (mq.empty()) mq.assign(4 * A_rho, 0.0);  // keep a valid buffer
   int r;
-  if ((r = upload(c, c->fb_mu, mu, (size_t)A_rho * D * sizeof(double)))) return r;
+  if ((r = upload(c, c->fb_mu, mq.data(), mq.size() * sizeof(double)))) return r;
+  if ((r = upload(c, c->fb_qidx, qidx.data(), qidx.size() * sizeof(int)))) return r;
   if ((r = upload(c, c->fb_c, cc, (size_t)A_rho * sizeof(double)))) return r;
   c->fb_A = A_rho;
   c->fb_D = D;
+  c->fb_nq = nq;
   return PIB_OK;
 }
 
@@ -824,6 +844,8 @@ int pib_flow_draw_base(pib_ctx* c, uint64_t seed, int64_t offset, int64_t B, flo
   a.key1 = (uint32_t)(seed >> 32);
   a.cdf = (const double*)c->s_cdf.p;
   a.mu = (const double*)c->fb_mu.p;
+  a.qidx = (const int*)c->fb_qidx.p;
+  a.nq = c->fb_nq;
   a.c = (const double*)c->fb_c.p;
   a.z = z;
   a.logrho = logrho;

@@ -61,8 +61,9 @@ struct pib_ctx {
   // flow basis
   int f_N = 0, f_P = 0;
   pib::DevBuf f_U, f_sigma, f_isigma, f_m;
-  pib::DevBuf fb_mu, fb_c;  // flow base distribution (pib_flow_set_base)
-  int fb_A = 0, fb_D = 0;
+  pib::DevBuf fb_mu, fb_c, fb_qidx;  // flow base distribution (pib_flow_set_base): the non-zero
+                                     // feature quads of mu and their index map
+  int fb_A = 0, fb_D = 0, fb_nq = 0;
 };
 
 namespace pib {

@@ -272,19 +272,26 @@ __device__ __forceinline__ double group16_sum(double v) {
   return v;
 }
 
-// The displacements mu (A_rho x D doubles) are read for every feature of every sample; with
-// MU_LDS (when they fit in 64 KB) each work-group stages them in LDS once and loops over groups
-// of sixteen samples, so that those reads bypass the vector-memory address units (which the
-// per-feature global loads otherwise saturate).
+// The displacements mu are non-zero only in a few feature quads (the constant normal mode of
+// each bead mode); qidx maps a quad to its row of the compact mu (nq * A_rho * 4) or -1. Both
+// are staged in LDS once per work-group when they fit in 64 KB (MU_LDS), the work-groups then
+// looping over groups of sixteen samples. Quads with qidx < 0 take z = eps (the added mu is 0)
+// and contribute nothing to z . mu_b.
 template <bool MU_LDS>
 __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
-  extern __shared__ double mu_lds[];
+  extern __shared__ double lds[];
   const double* __restrict__ mu = a.mu;
+  const int* __restrict__ qidx = a.qidx;
   if constexpr (MU_LDS) {
-    for (int i = threadIdx.x; i < a.A_rho * a.D; i += 256) mu_lds[i] = a.mu[i];
+    const int nmu = a.nq * a.A_rho * 4;
+    for (int i = threadIdx.x; i < nmu; i += 256) lds[i] = a.mu[i];
+    int* qi_lds = (int*)(lds + nmu);
+    for (int i = threadIdx.x; i < a.D / 4; i += 256) qi_lds[i] = a.qidx[i];
     __syncthreads();
-    mu = mu_lds;
+    mu = lds;
+    qidx = qi_lds;
   }
+  const int AR = a.A_rho;
   const int sl = threadIdx.x & (kBaseLanes - 1);
   const int64_t groups = (a.B + 256 / kBaseLanes - 1) / (256 / kBaseLanes);
   for (int64_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {  // uniform over the block
@@ -292,12 +299,11 @@ __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
     const bool live = s < a.B;  // uniform over the sixteen lanes of a sample
     const int64_t gs = a.offset + (live ? s : 0);
     int src = 0;
-    if (a.A_rho > 1) {
+    if (AR > 1) {
       const u32x4 r = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), 0u, 1u}, a.key0, a.key1);
       const double u = u53(r.x, r.y);
-      while (src < a.A_rho - 1 && u >= a.cdf[src]) ++src;
+      while (src < AR - 1 && u >= a.cdf[src]) ++src;
     }
-    const double* __restrict__ mus = mu + (int64_t)src * a.D;
     float* zr = a.z + (live ? s : 0) * a.D;
     double zz = 0.0, dot[kBaseMaxA];
 #pragma unroll
@@ -319,19 +325,26 @@ __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
         const float ra = __fsqrt_rn(-1.3862943611198906f * l2a), rb = __fsqrt_rn(-1.3862943611198906f * l2b);
         const float ua = ((float)r.y + 0.5f) * 0x1.0p-32f, ub = ((float)r.w + 0.5f) * 0x1.0p-32f;
         // v_cos_f32 / v_sin_f32 take their argument in revolutions
-        const float zf[4] = {ra * __builtin_amdgcn_cosf(ua), ra * __builtin_amdgcn_sinf(ua),
-                             rb * __builtin_amdgcn_cosf(ub), rb * __builtin_amdgcn_sinf(ub)};
-        float zo[4];
+        float zo[4] = {ra * __builtin_amdgcn_cosf(ua), ra * __builtin_amdgcn_sinf(ua),
+                       rb * __builtin_amdgcn_cosf(ub), rb * __builtin_amdgcn_sinf(ub)};
+        const int qi = qidx[m];
+        if (qi >= 0) {
+          const double* __restrict__ mq = mu + (int64_t)qi * AR * 4;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) zo[e] = (float)((double)zf[e] + mus[4 * m + e]);
+          for (int e = 0; e < 4; ++e) zo[e] = (float)((double)zo[e] + mq[src * 4 + e]);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const double d = zo[e];
+#pragma unroll
+            for (int b = 0; b < kBaseMaxA; ++b)
+              if (b < AR) dot[b] = fma(d, mq[b * 4 + e], dot[b]);
+          }
+        }
         *(float4*)(zr + 4 * m) = float4{zo[0], zo[1], zo[2], zo[3]};
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const double d = zo[e];
           zz = fma(d, d, zz);
-#pragma unroll
-          for (int b = 0; b < kBaseMaxA; ++b)
-            if (b < a.A_rho) dot[b] = fma(d, mu[(int64_t)b * a.D + 4 * m + e], dot[b]);
         }
       }
     }
@@ -355,9 +368,9 @@ __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
 }
 
 hipError_t launch_flow_base(const BaseArgs& a, hipStream_t s) {
-  if (a.B <= 0 || a.D <= 0 || (a.D & 3) || a.A_rho < 1 || a.A_rho > kBaseMaxA) return hipErrorInvalidValue;
+  if (a.B <= 0 || a.D <= 0 || (a.D & 3) || a.A_rho < 1 || a.A_rho > kBaseMaxA || a.nq < 0) return hipErrorInvalidValue;
   const int64_t groups = (a.B + 256 / kBaseLanes - 1) / (256 / kBaseLanes);
-  const size_t mu_bytes = (size_t)a.A_rho * a.D * sizeof(double);
+  const size_t mu_bytes = (size_t)a.nq * a.A_rho * 4 * sizeof(double) + (size_t)(a.D / 4) * sizeof(int);
   if (mu_bytes <= 65536) {
     // a few work-groups per CU, each staging mu once
     const int64_t grid = std::min<int64_t>(groups, 2048);

@@ -72,7 +72,9 @@ struct BaseArgs {
   int D, A_rho;
   uint32_t key0, key1;
   const double* cdf;  // A_rho cumulative surface weights
-  const double* mu;   // A_rho * D
+  const double* mu;   // the non-zero feature quads of the displacements: nq * A_rho * 4
+  const int* qidx;    // D / 4: index of a feature quad in mu, or -1 where every mu_a is 0 there
+  int nq;             // feature quads with a non-zero displacement
   const double* c;    // A_rho
   float* z;           // B * D
   double* logrho;     // B

@@ -101,6 +101,10 @@ class Flow:
         if np.any(pref <= 0.0):
             raise ValueError("flow base distribution needs positive O-matrix prefactors")
         colsum = U.sum(axis=0)                                            # (P,)
+        # every eigenvector but the constant one is orthogonal to the constant vector: their
+        # column sums are rounding noise (~1e-16); set them to exactly 0, so that mu is non-zero
+        # only in the constant normal mode of each bead mode and the draw skips the rest
+        colsum[np.abs(colsum) < 1e-12 * np.abs(colsum).max()] = 0.0
         mu = (shift - m[None, :])[:, :, None] * colsum[None, None, :] / np.asarray(sigma)[None]
         mu_p = np.concatenate([mu[:, :, 0::2].reshape(len(pref), -1), mu[:, :, 1::2].reshape(len(pref), -1)], axis=1)
         c = self.P * np.log(pref) - 0.5 * np.sum(mu_p * mu_p, axis=1)
