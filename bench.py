@@ -411,6 +411,15 @@ def fp64_line(FS, name, cfg, args, cpu, log):
     B = args.block
     data = setup_data(FS, cfg, B, args.fp64_steps + args.warmup, args.seed)
     lanes = data.context().lanes_per_sample(B)
+    # this line runs first on an idle GPU: untimed blocks for args.fp64_warm_s seconds bring the
+    # clocks up before the W warm-up blocks and the timed ones
+    ctx, dev = data.context(), data.device_buffers(B)
+    t_end = time.perf_counter() + args.fp64_warm_s
+    while time.perf_counter() < t_end:
+        for _ in range(8):
+            ctx.sample_block(args.seed, 0, B, dev["q"], None)
+            ctx.eval_block(dev["q"], B, 0, dev["out"], dev["out"].shape[1], None)
+        torch.cuda.synchronize()
     h = harmonic_leg(data, B, args.fp64_steps, args.warmup, args.seed, 0, False)
     sbs = h["samples"] * P / h["seconds"]
     kname = f"k_eval_fp64<{A}, {A}, false, false>" if A <= 8 else "k_eval_wide"
@@ -545,7 +554,9 @@ def main():
     ap.add_argument("--workload", default="A8_N16_P64_T300")
     ap.add_argument("--fp64-workload", default="A4_N8_P64_T300",
                     help="workload of the plain FP64 kernel's samples*beads/s line ('none' to skip)")
-    ap.add_argument("--fp64-steps", type=int, default=10, help="timed blocks of the FP64 line")
+    ap.add_argument("--fp64-steps", type=int, default=20, help="timed blocks of the FP64 line")
+    ap.add_argument("--fp64-warm-s", type=float, default=0.5,
+                    help="seconds of untimed blocks before the FP64 line's warm-up and timed blocks")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")

@@ -27,7 +27,10 @@ int rebuild_mode_table(pib_ctx* c) {
     return PIB_OK;
   const ModeRec R(A, AR);
   const size_t AA = (size_t)A * A;
-  std::vector<double> tab((size_t)N * R.len(), 0.0);
+  // zero records after the last mode up to a multiple of four (the lanes kernel takes modes in
+  // groups of four; a zero record adds exactly nothing)
+  const int N4 = (N + 3) & ~3;
+  std::vector<double> tab((size_t)N4 * R.len(), 0.0);
   const bool pm = c->set_A[SET_VIB_PLUS] == A && c->set_A[SET_VIB_MINUS] == A &&
                   c->set_N[SET_VIB_PLUS] == N && c->set_N[SET_VIB_MINUS] == N;
   for (int j = 0; j < N; ++j) {

@@ -141,21 +141,38 @@ __device__ __forceinline__ void expm_lane_ps12(double (&X)[A][A], double (&T)[A]
   }
 }
 
-// The degree-8 Taylor polynomial by Horner's rule (seven products), alpha = |X|_F and
-// theta_8 = 0.049 (below the root of |y|^8 / 9! = u): three matrices live instead of five.
+// theta_8 = 0.049: below the root of |y|^8 / 9! = u (the degree-8 polynomial's truncation)
 constexpr double kLog2Theta8 = -4.3511;  // log2(0.049)
 
+// The degree-8 Taylor polynomial with three matrix products (Horner's rule takes seven), in the
+// evaluation form of Sastre (J. Sastre, "Efficient evaluation of matrix polynomials", Linear
+// Algebra Appl. 539 (2018) 229-250):
+//   Y  = X2 (c1 X2 + c2 X),
+//   T8 = (Y + c3 X2 + c4 X)(Y + c5 X2) + c6 Y + X2/2 + X + I,
+// whose coefficients solve the six equations that make the expansion's X^3..X^8 terms 1/k!
+// (c1^2 = 1/8!, 2 c1 c2 = 1/7!, ...; of the two real solution pairs, the one with the smaller
+// c6). Y's product is taken as two dot products per entry (X2 X2 and X2 X), so that no fourth
+// matrix is live beside X, X2 and Y; the three factors of the last product then replace X, X2
+// and Y entry by entry. alpha = |X^2|_F^(1/2), a tighter bound on the spectral radius of a
+// symmetric X than |X|_F (the root of the largest of sum lambda^4), from X2, which the
+// polynomial needs anyway. For |X| <= theta_8 the terms beyond I + X + X^2/2 are below 1e-5 of
+// the result, so the form rounds like Horner's rule.
 template <int A>
-__device__ __forceinline__ void expm_lane_h8(double (&X)[A][A], double (&T)[A][A]) {
-  double nf = 0.0;
+__device__ __forceinline__ void expm_lane_s8(double (&X)[A][A], double (&T)[A][A]) {
+  constexpr double c1 = 4.98011920555997349987007158e-3, c2 = 1.99204768222398939994802863e-2;
+  constexpr double c3 = 7.66526532111914669031909419e-2, c4 = 8.76500980178555335977132598e-1;
+  constexpr double c5 = 1.22552115011207473091611921e-1, c6 = 2.97430720484762666375079572e0;
+  double X2[A][A];
+  sym_prod_lane<A>(X, X, 0.0, X2);
+  double n2 = 0.0;
 #pragma unroll
   for (int i = 0; i < A; ++i) {
-    nf = fma(X[i][i], X[i][i], nf);
+    n2 = fma(X2[i][i], X2[i][i], n2);
 #pragma unroll
-    for (int k = i + 1; k < A; ++k) nf = fma(2.0 * X[i][k], X[i][k], nf);
+    for (int k = i + 1; k < A; ++k) n2 = fma(2.0 * X2[i][k], X2[i][k], n2);
   }
-  // smallest s >= 0 with |X|_F 2^-s <= theta_8; a zero matrix gives -inf, NaN gives s = 0
-  const double sd = 0.5 * log2_coarse(nf) - kLog2Theta8;
+  // smallest s >= 0 with |X^2|_F^(1/2) 2^-s <= theta_8; a zero matrix gives -inf, NaN s = 0
+  const double sd = 0.25 * log2_coarse(n2) - kLog2Theta8;
   int s = 0;
   if (sd > 0.0) s = sd < 64.0 ? (int)ceil(sd) : 64;
 #pragma unroll
@@ -164,21 +181,49 @@ __device__ __forceinline__ void expm_lane_h8(double (&X)[A][A], double (&T)[A][A
     for (int k = i; k < A; ++k) {
       X[i][k] = ldexp(X[i][k], -s);
       X[k][i] = X[i][k];
+      X2[i][k] = ldexp(X2[i][k], -2 * s);
+      X2[k][i] = X2[i][k];
     }
-  constexpr double c[9] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040, 1.0 / 40320};
+  // T <- Y = c1 X2 X2 + c2 X2 X (symmetric: polynomials in X commute)
 #pragma unroll
   for (int i = 0; i < A; ++i)
 #pragma unroll
-    for (int k = 0; k < A; ++k) T[i][k] = (i == k) ? fma(c[8], X[i][k], c[7]) : c[8] * X[i][k];
+    for (int k = i; k < A; ++k) {
+      double a = X2[i][0] * X2[0][k], b = X2[i][0] * X[0][k];
 #pragma unroll
-  for (int n = 6; n >= 0; --n) {
-    double Z[A][A];
-    sym_prod_lane<A>(X, T, c[n], Z);
+      for (int c = 1; c < A; ++c) {
+        a = fma(X2[i][c], X2[c][k], a);
+        b = fma(X2[i][c], X[c][k], b);
+      }
+      T[i][k] = fma(c1, a, c2 * b);
+      T[k][i] = T[i][k];
+    }
+  // factors in place: X <- Y + c3 X2 + c4 X, X2 <- Y + c5 X2, T <- c6 Y + X2/2 + X + I
 #pragma unroll
-    for (int i = 0; i < A; ++i)
+  for (int i = 0; i < A; ++i)
 #pragma unroll
-      for (int k = 0; k < A; ++k) T[i][k] = Z[i][k];
-  }
+    for (int k = i; k < A; ++k) {
+      const double x = X[i][k], x2 = X2[i][k], y = T[i][k];
+      double w = fma(c6, y, fma(0.5, x2, x));
+      if (i == k) w += 1.0;
+      X[i][k] = fma(c3, x2, fma(c4, x, y));
+      X[k][i] = X[i][k];
+      X2[i][k] = fma(c5, x2, y);
+      X2[k][i] = X2[i][k];
+      T[i][k] = w;
+      T[k][i] = w;
+    }
+  // T <- X X2 + T: each output entry reads only its own entry of T
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = i; k < A; ++k) {
+      double acc = T[i][k];
+#pragma unroll
+      for (int c = 0; c < A; ++c) acc = fma(X[i][c], X2[c][k], acc);
+      T[i][k] = acc;
+      T[k][i] = acc;
+    }
   for (int k = 0; k < s; ++k) {
     double Z[A][A];
     sym_prod_lane<A>(T, T, 0.0, Z);
@@ -190,11 +235,11 @@ __device__ __forceinline__ void expm_lane_h8(double (&X)[A][A], double (&T)[A][A
 }
 
 // Which scheme each surface count uses (index A): Paterson-Stockmeyer degree 12 where its two
-// extra live matrices cost no occupancy that the saved products do not buy back. Measured on
-// MI355X (eval ms, Horner-8 against PS-12, same GPU): A=4, N=8, P=64, 65,536 samples 0.209
-// against 0.197; A=6, N=12, P=128, 1e5 samples 1.70 against 2.16 (PS-12 needs 318 VGPRs there:
-// one wave per SIMD instead of two); A=8, N=16, P=64, 65,536 samples 1.51 against 1.57 (both
-// at one wave per SIMD, PS-12 spilling).
+// extra live matrices cost no occupancy that the saved products do not buy back, else the
+// three-product degree-8 form. Measured on MI355X (eval ms, Horner-8 against PS-12, same GPU):
+// A=4, N=8, P=64, 65,536 samples 0.209 against 0.197; A=6, N=12, P=128, 1e5 samples 1.70
+// against 2.16 (PS-12 needs 318 VGPRs there: one wave per SIMD instead of two); A=8, N=16,
+// P=64, 65,536 samples 1.51 against 1.57 (both at one wave per SIMD, PS-12 spilling).
 constexpr bool kExpmPS12[9] = {true, true, true, true, true, false, false, false, false};
 
 template <int A>
@@ -202,11 +247,28 @@ __device__ __forceinline__ void expm_lane(double (&X)[A][A], double (&T)[A][A]) 
   if constexpr (kExpmPS12[A])
     expm_lane_ps12<A>(X, T);
   else
-    expm_lane_h8<A>(X, T);
+    expm_lane_s8<A>(X, T);
 }
 
+// One wave's coordinates through a buffer resource (base and size wave-uniform; loads past the
+// end return 0). The mode loop loads each mode's coordinates one or two modes ahead of their
+// use; with plain loads the optimizer folds the loop's phi of two loads (the prologue's and the
+// previous iteration's) into one load of a phi of addresses at the top of the iteration, right
+// before the use, so a wave waited a full memory latency per mode (at A = 8, one wave per
+// SIMD, nothing covers it). The buffer-load intrinsic is a call, which that fold leaves alone.
+struct CoordBuf {
+  __amdgpu_buffer_rsrc_t rsrc;
+  int base;  // this lane's sample: index of its first coordinate in the resource (doubles)
+  __device__ __forceinline__ double operator[](int i) const {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, (base + i) * 8, 0, 0));
+  }
+};
+
+// gfx9 buffer descriptor word 3: 32-bit data format, raw (stride 0) addressing
+constexpr int kBufferWord3 = 0x00020000;
+
 template <int A, int AR, bool PM, bool QD>
-__device__ __forceinline__ void mode_terms(const_as<double>* tab, const double* __restrict__ qs, int N,
+__device__ __forceinline__ void mode_terms(const_as<double>* tab, const CoordBuf& qs, int N,
                                            int P, int p, int pn, double& tr, double& tv, double& tp,
                                            double& tm, double (&er)[AR], double (&ev)[A], double (&ep)[A],
                                            double (&em)[A], double (&v)[A][A]) {
@@ -242,42 +304,48 @@ __device__ __forceinline__ void mode_terms(const_as<double>* tab, const double* 
         if (QD) v[b][c] = fma(t[R.quad() + e], xx, v[b][c]);
       }
   };
-  if constexpr (A <= 4) {
-    // coordinates loaded two modes ahead, the loop unrolled by two so that the rotation needs
-    // no register copy (a copy of a value still in flight would wait for the load just issued;
-    // 4% faster at A = 4; the larger kernels, at the register limit, keep one mode ahead)
-    const int j1 = N > 1 ? 1 : 0;
-    double x0 = qs[p], xn = qs[pn], x0_1 = qs[j1 * P + p], xn_1 = qs[j1 * P + pn];
-#pragma unroll 2
-    for (int j = 0; j < N; ++j) {
-      const int j2 = (j + 2 < N) ? j + 2 : N - 1;
-      const double x0_2 = qs[j2 * P + p], xn_2 = qs[j2 * P + pn];
-      mode(j, x0, xn);
-      x0 = x0_1;
-      xn = xn_1;
-      x0_1 = x0_2;
-      xn_1 = xn_2;
+  // Eight modes per iteration in two halves of four: the coordinates of one half are loaded
+  // while the other half's arithmetic runs, four modes ahead of their use, into registers that
+  // are free by then (a copy of a value still in flight would wait for its load). The modes are
+  // taken in groups of four: modes past N read the zero records the table is padded with (and
+  // the last mode's coordinates again). The loop body is straight-line code, which keeps the
+  // compiler's wait counts exact (with a branch per mode it waited for every load in flight at
+  // the top of each iteration).
+  auto ld4 = [&](int j, double (&x)[4][2]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int jj = j + k < N ? j + k : N - 1;
+      x[k][0] = qs[jj * P + p];
+      x[k][1] = qs[jj * P + pn];
     }
-  } else {
-    // the next mode's coordinates are loaded while this mode's arithmetic runs
-    double x0 = qs[p], xn = qs[pn];
-    for (int j = 0; j < N; ++j) {
-      const int jn = (j + 1 < N) ? j + 1 : j;
-      const double x0_next = qs[jn * P + p], xn_next = qs[jn * P + pn];
-      mode(j, x0, xn);
-      x0 = x0_next;
-      xn = xn_next;
-    }
+  };
+  const int N4 = (N + 3) & ~3;
+  double xa[4][2], xb[4][2];
+  ld4(0, xa);
+  int j = 0;
+  for (; j + 8 <= N4; j += 8) {
+    ld4(j + 4, xb);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) mode(j + k, xa[k][0], xa[k][1]);
+    ld4(j + 8, xa);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) mode(j + 4 + k, xb[k][0], xb[k][1]);
+  }
+  if (j < N4) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) mode(j + k, xa[k][0], xa[k][1]);
   }
 }
 
 // Occupancy: for A <= 4 the VGPR budget is capped at 256 (two waves per SIMD): the plain
 // kernel needs 128 (four waves), the +-dbeta kernel's three ring states need ~200, and a cap
 // of 168 made it spill (0.78 against 0.59 ms at A = 4, N = 8, P = 64, 1e5 samples, measured on
-// MI355X). Larger A keeps the default budget (512 with the accumulation registers).
-template <int A>
+// MI355X). A = 5, 6 without +-dbeta are held to 256 as well (the persistent batched kernel
+// otherwise takes 268 at A = 6 and drops to one wave per SIMD). Larger A, and the +-dbeta
+// kernels of A > 4, keep the default budget (512 with the accumulation registers).
+template <int A, bool PM>
 struct EvalOccupancy {
-  static constexpr int waves = (A <= 4) ? 2 : 1;
+  static constexpr int waves = (A <= 4 || (A <= 6 && !PM)) ? 2 : 1;
 };
 
 // The samples of one wave: wave w evaluates samples [w * 64/G, (w + 1) * 64/G) of `args`.
@@ -291,6 +359,13 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
   const bool live = sample < args.B;
   const int64_t NP = (int64_t)N * P;
   const double* __restrict__ qs = args.q + (live ? sample : 0) * NP;
+  // the wave's samples [s0, s0 + nsw) as one buffer resource (s0 made wave-uniform explicitly)
+  const int64_t s0 = ((int64_t)__builtin_amdgcn_readfirstlane((int)((uint64_t)(wave * spw) >> 32)) << 32) |
+                     (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(wave * spw));
+  const int64_t nsw = args.B - s0 < spw ? (args.B - s0 > 0 ? args.B - s0 : 0) : spw;
+  const CoordBuf qb{__builtin_amdgcn_make_buffer_rsrc((void*)(args.q + s0 * NP), (short)0, (int)(nsw * NP * 8),
+                                                      kBufferWord3),
+                    live ? (int)((sample - s0) * NP) : 0};
 
   // read-only constants through the constant address space: scalar loads
   struct RO {
@@ -339,9 +414,9 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
       for (int c = 0; c <= a; ++c) v[a][c] = 0.0;
     }
     if (Qd)
-      mode_terms<A, AR, PM, true>(as_const(args.mode_tab), qs, N, P, p, pn, tr, tv, tp, tm, er, ev, ep, em, v);
+      mode_terms<A, AR, PM, true>(as_const(args.mode_tab), qb, N, P, p, pn, tr, tv, tp, tm, er, ev, ep, em, v);
     else
-      mode_terms<A, AR, PM, false>(as_const(args.mode_tab), qs, N, P, p, pn, tr, tv, tp, tm, er, ev, ep, em, v);
+      mode_terms<A, AR, PM, false>(as_const(args.mode_tab), qb, N, P, p, pn, tr, tv, tp, tm, er, ev, ep, em, v);
     // off-diagonal quadratic mode pairs (j != k)
     for (int t = 0; t < args.npair; ++t) {
       const int jj = pairs[2 * t], kk = pairs[2 * t + 1];
@@ -523,7 +598,7 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
 }
 
 template <int A, int AR, bool PM, bool DBG>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A>::waves)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A, PM>::waves)))
 k_eval_fp64(const EvalArgs args) {
   eval_wave<A, AR, PM, DBG>(args, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
 }
@@ -560,7 +635,7 @@ LaneProb lane_prob(const EvalArgs& a) {
 // the exit once the counter passes the tile count. The problems' arguments come by value in
 // the kernel arguments (scalar registers), as in the stand-alone launch.
 template <int A, int AR, bool PM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A>::waves)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A, PM>::waves)))
 k_eval_batched(const LaneProbs probs, const BatchTile* __restrict__ tiles, int ntiles, int* counter) {
   const int lane = threadIdx.x & (kWave - 1);
   for (;;) {

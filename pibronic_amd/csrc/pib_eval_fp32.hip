@@ -24,10 +24,14 @@ namespace pib {
 namespace {
 
 // M = exp(X) for a symmetric A x A matrix in one lane's registers, X = -tau V, in FP32: the
-// FP64 kernel's scaled and squared Taylor polynomial (see expm_lane there) at degree 5, with
-// |X|_F 2^-s <= theta_5 = 0.13, below the root of |y|^5 / 6! = 2^-24 (degree 5 minimises
-// products plus squarings for FP32 Horner evaluation).
-constexpr float kLog2Theta5f = -2.9434f;  // log2(0.13)
+// FP64 kernel's scaled and squared degree-8 Taylor polynomial in three products (see
+// expm_lane_s8 there: Y = X2 (c1 X2 + c2 X), T8 = (Y + c3 X2 + c4 X)(Y + c5 X2) + c6 Y + X2/2 +
+// X + I), with |X^2|_F^(1/2) 2^-s <= theta_8 = 0.619, below the root of |y|^8 / 9! = 2^-24.
+// Registers are half as dear as in FP64, so Y's right factor is formed as a matrix (three
+// products). Against Horner's rule at degree 5 (theta_5 = 0.13 on |X|_F, four products) the
+// squaring loop runs 2.1 instead of 5.0 times per wave at A = 8, N = 16, P = 64 and 0.8 instead
+// of 3.0 at A = 4, N = 8, P = 64 (products per bead 5.1 against 9.0 and 3.8 against 7.0).
+constexpr float kLog2Theta8f = -0.69197f;  // log2(0.619)
 
 // Z = X Y + d I for commuting symmetric X, Y (Z symmetric; Z must not alias X or Y)
 template <int A>
@@ -47,38 +51,63 @@ __device__ __forceinline__ void sym_prod_lane_f(const float (&X)[A][A], const fl
 
 template <int A>
 __device__ __forceinline__ void expm_lane_f(float (&X)[A][A], float (&T)[A][A]) {
-  float nf = 0.0f;
+  constexpr float c1 = 4.98011920555997e-3f, c2 = 1.99204768222399e-2f, c3 = 7.66526532111915e-2f;
+  constexpr float c4 = 8.76500980178555e-1f, c5 = 1.22552115011207e-1f, c6 = 2.97430720484763e0f;
+  float X2[A][A];
+  sym_prod_lane_f<A>(X, X, 0.0f, X2);
+  float n2 = 0.0f;
 #pragma unroll
   for (int i = 0; i < A; ++i) {
-    nf = fmaf(X[i][i], X[i][i], nf);
+    n2 = fmaf(X2[i][i], X2[i][i], n2);
 #pragma unroll
-    for (int k = i + 1; k < A; ++k) nf = fmaf(2.0f * X[i][k], X[i][k], nf);
+    for (int k = i + 1; k < A; ++k) n2 = fmaf(2.0f * X2[i][k], X2[i][k], n2);
   }
-  // smallest s >= 0 with |X|_F 2^-s <= theta_5; a zero matrix gives -inf, NaN gives s = 0
-  const float sd = 0.5f * __log2f(nf) - kLog2Theta5f;
+  // smallest s >= 0 with |X^2|_F^(1/2) 2^-s <= theta_8; a zero matrix gives -inf, NaN s = 0
+  const float sd = 0.25f * __log2f(n2) - kLog2Theta8f;
   int s = 0;
   if (sd > 0.0f) s = sd < 64.0f ? (int)ceilf(sd) : 64;
+  float Y[A][A];
+  {
+    float W[A][A];
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int k = i; k < A; ++k) {
+        X[i][k] = ldexpf(X[i][k], -s);
+        X[k][i] = X[i][k];
+        X2[i][k] = ldexpf(X2[i][k], -2 * s);
+        X2[k][i] = X2[i][k];
+        W[i][k] = fmaf(c1, X2[i][k], c2 * X[i][k]);
+        W[k][i] = W[i][k];
+      }
+    sym_prod_lane_f<A>(X2, W, 0.0f, Y);
+  }
+  // factors in place: X <- Y + c3 X2 + c4 X, X2 <- Y + c5 X2, T <- c6 Y + X2/2 + X + I
 #pragma unroll
   for (int i = 0; i < A; ++i)
 #pragma unroll
     for (int k = i; k < A; ++k) {
-      X[i][k] = ldexpf(X[i][k], -s);
+      const float x = X[i][k], x2 = X2[i][k], y = Y[i][k];
+      float w = fmaf(c6, y, fmaf(0.5f, x2, x));
+      if (i == k) w += 1.0f;
+      X[i][k] = fmaf(c3, x2, fmaf(c4, x, y));
       X[k][i] = X[i][k];
+      X2[i][k] = fmaf(c5, x2, y);
+      X2[k][i] = X2[i][k];
+      T[i][k] = w;
+      T[k][i] = w;
     }
-  constexpr float c[6] = {1.0f, 1.0f, 1.0f / 2, 1.0f / 6, 1.0f / 24, 1.0f / 120};
+  // T <- X X2 + T: each output entry reads only its own entry of T
 #pragma unroll
   for (int i = 0; i < A; ++i)
 #pragma unroll
-    for (int k = 0; k < A; ++k) T[i][k] = (i == k) ? fmaf(c[5], X[i][k], c[4]) : c[5] * X[i][k];
+    for (int k = i; k < A; ++k) {
+      float acc = T[i][k];
 #pragma unroll
-  for (int n = 3; n >= 0; --n) {
-    float Z[A][A];
-    sym_prod_lane_f<A>(X, T, c[n], Z);
-#pragma unroll
-    for (int i = 0; i < A; ++i)
-#pragma unroll
-      for (int k = 0; k < A; ++k) T[i][k] = Z[i][k];
-  }
+      for (int c = 0; c < A; ++c) acc = fmaf(X[i][c], X2[c][k], acc);
+      T[i][k] = acc;
+      T[k][i] = acc;
+    }
   for (int k = 0; k < s; ++k) {
     float Z[A][A];
     sym_prod_lane_f<A>(T, T, 0.0f, Z);

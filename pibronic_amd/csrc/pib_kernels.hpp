@@ -54,7 +54,8 @@ struct EvalArgs {
   int npair;              // off-diagonal quadratic mode pairs (j != k)
   const int* pair;        // 2*npair (j, k)
   const double* qpair;    // npair*A*A = 0.5*Q[j,k]
-  const double* mode_tab; // N records of ModeRec(A, AR).len() doubles (lanes kernel)
+  const double* mode_tab; // N records of ModeRec(A, AR).len() doubles (lanes kernel), then
+                          // zero records up to a multiple of four
   const float* mode_tab_f; // N records of ModeRec(A, AR).len_f() floats (FP32 kernel)
   double* out;            // NUM_OUT rows of out_stride doubles
   int64_t out_stride;

@@ -8,8 +8,10 @@ and squared s times. For a symmetric X every intermediate is a polynomial in X, 
 2-norm error is the largest scalar error over the eigenvalues. Schemes:
   * per-lane FP64 kernel, A <= 4: m = 12 by Paterson-Stockmeyer in X^3,
     alpha = min(|X|_F, |X^3|_F^(1/3)), theta_12 = 0.2996;
-  * per-lane FP64 kernel, 5 <= A <= 8: m = 8 by Horner's rule, alpha = |X|_F, theta_8 = 0.049;
-  * per-lane FP32 kernel: m = 5 by Horner's rule in float32, alpha = |X|_F, theta_5 = 0.13;
+  * per-lane FP64 kernel, 5 <= A <= 8: m = 8 in three products (Sastre's form),
+    alpha = |X^2|_F^(1/2), theta_8 = 0.049 (Horner's rule at degree 8 with alpha = |X|_F, the
+    earlier scheme, is kept as a comparison);
+  * per-lane FP32 kernel: the same three-product form in float32, theta_8 = 0.619;
   * one-wave FP64 kernel (A <= 16): m = 12 by Paterson-Stockmeyer in X^4,
     alpha = max(|X^3|_F^(1/3), |X^4|_F^(1/4)), theta_12 = 0.2996.
 The bound checked is normwise: |M - M_eigh|_max <= 16 (2^s + m) max(1, |X|_2) u |M|_max,
@@ -98,6 +100,29 @@ def expm_ps12_x3(X):
     return T, s
 
 
+S8_COEF = (4.98011920555997349987007158e-3, 1.99204768222398939994802863e-2,
+           7.66526532111914669031909419e-2, 8.76500980178555335977132598e-1,
+           1.22552115011207473091611921e-1, 2.97430720484762666375079572e0)
+
+
+def expm_s8(X, theta=0.049, dtype=np.float64):
+    """Per-lane scheme for 5 <= A <= 8 (FP64, theta_8 = 0.049) and of the FP32 kernel (float32,
+    theta_8 = 0.619): alpha = |X^2|_F^(1/2), T_8 in three products: Y = X2 (c1 X2 + c2 X),
+    T = (Y + c3 X2 + c4 X)(Y + c5 X2) + c6 Y + X2/2 + X + I."""
+    X = np.asarray(X, dtype=dtype)
+    n = X.shape[-1]
+    X2 = X @ X
+    s = _squarings(np.linalg.norm(X2.astype(np.float64), axis=(-2, -1)) ** 0.5, theta)
+    f = np.ldexp(np.ones_like(s, dtype=dtype), -s)[..., None, None]
+    x1, x2 = X * f, X2 * f ** 2
+    c1, c2, c3, c4, c5, c6 = (dtype(c) for c in S8_COEF)
+    Y = x2 @ (c1 * x2 + c2 * x1)
+    T = (Y + c3 * x2 + c4 * x1) @ (Y + c5 * x2) + (c6 * Y + dtype(0.5) * x2 + x1 + np.eye(n, dtype=dtype))
+    for k in range(int(s.max()) if s.size else 0):
+        T = np.where((s > k)[..., None, None], T @ T, T)
+    return T, s
+
+
 def expm_eigh(X):
     lam, U = np.linalg.eigh(np.asarray(X, dtype=np.float64))
     return np.einsum("...ik,...k,...jk->...ij", U, np.exp(lam), U)
@@ -106,7 +131,8 @@ def expm_eigh(X):
 SCHEMES = {  # (function, unit roundoff, degree)
     "lanes_fp64": (expm_ps12_x3, U64, 12),
     "lanes_fp64_h8": (lambda X: expm_horner(X, 8, 0.049, np.float64), U64, 8),
-    "lanes_fp32": (lambda X: expm_horner(X, 5, 0.13, np.float32), U32, 5),
+    "lanes_fp64_s8": (expm_s8, U64, 8),
+    "lanes_fp32": (lambda X: expm_s8(X, 0.619, np.float32), U32, 8),
     "wide_fp64": (expm_ps12, U64, 12),
 }
 
@@ -139,7 +165,7 @@ def _workload_X(A, N, P, T, off, B=24):
 
 @pytest.mark.parametrize("shape,schemes", [
     ((4, 8, 64, 300.0, None), ["lanes_fp64", "lanes_fp32", "wide_fp64"]),
-    ((8, 16, 64, 300.0, None), ["lanes_fp64_h8", "lanes_fp32", "wide_fp64"]),
+    ((8, 16, 64, 300.0, None), ["lanes_fp64_s8", "lanes_fp64_h8", "lanes_fp32", "wide_fp64"]),
     ((12, 24, 128, 250.0, 0.1), ["wide_fp64"]),
 ])
 def test_workload_matrices(shape, schemes):
@@ -180,3 +206,27 @@ def test_special_cases(scheme):
     M = M.astype(np.float64)[0]
     assert np.allclose(np.diag(M), np.exp(d), rtol=16 * (2.0 ** s[0] + m) * u, atol=0.0)
     assert np.all(M[~np.eye(5, dtype=bool)] == 0.0)
+
+
+def test_s8_coefficients_give_the_taylor_polynomial():
+    """The three-product form expands to 1 + x + ... + x^8/8! (each coefficient to 1e-15
+    relative), and each of its six coefficients matters: perturbing any one by 1e-6 moves some
+    Taylor coefficient by more than 1e-8 relative."""
+    from numpy.polynomial import polynomial as npp
+
+    def expand(c):
+        c1, c2, c3, c4, c5, c6 = c
+        x, x2 = [0.0, 1.0], [0.0, 0.0, 1.0]
+        y = npp.polymul(x2, npp.polyadd(npp.polymul([c1], x2), npp.polymul([c2], x)))
+        left = npp.polyadd(npp.polyadd(y, npp.polymul([c3], x2)), npp.polymul([c4], x))
+        right = npp.polyadd(y, npp.polymul([c5], x2))
+        t = npp.polyadd(npp.polymul(left, right), npp.polymul([c6], y))
+        t = npp.polyadd(t, [1.0, 1.0, 0.5])
+        return np.pad(t, (0, 9 - len(t)))[:9]
+
+    taylor = np.array([1.0 / math.factorial(k) for k in range(9)])
+    assert np.max(np.abs(expand(S8_COEF) / taylor - 1.0)) < 1e-15
+    for i in range(6):
+        c = list(S8_COEF)
+        c[i] *= 1.0 + 1e-6
+        assert np.max(np.abs(expand(c) / taylor - 1.0)) > 1e-8, i
