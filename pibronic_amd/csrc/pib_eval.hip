@@ -80,6 +80,25 @@ __device__ __forceinline__ void sym_prod_add(const double (&T)[A][A], const doub
     }
 }
 
+// T <- T^(2^s) for a symmetric T. Two squarings per iteration (T -> Z -> T), so that no
+// register copy follows each one; an odd count ends with one squaring and a copy. s differs
+// between lanes; the loop runs the wave's largest count.
+template <int A>
+__device__ __forceinline__ void square_times(double (&T)[A][A], int s) {
+  double Z[A][A];
+  for (int k = 0; k + 1 < s; k += 2) {
+    sym_prod_lane<A>(T, T, 0.0, Z);
+    sym_prod_lane<A>(Z, Z, 0.0, T);
+  }
+  if (s & 1) {
+    sym_prod_lane<A>(T, T, 0.0, Z);
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int j = 0; j < A; ++j) T[i][j] = Z[i][j];
+  }
+}
+
 constexpr double kLog2Theta12 = -1.738865;  // log2(0.2996)
 
 template <int A>
@@ -131,14 +150,7 @@ __device__ __forceinline__ void expm_lane_ps12(double (&X)[A][A], double (&T)[A]
 #pragma unroll
       for (int k = 0; k < A; ++k) T[i][k] = Z[i][k];
   }
-  for (int k = 0; k < s; ++k) {
-    double Z[A][A];
-    sym_prod_lane<A>(T, T, 0.0, Z);
-#pragma unroll
-    for (int i = 0; i < A; ++i)
-#pragma unroll
-      for (int j = 0; j < A; ++j) T[i][j] = Z[i][j];
-  }
+  square_times<A>(T, s);
 }
 
 // theta_8 = 0.049: below the root of |y|^8 / 9! = u (the degree-8 polynomial's truncation)
@@ -224,14 +236,7 @@ __device__ __forceinline__ void expm_lane_s8(double (&X)[A][A], double (&T)[A][A
       T[i][k] = acc;
       T[k][i] = acc;
     }
-  for (int k = 0; k < s; ++k) {
-    double Z[A][A];
-    sym_prod_lane<A>(T, T, 0.0, Z);
-#pragma unroll
-    for (int i = 0; i < A; ++i)
-#pragma unroll
-      for (int j = 0; j < A; ++j) T[i][j] = Z[i][j];
-  }
+  square_times<A>(T, s);
 }
 
 // Which scheme each surface count uses (index A): Paterson-Stockmeyer degree 12 where its two

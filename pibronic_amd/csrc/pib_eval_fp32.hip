@@ -108,13 +108,20 @@ __device__ __forceinline__ void expm_lane_f(float (&X)[A][A], float (&T)[A][A]) 
       T[i][k] = acc;
       T[k][i] = acc;
     }
-  for (int k = 0; k < s; ++k) {
+  // two squarings per iteration (T -> Z -> T): no register copy after each one
+  {
     float Z[A][A];
-    sym_prod_lane_f<A>(T, T, 0.0f, Z);
+    for (int k = 0; k + 1 < s; k += 2) {
+      sym_prod_lane_f<A>(T, T, 0.0f, Z);
+      sym_prod_lane_f<A>(Z, Z, 0.0f, T);
+    }
+    if (s & 1) {
+      sym_prod_lane_f<A>(T, T, 0.0f, Z);
 #pragma unroll
-    for (int i = 0; i < A; ++i)
+      for (int i = 0; i < A; ++i)
 #pragma unroll
-      for (int j = 0; j < A; ++j) T[i][j] = Z[i][j];
+        for (int j = 0; j < A; ++j) T[i][j] = Z[i][j];
+    }
   }
 }
 
