@@ -112,7 +112,8 @@ int pib_set_omatrix_set(pib_ctx* ctx, int slot, int A_set, int N, const double* 
  * M = exp(-tau V) in every ring product (build_numerator, :1183; quirk Q1). */
 int pib_set_beads(pib_ctx* ctx, int P, double tau);
 
-/* Lanes cooperating on one sample (power of two <= 64), 0 = automatic. Tuning only. */
+/* Lanes cooperating on one sample (power of two <= 64), 0 = automatic. Tuning only. The getter
+ * reports the count the FP64 lanes kernel uses for a block of B samples. */
 int pib_set_lanes_per_sample(pib_ctx* ctx, int lanes);
 int pib_get_lanes_per_sample(pib_ctx* ctx, int64_t B);
 

@@ -58,8 +58,8 @@ int rebuild_mode_table(pib_ctx* c) {
       }
     }
   }
-  // float copy, records padded to 16 bytes
-  std::vector<float> tabf((size_t)N * R.len_f(), 0.f);
+  // float copy, records padded to 16 bytes (and zero records up to N4, as above)
+  std::vector<float> tabf((size_t)N4 * R.len_f(), 0.f);
   for (int j = 0; j < N; ++j)
     for (int i = 0; i < R.lin_minus() + A; ++i)
       tabf[(size_t)j * R.len_f() + i] = (float)tab[(size_t)j * R.len() + i];
@@ -108,7 +108,7 @@ int make_eval_args(pib_ctx* c, int flags, int64_t B, EvalArgs* out_args, bool* w
   a.B = B;
   a.N = c->N;
   a.P = c->P;
-  a.G = c->lanes ? c->lanes : auto_lanes(c, B);
+  a.G = c->lanes ? c->lanes : auto_lanes(c, B, fp32);
   a.K = (c->P + a.G - 1) / a.G;
   a.noscale = (flags & PIB_EVAL_NOSCALE) ? 1 : 0;
   a.tau = c->tau;
@@ -215,6 +215,7 @@ int pib_ctx_destroy(pib_ctx* c) {
   release(c->fb_mu);
   release(c->fb_c);
   release(c->fb_qidx);
+  release(c->fb_qlist);
   release(c->mode_tab_f);
   release(c->s_cdf);
   release(c->s_chol);
@@ -397,7 +398,7 @@ int pib_get_lanes_per_sample(pib_ctx* c, int64_t B) {
   if (!c) return PIB_ERR_ARG;
   if (c->lanes) return c->lanes;
   if (c->P < 3) return fail(c, PIB_ERR_STATE, "beads not set");
-  return auto_lanes(c, B);
+  return auto_lanes(c, B, false);
 }
 
 int pib_eval_block(pib_ctx* c, const double* q, int64_t B, int flags, double* out,
@@ -806,7 +807,7 @@ int pib_flow_set_base(pib_ctx* c, int A_rho, int D, const double* mu, const doub
   // only the feature quads where some mu_a is non-zero are kept (the displacements live in the
   // constant normal mode of each bead mode: N of the D / 4 quads); the draw adds and dots
   // exactly those
-  std::vector<int> qidx(D / 4, -1);
+  std::vector<int> qidx(D / 4, -1), qlist;
   std::vector<double> mq;
   int nq = 0;
   for (int m = 0; m < D / 4; ++m) {
@@ -815,13 +816,16 @@ int pib_flow_set_base(pib_ctx* c, int A_rho, int D, const double* mu, const doub
       for (int e = 0; e < 4; ++e) nz = nz || mu[(size_t)b * D + 4 * m + e] != 0.0;
     if (!nz) continue;
     qidx[m] = nq++;
+    qlist.push_back(m);
     for (int b = 0; b < A_rho; ++b)
       for (int e = 0; e < 4; ++e) mq.push_back(mu[(size_t)b * D + 4 * m + e]);
   }
-  if (mq.empty()) mq.assign(4 * A_rho, 0.0);  // keep a valid buffer
+  if (mq.empty()) mq.assign(4 * A_rho, 0.0);  // keep valid buffers
+  if (qlist.empty()) qlist.push_back(0);
   int r;
   if ((r = upload(c, c->fb_mu, mq.data(), mq.size() * sizeof(double)))) return r;
   if ((r = upload(c, c->fb_qidx, qidx.data(), qidx.size() * sizeof(int)))) return r;
+  if ((r = upload(c, c->fb_qlist, qlist.data(), qlist.size() * sizeof(int)))) return r;
   if ((r = upload(c, c->fb_c, cc, (size_t)A_rho * sizeof(double)))) return r;
   c->fb_A = A_rho;
   c->fb_D = D;
@@ -848,6 +852,7 @@ int pib_flow_draw_base(pib_ctx* c, uint64_t seed, int64_t offset, int64_t B, flo
   a.cdf = (const double*)c->s_cdf.p;
   a.mu = (const double*)c->fb_mu.p;
   a.qidx = (const int*)c->fb_qidx.p;
+  a.qlist = (const int*)c->fb_qlist.p;
   a.nq = c->fb_nq;
   a.c = (const double*)c->fb_c.p;
   a.z = z;

@@ -290,7 +290,7 @@ int pib_batch_create(int n, pib_ctx* const* ctxs, int64_t B, int flags, double d
     if (wide || (i > 0 && (A != b->A || AR != b->AR))) b->lane_ok = false;
     if (i == 0) { b->A = A; b->AR = AR; }
     if (A > kWideMaxStates || AR > kWideMaxStates) b->wide_ok = false;
-    b->lanes[i] = auto_lanes(c, B);
+    b->lanes[i] = auto_lanes(c, B, (flags & PIB_EVAL_FP32) != 0);
   }
   if (flags & PIB_EVAL_FP32) b->wide_ok = false;  // FP32 runs as one launch per problem
   if (n > kBatchLaunchProbs) b->lane_ok = false;   // the lanes kernel takes its problems by value

@@ -53,6 +53,33 @@ __device__ __forceinline__ const_as<T>* as_const(const T* p) {
   return (const_as<T>*)p;
 }
 
+// One wave's bead coordinates (samples [s0, s0 + n) of a (B, N*P) double array) through a
+// buffer resource: base and size wave-uniform, loads past the end return 0. The evaluation
+// kernels' mode loops load each mode's coordinates ahead of their use; with plain loads the
+// optimizer folds the loop's phi of two loads (the prologue's and the previous iteration's)
+// into one load of a phi of addresses at the top of the iteration, right before the use, so a
+// wave waited a full memory latency per mode. The buffer-load intrinsic is a call, which that
+// fold leaves alone.
+struct CoordBuf {
+  __amdgpu_buffer_rsrc_t rsrc;
+  int base;  // this lane's sample: index of its first coordinate in the resource (doubles)
+  __device__ __forceinline__ double operator[](int i) const {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, (base + i) * 8, 0, 0));
+  }
+  // the wave's samples [first, first + per_wave) (first made wave-uniform explicitly); `sample`
+  // is this lane's sample, or -1 for a lane past the block (which reads the wave's first)
+  __device__ __forceinline__ static CoordBuf wave_samples(const double* q, int64_t B, int64_t NP, int64_t first,
+                                                          int per_wave, int64_t sample) {
+    // gfx9 buffer descriptor word 3: 32-bit data format, raw (stride 0) addressing
+    constexpr int kWord3 = 0x00020000;
+    const int64_t s0 = ((int64_t)__builtin_amdgcn_readfirstlane((int)((uint64_t)first >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)first);
+    const int64_t n = B - s0 < per_wave ? (B - s0 > 0 ? B - s0 : 0) : per_wave;
+    return CoordBuf{__builtin_amdgcn_make_buffer_rsrc((void*)(q + s0 * NP), (short)0, (int)(n * NP * 8), kWord3),
+                    sample >= 0 ? (int)((sample - s0) * NP) : 0};
+  }
+};
+
 // 53-bit uniform in [0, 1) from two 32-bit words.
 __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
   const uint64_t m = ((uint64_t)(hi >> 5) << 26) | (uint64_t)(lo >> 6);

@@ -61,8 +61,8 @@ struct pib_ctx {
   // flow basis
   int f_N = 0, f_P = 0;
   pib::DevBuf f_U, f_sigma, f_isigma, f_m;
-  pib::DevBuf fb_mu, fb_c, fb_qidx;  // flow base distribution (pib_flow_set_base): the non-zero
-                                     // feature quads of mu and their index map
+  pib::DevBuf fb_mu, fb_c, fb_qidx, fb_qlist;  // flow base distribution (pib_flow_set_base): the
+                                               // non-zero feature quads of mu and their index maps
   int fb_A = 0, fb_D = 0, fb_nq = 0;
 };
 
@@ -132,18 +132,23 @@ inline bool finite_all(const double* v, size_t n) {
   return true;
 }
 
-inline int auto_lanes(const pib_ctx* c, int64_t B) {
-  // the fewest lanes per sample that give ~16 waves per SIMD over the launch; each doubling
-  // adds a level to the cross-lane product tree. Measured on MI355X (eval ms at 4/8/16/32 lanes):
-  // A=4, N=8, P=64, 1e5 samples 0.95/0.48/0.31/0.34, 65,536 samples 0.61/0.31/0.21/0.22;
-  // A=8, N=16, P=64, 1e5 samples 2.70/2.44/2.33/2.88, 65,536 samples 1.74/1.56/1.51/1.90 (16 in
-  // every case; a target of 24 waves per SIMD picked 32 at 65,536). Never more lanes per sample
-  // than beads, so that each lane owns at least one bead.
-  const int64_t target = (int64_t)c->num_cu * 4 * kWave * 16;
+inline int auto_lanes(const pib_ctx* c, int64_t B, bool fp32) {
+  // The fewest lanes per sample (a power of two, each doubling adds a level to the cross-lane
+  // product tree) that give the launch enough waves and each lane at most kmax beads. The FP64
+  // kernel at A >= 7 runs one wave per SIMD (its register budget): 8 waves per SIMD and up to 8
+  // beads per lane; every other kernel 16 waves per SIMD and up to 4 beads. Measured on MI355X
+  // (eval ms at 4/8/16/32 lanes): FP64 A=8, N=16, P=64, 65,536 samples 1.43/1.18/1.27/1.53,
+  // 1e5 2.25/1.81/1.93/2.29; FP64 A=4, N=8, P=64, 65,536 0.62/0.30/0.19/0.21, 1e5
+  // 0.98/0.46/0.29/0.33; FP64 A=6, N=12, P=128, 1e5 2.95/2.69/1.46/1.39; FP32 A=6, N=12, P=128,
+  // 1e5 2.86/2.65/1.34/0.83; FP32 A=8, N=16, P=64, 65,536 1.30/0.68/0.57/0.63. Never more lanes
+  // per sample than beads, so that each lane owns at least one bead.
+  const bool one_wave = !fp32 && c->A >= 7;
+  const int64_t target = (int64_t)c->num_cu * 4 * kWave * (one_wave ? 8 : 16);
+  const int kmax = one_wave ? 8 : 4;
   int maxg = 1;
   while (maxg < 64 && maxg * 2 <= c->P) maxg *= 2;
   int g = 1;
-  while (g < maxg && B * g < target) g *= 2;
+  while (g < maxg && (B * g < target || (c->P + g - 1) / g > kmax)) g *= 2;
   return g;
 }
 

@@ -255,23 +255,6 @@ __device__ __forceinline__ void expm_lane(double (&X)[A][A], double (&T)[A][A]) 
     expm_lane_s8<A>(X, T);
 }
 
-// One wave's coordinates through a buffer resource (base and size wave-uniform; loads past the
-// end return 0). The mode loop loads each mode's coordinates one or two modes ahead of their
-// use; with plain loads the optimizer folds the loop's phi of two loads (the prologue's and the
-// previous iteration's) into one load of a phi of addresses at the top of the iteration, right
-// before the use, so a wave waited a full memory latency per mode (at A = 8, one wave per
-// SIMD, nothing covers it). The buffer-load intrinsic is a call, which that fold leaves alone.
-struct CoordBuf {
-  __amdgpu_buffer_rsrc_t rsrc;
-  int base;  // this lane's sample: index of its first coordinate in the resource (doubles)
-  __device__ __forceinline__ double operator[](int i) const {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, (base + i) * 8, 0, 0));
-  }
-};
-
-// gfx9 buffer descriptor word 3: 32-bit data format, raw (stride 0) addressing
-constexpr int kBufferWord3 = 0x00020000;
-
 template <int A, int AR, bool PM, bool QD>
 __device__ __forceinline__ void mode_terms(const_as<double>* tab, const CoordBuf& qs, int N,
                                            int P, int p, int pn, double& tr, double& tv, double& tp,
@@ -364,13 +347,7 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
   const bool live = sample < args.B;
   const int64_t NP = (int64_t)N * P;
   const double* __restrict__ qs = args.q + (live ? sample : 0) * NP;
-  // the wave's samples [s0, s0 + nsw) as one buffer resource (s0 made wave-uniform explicitly)
-  const int64_t s0 = ((int64_t)__builtin_amdgcn_readfirstlane((int)((uint64_t)(wave * spw) >> 32)) << 32) |
-                     (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(wave * spw));
-  const int64_t nsw = args.B - s0 < spw ? (args.B - s0 > 0 ? args.B - s0 : 0) : spw;
-  const CoordBuf qb{__builtin_amdgcn_make_buffer_rsrc((void*)(args.q + s0 * NP), (short)0, (int)(nsw * NP * 8),
-                                                      kBufferWord3),
-                    live ? (int)((sample - s0) * NP) : 0};
+  const CoordBuf qb = CoordBuf::wave_samples(args.q, args.B, NP, wave * spw, spw, live ? sample : -1);
 
   // read-only constants through the constant address space: scalar loads
   struct RO {

@@ -126,20 +126,13 @@ __device__ __forceinline__ void expm_lane_f(float (&X)[A][A], float (&T)[A][A]) 
 }
 
 template <int A, int AR, bool PM, bool QD>
-__device__ __forceinline__ void mode_terms_f(const float* __restrict__ tab, const double* __restrict__ qs, int N,
-                                             int P, int p, int pn, float& tr, float& tv, float& tp,
-                                             float& tm, float (&er)[AR], float (&ev)[A], float (&ep)[A],
-                                             float (&em)[A], float (&v)[A][A]) {
+__device__ __forceinline__ void mode_terms_f(const float* __restrict__ tab, const CoordBuf& qs, int N, int P,
+                                             int p, int pn, float& tr, float& tv, float& tp, float& tm,
+                                             float (&er)[AR], float (&ev)[A], float (&ep)[A], float (&em)[A],
+                                             float (&v)[A][A]) {
   constexpr ModeRec R(A, AR);
-  // coordinates loaded two modes ahead, the loop unrolled by two so that the rotation needs
-  // no register copy (a copy of a value still in flight would wait for the load just issued)
-  const int j1 = N > 1 ? 1 : 0;
-  double x0d = qs[p], xnd = qs[pn], x0_1 = qs[j1 * P + p], xn_1 = qs[j1 * P + pn];
-#pragma unroll 2
-  for (int j = 0; j < N; ++j) {
+  auto mode = [&](int j, double x0d, double xnd) {
     const float* __restrict__ t = tab + j * R.len_f();
-    const int j2 = (j + 2 < N) ? j + 2 : N - 1;
-    const double x0_2 = qs[j2 * P + p], xn_2 = qs[j2 * P + pn];
     const float x0 = (float)x0d, xn = (float)xnd;
     const float sx = x0 + xn, dx = x0 - xn;
     const float q2s = fmaf(x0, x0, xn * xn), d2 = dx * dx;
@@ -167,10 +160,33 @@ __device__ __forceinline__ void mode_terms_f(const float* __restrict__ tab, cons
         v[b][c] = fmaf(t[R.lin_q() + e], x0, v[b][c]);
         if (QD) v[b][c] = fmaf(t[R.quad() + e], xx, v[b][c]);
       }
-    x0d = x0_1;
-    xnd = xn_1;
-    x0_1 = x0_2;
-    xn_1 = xn_2;
+  };
+  // the FP64 kernel's schedule (see mode_terms there): eight modes per iteration in two halves
+  // of four, one half's coordinates loaded while the other half's arithmetic runs; modes past N
+  // read the zero records the table is padded with
+  auto ld4 = [&](int j, double (&x)[4][2]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int jj = j + k < N ? j + k : N - 1;
+      x[k][0] = qs[jj * P + p];
+      x[k][1] = qs[jj * P + pn];
+    }
+  };
+  const int N4 = (N + 3) & ~3;
+  double xa[4][2], xb[4][2];
+  ld4(0, xa);
+  int j = 0;
+  for (; j + 8 <= N4; j += 8) {
+    ld4(j + 4, xb);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) mode(j + k, xa[k][0], xa[k][1]);
+    ld4(j + 8, xa);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) mode(j + 4 + k, xb[k][0], xb[k][1]);
+  }
+  if (j < N4) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) mode(j + k, xa[k][0], xa[k][1]);
   }
 }
 
@@ -225,6 +241,8 @@ __global__ void __launch_bounds__(256) k_eval_fp32(const EvalArgs args) {
   const int64_t sample = wave * (kWave / G) + lane / G;
   const bool live = sample < args.B;
   const double* __restrict__ qs = args.q + (live ? sample : 0) * (int64_t)N * P;
+  const CoordBuf qb = CoordBuf::wave_samples(args.q, args.B, (int64_t)N * P, wave * (kWave / G), kWave / G,
+                                             live ? sample : -1);
   const OSetF fr = args.setf[SET_RHO], fv = args.setf[SET_VIB];
   const OSetF fp = args.setf[SET_VIB_PLUS], fm = args.setf[SET_VIB_MINUS];
   const float* __restrict__ Ev = args.energy_f;
@@ -262,9 +280,9 @@ __global__ void __launch_bounds__(256) k_eval_fp32(const EvalArgs args) {
       for (int c = 0; c <= a; ++c) v[a][c] = 0.0f;
     }
     if (has_q)
-      mode_terms_f<A, AR, PM, true>(args.mode_tab_f, qs, N, P, p, pn, tr, tv, tpl, tmi, er, ev, epl, emi, v);
+      mode_terms_f<A, AR, PM, true>(args.mode_tab_f, qb, N, P, p, pn, tr, tv, tpl, tmi, er, ev, epl, emi, v);
     else
-      mode_terms_f<A, AR, PM, false>(args.mode_tab_f, qs, N, P, p, pn, tr, tv, tpl, tmi, er, ev, epl, emi, v);
+      mode_terms_f<A, AR, PM, false>(args.mode_tab_f, qb, N, P, p, pn, tr, tv, tpl, tmi, er, ev, epl, emi, v);
     for (int t = 0; t < args.npair; ++t) {
       const int jj = args.pair[2 * t], kk = args.pair[2 * t + 1];
       const float xj = (float)qs[jj * P + p], xk = (float)qs[kk * P + p];

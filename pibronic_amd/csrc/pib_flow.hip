@@ -272,24 +272,54 @@ __device__ __forceinline__ double group16_sum(double v) {
   return v;
 }
 
+// The four normals of feature quad m of global sample gs: two Box-Muller pairs from one Philox
+// call (32-bit uniforms, the radius one refined to 64 bits from a second call when it is below
+// 2^-20, so that the tails reach ~9.4 sigma; a 32-bit radius uniform alone stops at 6.7 sigma),
+// the transform in FP32 with the hardware log2, sqrt and sin/cos (z is float)
+__device__ __forceinline__ void base_normals(int64_t gs, int m, uint32_t k0, uint32_t k1, float (&zo)[4]) {
+  const u32x4 r = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), (uint32_t)m, 2u}, k0, k1);
+  float l2a = __log2f(((float)r.x + 0.5f) * 0x1.0p-32f);
+  float l2b = __log2f(((float)r.z + 0.5f) * 0x1.0p-32f);
+  if (r.x < (1u << 12) || r.z < (1u << 12)) {  // rare: P = 2^-19 per call
+    const u32x4 t = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), (uint32_t)m, 3u}, k0, k1);
+    if (r.x < (1u << 12)) l2a = (float)log2(((double)r.x * 0x1.0p32 + (double)t.x + 0.5) * 0x1.0p-64);
+    if (r.z < (1u << 12)) l2b = (float)log2(((double)r.z * 0x1.0p32 + (double)t.z + 0.5) * 0x1.0p-64);
+  }
+  const float ra = __fsqrt_rn(-1.3862943611198906f * l2a), rb = __fsqrt_rn(-1.3862943611198906f * l2b);
+  const float ua = ((float)r.y + 0.5f) * 0x1.0p-32f, ub = ((float)r.w + 0.5f) * 0x1.0p-32f;
+  // v_cos_f32 / v_sin_f32 take their argument in revolutions
+  zo[0] = ra * __builtin_amdgcn_cosf(ua);
+  zo[1] = ra * __builtin_amdgcn_sinf(ua);
+  zo[2] = rb * __builtin_amdgcn_cosf(ub);
+  zo[3] = rb * __builtin_amdgcn_sinf(ub);
+}
+
 // The displacements mu are non-zero only in a few feature quads (the constant normal mode of
-// each bead mode); qidx maps a quad to its row of the compact mu (nq * A_rho * 4) or -1. Both
-// are staged in LDS once per work-group when they fit in 64 KB (MU_LDS), the work-groups then
-// looping over groups of sixteen samples. Quads with qidx < 0 take z = eps (the added mu is 0)
-// and contribute nothing to z . mu_b.
-template <bool MU_LDS>
+// each bead mode, N of the D / 4 quads). qidx maps a quad to its row of the compact mu
+// (nq * A_rho * 4) or -1, qlist a row to its quad. The lanes first draw every other quad
+// (z = eps; no displacement, no contribution to z . mu_b), then the displaced quads, one per
+// lane and pass (z = eps + mu_src, and z . mu_b): with the displaced quads inside the main loop,
+// the lane that met one (lane 0 of each sample for P = 64, every step) held its whole wave in a
+// branch of FP64 dot products. mu, qidx and qlist are staged in LDS once per work-group when
+// they fit in 64 KB (MU_LDS), the work-groups then looping over groups of sixteen samples.
+// MAXA >= A_rho bounds the per-surface sums.
+template <bool MU_LDS, int MAXA>
 __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
   extern __shared__ double lds[];
   const double* __restrict__ mu = a.mu;
   const int* __restrict__ qidx = a.qidx;
+  const int* __restrict__ qlist = a.qlist;
   if constexpr (MU_LDS) {
     const int nmu = a.nq * a.A_rho * 4;
     for (int i = threadIdx.x; i < nmu; i += 256) lds[i] = a.mu[i];
     int* qi_lds = (int*)(lds + nmu);
     for (int i = threadIdx.x; i < a.D / 4; i += 256) qi_lds[i] = a.qidx[i];
+    int* ql_lds = qi_lds + a.D / 4;
+    for (int i = threadIdx.x; i < a.nq; i += 256) ql_lds[i] = a.qlist[i];
     __syncthreads();
     mu = lds;
     qidx = qi_lds;
+    qlist = ql_lds;
   }
   const int AR = a.A_rho;
   const int sl = threadIdx.x & (kBaseLanes - 1);
@@ -305,54 +335,41 @@ __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
       while (src < AR - 1 && u >= a.cdf[src]) ++src;
     }
     float* zr = a.z + (live ? s : 0) * a.D;
-    double zz = 0.0, dot[kBaseMaxA];
+    double zz = 0.0, dot[MAXA];
 #pragma unroll
-    for (int b = 0; b < kBaseMaxA; ++b) dot[b] = 0.0;
+    for (int b = 0; b < MAXA; ++b) dot[b] = 0.0;
     if (live) {
-      // four features (two Box-Muller pairs) per Philox call: 32-bit uniforms, the radius one
-      // refined to 64 bits from a second call when it is below 2^-20, so that the tails reach
-      // ~9.4 sigma (a 32-bit radius uniform alone would stop at 6.7 sigma); the transform in
-      // FP32 with the hardware log2, sqrt and sin/cos (z is float)
       for (int m = sl; 4 * m < a.D; m += kBaseLanes) {
-        const u32x4 r = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), (uint32_t)m, 2u}, a.key0, a.key1);
-        float l2a = __log2f(((float)r.x + 0.5f) * 0x1.0p-32f);
-        float l2b = __log2f(((float)r.z + 0.5f) * 0x1.0p-32f);
-        if (r.x < (1u << 12) || r.z < (1u << 12)) {  // rare: P = 2^-19 per call
-          const u32x4 t = philox4x32_10(u32x4{(uint32_t)gs, (uint32_t)(gs >> 32), (uint32_t)m, 3u}, a.key0, a.key1);
-          if (r.x < (1u << 12)) l2a = (float)log2(((double)r.x * 0x1.0p32 + (double)t.x + 0.5) * 0x1.0p-64);
-          if (r.z < (1u << 12)) l2b = (float)log2(((double)r.z * 0x1.0p32 + (double)t.z + 0.5) * 0x1.0p-64);
+        float zo[4];
+        base_normals(gs, m, a.key0, a.key1, zo);
+        if (qidx[m] < 0) {
+          *(float4*)(zr + 4 * m) = float4{zo[0], zo[1], zo[2], zo[3]};
+#pragma unroll
+          for (int e = 0; e < 4; ++e) zz = fma((double)zo[e], (double)zo[e], zz);
         }
-        const float ra = __fsqrt_rn(-1.3862943611198906f * l2a), rb = __fsqrt_rn(-1.3862943611198906f * l2b);
-        const float ua = ((float)r.y + 0.5f) * 0x1.0p-32f, ub = ((float)r.w + 0.5f) * 0x1.0p-32f;
-        // v_cos_f32 / v_sin_f32 take their argument in revolutions
-        float zo[4] = {ra * __builtin_amdgcn_cosf(ua), ra * __builtin_amdgcn_sinf(ua),
-                       rb * __builtin_amdgcn_cosf(ub), rb * __builtin_amdgcn_sinf(ub)};
-        const int qi = qidx[m];
-        if (qi >= 0) {
-          const double* __restrict__ mq = mu + (int64_t)qi * AR * 4;
-#pragma unroll
-          for (int e = 0; e < 4; ++e) zo[e] = (float)((double)zo[e] + mq[src * 4 + e]);
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const double d = zo[e];
-#pragma unroll
-            for (int b = 0; b < kBaseMaxA; ++b)
-              if (b < AR) dot[b] = fma(d, mq[b * 4 + e], dot[b]);
-          }
-        }
-        *(float4*)(zr + 4 * m) = float4{zo[0], zo[1], zo[2], zo[3]};
+      }
+      for (int t = sl; t < a.nq; t += kBaseLanes) {
+        const int m = qlist[t];
+        float zo[4];
+        base_normals(gs, m, a.key0, a.key1, zo);
+        const double* __restrict__ mq = mu + (int64_t)t * AR * 4;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
+          zo[e] = (float)((double)zo[e] + mq[src * 4 + e]);
           const double d = zo[e];
           zz = fma(d, d, zz);
+#pragma unroll
+          for (int b = 0; b < MAXA; ++b)
+            if (b < AR) dot[b] = fma(d, mq[b * 4 + e], dot[b]);
         }
+        *(float4*)(zr + 4 * m) = float4{zo[0], zo[1], zo[2], zo[3]};
       }
     }
     zz = group16_sum(zz);
-    double mx = -INFINITY, t[kBaseMaxA];
+    double mx = -INFINITY, t[MAXA];
 #pragma unroll
-    for (int b = 0; b < kBaseMaxA; ++b) {
-      if (b < a.A_rho) {
+    for (int b = 0; b < MAXA; ++b) {
+      if (b < AR) {
         t[b] = a.c[b] + group16_sum(dot[b]);
         mx = fmax(mx, t[b]);
       }
@@ -360,26 +377,33 @@ __global__ void __launch_bounds__(256) k_flow_base(BaseArgs a) {
     if (live && sl == 0) {
       double acc = 0.0;
 #pragma unroll
-      for (int b = 0; b < kBaseMaxA; ++b)
-        if (b < a.A_rho) acc += exp(t[b] - mx);
+      for (int b = 0; b < MAXA; ++b)
+        if (b < AR) acc += exp(t[b] - mx);
       a.logrho[s] = mx + log(acc) - 0.5 * zz;
     }
   }
 }
 
-hipError_t launch_flow_base(const BaseArgs& a, hipStream_t s) {
-  if (a.B <= 0 || a.D <= 0 || (a.D & 3) || a.A_rho < 1 || a.A_rho > kBaseMaxA || a.nq < 0) return hipErrorInvalidValue;
+template <int MAXA>
+static hipError_t launch_flow_base_a(const BaseArgs& a, hipStream_t s) {
   const int64_t groups = (a.B + 256 / kBaseLanes - 1) / (256 / kBaseLanes);
-  const size_t mu_bytes = (size_t)a.nq * a.A_rho * 4 * sizeof(double) + (size_t)(a.D / 4) * sizeof(int);
+  const size_t mu_bytes = (size_t)a.nq * a.A_rho * 4 * sizeof(double) + (size_t)(a.D / 4 + a.nq) * sizeof(int);
   if (mu_bytes <= 65536) {
     // a few work-groups per CU, each staging mu once
     const int64_t grid = std::min<int64_t>(groups, 2048);
-    hipLaunchKernelGGL(k_flow_base<true>, dim3((unsigned)grid), dim3(256), mu_bytes, s, a);
+    hipLaunchKernelGGL((k_flow_base<true, MAXA>), dim3((unsigned)grid), dim3(256), mu_bytes, s, a);
   } else {
     if (groups > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_flow_base<false>, dim3((unsigned)groups), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_flow_base<false, MAXA>), dim3((unsigned)groups), dim3(256), 0, s, a);
   }
   return hipGetLastError();
+}
+
+hipError_t launch_flow_base(const BaseArgs& a, hipStream_t s) {
+  if (a.B <= 0 || a.D <= 0 || (a.D & 3) || a.A_rho < 1 || a.A_rho > kBaseMaxA || a.nq < 0) return hipErrorInvalidValue;
+  if (a.A_rho <= 4) return launch_flow_base_a<4>(a, s);
+  if (a.A_rho <= 8) return launch_flow_base_a<8>(a, s);
+  return launch_flow_base_a<16>(a, s);
 }
 
 // ------------------------------------------------------------------ block reduction helper

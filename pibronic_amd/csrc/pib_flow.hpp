@@ -74,6 +74,7 @@ struct BaseArgs {
   const double* cdf;  // A_rho cumulative surface weights
   const double* mu;   // the non-zero feature quads of the displacements: nq * A_rho * 4
   const int* qidx;    // D / 4: index of a feature quad in mu, or -1 where every mu_a is 0 there
+  const int* qlist;   // nq: the feature quad of each row of mu
   int nq;             // feature quads with a non-zero displacement
   const double* c;    // A_rho
   float* z;           // B * D

@@ -56,7 +56,8 @@ struct EvalArgs {
   const double* qpair;    // npair*A*A = 0.5*Q[j,k]
   const double* mode_tab; // N records of ModeRec(A, AR).len() doubles (lanes kernel), then
                           // zero records up to a multiple of four
-  const float* mode_tab_f; // N records of ModeRec(A, AR).len_f() floats (FP32 kernel)
+  const float* mode_tab_f; // N records of ModeRec(A, AR).len_f() floats (FP32 kernel), then
+                           // zero records up to a multiple of four
   double* out;            // NUM_OUT rows of out_stride doubles
   int64_t out_stride;
   double* dbg;            // optional per-(sample, bead) dump, see eval_debug_record()
