@@ -244,7 +244,7 @@ def harmonic_leg(data, B, nblocks, W, seed, flags, pm):
     return res
 
 
-def eval_roofline(A, N, P, nq, pm, B, eval_ms, kernel, traffic=None, tdetail=None):
+def eval_roofline(A, N, P, nq, pm, B, eval_ms, kernel, traffic=None, tdetail=None, coord_bytes=8):
     fl = algorithmic_flops(A, N, P, nq, pm)
     achieved = fl["eval"] * B / (eval_ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -253,13 +253,15 @@ def eval_roofline(A, N, P, nq, pm, B, eval_ms, kernel, traffic=None, tdetail=Non
             "hbm_frac": (traffic / (eval_ms * 1e-3) / HBM_PEAK_BPS) if traffic else None,
             "kernel": kernel, "launch_samples": B, "kernel_ms": eval_ms,
             "flops_per_sample": fl["eval"], "flops_per_launch": fl["eval"] * B,
-            "algorithmic_bytes": B * N * P * 8 + B * (5 if pm else 3) * 8,
+            "algorithmic_bytes": B * N * P * coord_bytes + B * (5 if pm else 3) * 8,
             "traffic_detail": tdetail or "not profiled for this launch size",
             "peak_note": "78.6 TFLOP/s: MI355X FP64 vector (= FP64 matrix) peak"}
 
 
 def timed_draws(fl, data, B, K, W, seed, variant, eval_events=True):
-    """K timed steps: flow draw + FP64 g + block sums with q as the denominator."""
+    """K timed steps: flow draw + FP64 g + block sums with q as the denominator. The draws come
+    as float offsets from the basis centre where the FP64 lanes kernel takes them (the same
+    coordinates in FP64, half the bytes), else as R."""
     import torch
     from pibronic_amd import _native
     out = torch.empty((_native.NUM_OUT, B), dtype=torch.float64, device="cuda")
@@ -268,16 +270,21 @@ def timed_draws(fl, data, B, K, W, seed, variant, eval_events=True):
     stream = torch.cuda.current_stream()
     mk = lambda: [torch.cuda.Event(enable_timing=True) for _ in range(K)]  # noqa: E731
     d0, d1, l0, l1, g0, g1 = mk(), mk(), mk(), mk(), mk(), mk()
+    offsets = fl.offsets_ok() and not data.fp32
 
     def step(i, timed):
         j = i - W
         if timed:
             d0[j].record(stream)
-        R, logq = fl.draw(B, seed, (1 << 40) + i * B, stream, layer_events=(l0[j], l1[j]) if timed else None)
+        R, logq = fl.draw(B, seed, (1 << 40) + i * B, stream, layer_events=(l0[j], l1[j]) if timed else None,
+                          offsets=offsets)
         if timed:
             d1[j].record(stream)
             g0[j].record(stream)
-        data.evaluate_device(R, B, False, out, B, stream=stream)
+        if offsets:
+            data.evaluate_offsets_device(R, B, out, B, stream=stream)
+        else:
+            data.evaluate_device(R, B, False, out, B, stream=stream)
         if timed:
             g1[j].record(stream)
         ctx.block_sums_q(out, B, B, False, 1.0, logq, sums[i], stream)
@@ -292,7 +299,8 @@ def timed_draws(fl, data, B, K, W, seed, variant, eval_events=True):
     el = time.perf_counter() - t1
     avg = lambda a, b: sum(x.elapsed_time(y) for x, y in zip(a, b)) / K  # noqa: E731
     res = weight_summary(sums[W:].sum(dim=0).cpu().numpy(), el)
-    res.update(variant=variant, steps=K, samples_per_step=B, ms_per_step=el / K * 1e3,
+    res.update(variant=variant, coordinates="float offsets from the basis centre" if offsets else "R (double)",
+               steps=K, samples_per_step=B, ms_per_step=el / K * 1e3,
                flow_draw_ms=avg(d0, d1), layers_ms=avg(l0, l1), eval_kernel_ms=avg(g0, g1))
     return res
 
@@ -646,9 +654,11 @@ def main():
 
     harm_sbs = harm["samples"] * P / harm["seconds"]
     if learned:
-        roof = eval_roofline(A, N, P, nq, False, B_l, learned["eval_kernel_ms"], kname,
+        off = learned.get("coordinates", "").startswith("float offsets")
+        kfull = f"void pib::k_eval_fp64_offsets<{A}>" if off else f"void pib::k_eval_fp64<{A}, {A}, false, false>"
+        roof = eval_roofline(A, N, P, nq, False, B_l, learned["eval_kernel_ms"], kfull[len("void pib::"):],
                              *profiled_traffic(os.path.join(ROOT, "profiles", f"k_eval_fp64_A{A}_N{N}_P{P}_B{B_l}.md"),
-                                               f"void pib::k_eval_fp64<{A}, {A}, false, false>"))
+                                               kfull), coord_bytes=4 if off else 8)
         roof["step_note"] = "the FP64 evaluation of g inside the timed learned step (its largest kernel)"
         value = learned["ess_per_s"]
     else:

@@ -112,6 +112,16 @@ int pib_set_omatrix_set(pib_ctx* ctx, int slot, int A_set, int N, const double* 
  * M = exp(-tau V) in every ring product (build_numerator, :1183; quirk Q1). */
 int pib_set_beads(pib_ctx* ctx, int P, double tau);
 
+/* pib_eval_block for coordinates given as float offsets dq_dev (B*N*P) from the per-mode centre
+ * m of pib_flow_set_basis: bead coordinate (s, j, p) = m[j] + (double)dq[(s N + j) P + p],
+ * formed in FP64 inside the kernel -- the same doubles pib_flow_whiten(PIB_FLOW_UNWHITEN)
+ * writes, so the outputs equal pib_eval_block's on those, bit for bit. The FP64 lanes kernel
+ * (A <= 8, A_rho = A) only; flags 0 or PIB_EVAL_NOSCALE. No counterpart in the reference
+ * (whose coordinates are float64 arrays); for the learned proposal's draws, whose unwhitening
+ * then writes and this reads half the bytes. */
+int pib_eval_block_flow_offsets(pib_ctx* ctx, const float* dq_dev, int64_t B, int flags, double* out_dev,
+                                int64_t out_stride, void* stream);
+
 /* Lanes cooperating on one sample (power of two <= 64), 0 = automatic. Tuning only. The getter
  * reports the count the FP64 lanes kernel uses for a block of B samples. */
 int pib_set_lanes_per_sample(pib_ctx* ctx, int lanes);
@@ -186,6 +196,9 @@ int pib_block_sums_noalloc(pib_ctx* ctx, const double* out_dev, int64_t out_stri
 #define PIB_FLOW_WHITEN 0   /* R (double, B*N*P) -> zp (float, B*N*P), Z = (R - m) U / sigma  */
 #define PIB_FLOW_UNWHITEN 1 /* zp (float) -> R (double) = m + (Z sigma) U^T                   */
 #define PIB_FLOW_GRAD 2     /* dR (float) -> dzp (float) = (dR U) sigma: dL/dZ from dL/dR     */
+#define PIB_FLOW_UNWHITEN_OFFSETS 3 /* zp (float) -> dR (float) = (Z sigma) U^T, so that the
+                                     * coordinates are R = m + dR (P = 64 or 128): the input of
+                                     * pib_eval_block_flow_offsets, at half the bytes of R */
 
 /* C = op(A) op(B) (+ bias[N]) (relu) (* [mask > 0]) (+ C if accumulate), FP32 inputs and
  * accumulation on MFMA. M, N multiples of 64, K a multiple of 16. */

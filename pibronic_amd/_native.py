@@ -43,6 +43,7 @@ SIGNATURES = {
     "pib_set_lanes_per_sample": (_c_int, [_vp, _c_int]),
     "pib_get_lanes_per_sample": (_c_int, [_vp, _c_i64]),
     "pib_eval_block": (_c_int, [_vp, _vp, _c_i64, _c_int, _vp, _c_i64, _vp, _vp]),
+    "pib_eval_block_flow_offsets": (_c_int, [_vp, _vp, _c_i64, _c_int, _vp, _c_i64, _vp]),
     "pib_set_sampler": (_c_int, [_vp, _c_int, _c_int, _c_int, _dp, _dp, _dp, _dp]),
     "pib_sample_block": (_c_int, [_vp, _c_u64, _c_i64, _c_i64, _vp, _vp, _vp]),
     "pib_block_sums": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_int, _c_dbl, _vp, _vp]),
@@ -86,7 +87,7 @@ SIGNATURES = {
     "pib_batch_run": (_c_int, [_vp, _c_u64, _c_i64, _c_int, _vp]),
 }
 
-FLOW_WHITEN, FLOW_UNWHITEN, FLOW_GRAD = 0, 1, 2
+FLOW_WHITEN, FLOW_UNWHITEN, FLOW_GRAD, FLOW_UNWHITEN_OFFSETS = 0, 1, 2, 3
 
 _lib = None
 
@@ -209,6 +210,11 @@ class Context:
     def eval_block(self, q, B, flags, out, out_stride, debug=None, stream=None):
         self._check(self.lib.pib_eval_block(self.h, _devptr(q), int(B), int(flags), _devptr(out),
                                             int(out_stride), _devptr(debug), _stream(stream)))
+
+    def eval_block_flow_offsets(self, dq, B, flags, out, out_stride, stream=None):
+        """pib_eval_block on float offsets dq from the flow basis centre (R = m + dq)."""
+        self._check(self.lib.pib_eval_block_flow_offsets(self.h, _devptr(dq), int(B), int(flags), _devptr(out),
+                                                         int(out_stride), _stream(stream)))
 
     def sample_block(self, seed, offset, B, q, src=None, stream=None):
         self._check(self.lib.pib_sample_block(self.h, int(seed) & (2**64 - 1), int(offset), int(B),

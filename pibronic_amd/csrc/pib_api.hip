@@ -424,6 +424,33 @@ int pib_eval_block(pib_ctx* c, const double* q, int64_t B, int flags, double* ou
   return PIB_OK;
 }
 
+int pib_eval_block_flow_offsets(pib_ctx* c, const float* dq, int64_t B, int flags, double* out,
+                                int64_t out_stride, void* stream) {
+  if (!c) return PIB_ERR_ARG;
+  if (flags & ~PIB_EVAL_NOSCALE)
+    return fail(c, PIB_ERR_UNSUPPORTED, "eval_block_flow_offsets: flags other than PIB_EVAL_NOSCALE");
+  if (!c->f_P || !c->f_m.p) return fail(c, PIB_ERR_STATE, "eval_block_flow_offsets: flow basis not set");
+  if (c->f_N != c->N || c->f_P != c->P)
+    return fail(c, PIB_ERR_STATE, "eval_block_flow_offsets: flow basis and beads disagree in N or P");
+  EvalArgs a{};
+  bool wide = false;
+  int r = make_eval_args(c, flags, B, &a, &wide);
+  if (r) return r;
+  if (wide || a.AR != c->A || c->A > 8)
+    return fail(c, PIB_ERR_UNSUPPORTED, "eval_block_flow_offsets: the FP64 lanes kernel with A_rho = A <= 8 only");
+  if (out_stride < B) return fail(c, PIB_ERR_ARG, "eval_block_flow_offsets: out_stride < B");
+  if (B == 0) return PIB_OK;
+  if (!dq || !out) return fail(c, PIB_ERR_ARG, "eval_block_flow_offsets: null dq or out");
+  a.qf = dq;
+  a.qcentre = (const double*)c->f_m.p;
+  a.out = out;
+  a.out_stride = out_stride;
+  PIB_HIP(c, hipSetDevice(c->device));
+  hipError_t e = launch_eval_fp64_offsets(c->A, a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(c, e, "eval_block_flow_offsets launch");
+  return PIB_OK;
+}
+
 int pib_set_sampler(pib_ctx* c, int A_rho, int N, int P, const double* w, const double* coth,
                     const double* csch, const double* shift) {
   if (!c) return PIB_ERR_ARG;
@@ -731,7 +758,7 @@ int pib_flow_whiten(pib_ctx* c, int mode, int64_t B, const void* in, float* work
                     void* stream) {
   if (!c) return PIB_ERR_ARG;
   if (!c->f_P) return fail(c, PIB_ERR_STATE, "flow_whiten: basis not set");
-  if (mode < PIB_FLOW_WHITEN || mode > PIB_FLOW_GRAD) return fail(c, PIB_ERR_ARG, "flow_whiten: bad mode");
+  if (mode < PIB_FLOW_WHITEN || mode > PIB_FLOW_UNWHITEN_OFFSETS) return fail(c, PIB_ERR_ARG, "flow_whiten: bad mode");
   if (B < 1 || !in || !work || !out) return fail(c, PIB_ERR_ARG, "flow_whiten: B < 1 or null pointer");
   const int N = c->f_N, P = c->f_P;
   if ((B * N) % 64 != 0 || P % 16 != 0)
@@ -743,8 +770,13 @@ int pib_flow_whiten(pib_ctx* c, int mode, int64_t B, const void* in, float* work
   PIB_HIP(c, hipSetDevice(c->device));
   const float* U = (const float*)c->f_U.p;
   hipError_t e;
+  if (mode == PIB_FLOW_UNWHITEN_OFFSETS && !whiten_fused_supported(P))
+    return fail(c, PIB_ERR_UNSUPPORTED, "flow_whiten: offsets need P = 64 or 128");
   if (whiten_fused_supported(P)) {
-    const int m = mode == PIB_FLOW_WHITEN ? FLOW_WHITEN : mode == PIB_FLOW_UNWHITEN ? FLOW_UNWHITEN : FLOW_GRAD;
+    const int m = mode == PIB_FLOW_WHITEN     ? FLOW_WHITEN
+                  : mode == PIB_FLOW_UNWHITEN ? FLOW_UNWHITEN
+                  : mode == PIB_FLOW_GRAD     ? FLOW_GRAD
+                                              : FLOW_UNWHITEN_OFFSETS;
     if ((e = launch_whiten_fused(m, ba, in, out, U, c->num_cu, s)) != hipSuccess) return hip_fail(c, e, "flow whiten");
     return PIB_OK;
   }

@@ -63,9 +63,14 @@ __device__ __forceinline__ const_as<T>* as_const(const T* p) {
 struct CoordBuf {
   __amdgpu_buffer_rsrc_t rsrc;
   int base;  // this lane's sample: index of its first coordinate in the resource (doubles)
+  typedef double raw_t;
   __device__ __forceinline__ double operator[](int i) const {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, (base + i) * 8, 0, 0));
   }
+  // coordinate of mode j, bead p (P beads per mode), in two steps: the load, issued ahead,
+  // and the value, formed where it is used
+  __device__ __forceinline__ raw_t load(int j, int P, int p) const { return (*this)[j * P + p]; }
+  __device__ __forceinline__ double value(int, raw_t r) const { return r; }
   // the wave's samples [first, first + per_wave) (first made wave-uniform explicitly); `sample`
   // is this lane's sample, or -1 for a lane past the block (which reads the wave's first)
   __device__ __forceinline__ static CoordBuf wave_samples(const double* q, int64_t B, int64_t NP, int64_t first,
@@ -77,6 +82,31 @@ struct CoordBuf {
     const int64_t n = B - s0 < per_wave ? (B - s0 > 0 ? B - s0 : 0) : per_wave;
     return CoordBuf{__builtin_amdgcn_make_buffer_rsrc((void*)(q + s0 * NP), (short)0, (int)(n * NP * 8), kWord3),
                     sample >= 0 ? (int)((sample - s0) * NP) : 0};
+  }
+};
+
+// The same for coordinates given as float offsets from a per-mode centre (FP64, read through
+// the constant address space): x(j, p) = centre[j] + (double)dq, the doubles an unwhitening
+// that adds the centre in FP64 would have written, at half the bytes.
+struct CoordOffsetBuf {
+  __amdgpu_buffer_rsrc_t rsrc;
+  int base;  // this lane's sample: index of its first offset in the resource (floats)
+  const_as<double>* centre;
+  typedef float raw_t;
+  __device__ __forceinline__ raw_t load(int j, int P, int p) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (base + j * P + p) * 4, 0, 0));
+  }
+  __device__ __forceinline__ double value(int j, raw_t r) const { return centre[j] + (double)r; }
+  __device__ __forceinline__ static CoordOffsetBuf wave_samples(const float* q, const double* c, int64_t B,
+                                                                int64_t NP, int64_t first, int per_wave,
+                                                                int64_t sample) {
+    constexpr int kWord3 = 0x00020000;  // as CoordBuf
+    const int64_t s0 = ((int64_t)__builtin_amdgcn_readfirstlane((int)((uint64_t)first >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)first);
+    const int64_t n = B - s0 < per_wave ? (B - s0 > 0 ? B - s0 : 0) : per_wave;
+    return CoordOffsetBuf{
+        __builtin_amdgcn_make_buffer_rsrc((void*)(q + s0 * NP), (short)0, (int)(n * NP * 4), kWord3),
+        sample >= 0 ? (int)((sample - s0) * NP) : 0, as_const(c)};
   }
 };
 

@@ -24,6 +24,8 @@
 // (<= 1e-11 relative at A=12, N=24, P=128; ~1e-13 at the small workloads). V is built from
 // the unshifted coordinates (the reference's shift/unshift leaves an ulp of drift in q,
 // which moves V by ~1e-16). Linear algebra (V, M, ring) uses explicit fma().
+#include <type_traits>
+
 #include "pib_common.hpp"
 #include "pib_kernels.hpp"
 
@@ -255,8 +257,8 @@ __device__ __forceinline__ void expm_lane(double (&X)[A][A], double (&T)[A][A]) 
     expm_lane_s8<A>(X, T);
 }
 
-template <int A, int AR, bool PM, bool QD>
-__device__ __forceinline__ void mode_terms(const_as<double>* tab, const CoordBuf& qs, int N,
+template <int A, int AR, bool PM, bool QD, class Q>
+__device__ __forceinline__ void mode_terms(const_as<double>* tab, const Q& qs, int N,
                                            int P, int p, int pn, double& tr, double& tv, double& tp,
                                            double& tm, double (&er)[AR], double (&ev)[A], double (&ep)[A],
                                            double (&em)[A], double (&v)[A][A]) {
@@ -299,29 +301,36 @@ __device__ __forceinline__ void mode_terms(const_as<double>* tab, const CoordBuf
   // the last mode's coordinates again). The loop body is straight-line code, which keeps the
   // compiler's wait counts exact (with a branch per mode it waited for every load in flight at
   // the top of each iteration).
-  auto ld4 = [&](int j, double (&x)[4][2]) {
+  // (the loader's raw values are carried between iterations; the coordinate is formed at use)
+  typedef typename Q::raw_t raw_t;
+  auto ld4 = [&](int j, raw_t (&x)[4][2]) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int jj = j + k < N ? j + k : N - 1;
-      x[k][0] = qs[jj * P + p];
-      x[k][1] = qs[jj * P + pn];
+      x[k][0] = qs.load(jj, P, p);
+      x[k][1] = qs.load(jj, P, pn);
     }
   };
+  // modes past N read the centre of mode N - 1 (offset loader) with a zero record: nothing added
+  auto use = [&](int j, const raw_t (&x)[2]) {
+    const int jj = j < N ? j : N - 1;
+    mode(j, qs.value(jj, x[0]), qs.value(jj, x[1]));
+  };
   const int N4 = (N + 3) & ~3;
-  double xa[4][2], xb[4][2];
+  raw_t xa[4][2], xb[4][2];
   ld4(0, xa);
   int j = 0;
   for (; j + 8 <= N4; j += 8) {
     ld4(j + 4, xb);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) mode(j + k, xa[k][0], xa[k][1]);
+    for (int k = 0; k < 4; ++k) use(j + k, xa[k]);
     ld4(j + 8, xa);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) mode(j + 4 + k, xb[k][0], xb[k][1]);
+    for (int k = 0; k < 4; ++k) use(j + 4 + k, xb[k]);
   }
   if (j < N4) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) mode(j + k, xa[k][0], xa[k][1]);
+    for (int k = 0; k < 4; ++k) use(j + k, xa[k]);
   }
 }
 
@@ -337,7 +346,8 @@ struct EvalOccupancy {
 };
 
 // The samples of one wave: wave w evaluates samples [w * 64/G, (w + 1) * 64/G) of `args`.
-template <int A, int AR, bool PM, bool DBG>
+// QF: coordinates as float offsets from a per-mode centre (args.qf, args.qcentre)
+template <int A, int AR, bool PM, bool DBG, bool QF = false>
 __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
   const int lane = threadIdx.x & (kWave - 1);
   const int G = args.G, K = args.K, N = args.N, P = args.P;
@@ -346,8 +356,12 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
   const int64_t sample = wave * spw + lane / G;
   const bool live = sample < args.B;
   const int64_t NP = (int64_t)N * P;
-  const double* __restrict__ qs = args.q + (live ? sample : 0) * NP;
-  const CoordBuf qb = CoordBuf::wave_samples(args.q, args.B, NP, wave * spw, spw, live ? sample : -1);
+  typedef typename std::conditional<QF, CoordOffsetBuf, CoordBuf>::type Loader;
+  Loader qb;
+  if constexpr (QF)
+    qb = CoordOffsetBuf::wave_samples(args.qf, args.qcentre, args.B, NP, wave * spw, spw, live ? sample : -1);
+  else
+    qb = CoordBuf::wave_samples(args.q, args.B, NP, wave * spw, spw, live ? sample : -1);
 
   // read-only constants through the constant address space: scalar loads
   struct RO {
@@ -402,7 +416,7 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
     // off-diagonal quadratic mode pairs (j != k)
     for (int t = 0; t < args.npair; ++t) {
       const int jj = pairs[2 * t], kk = pairs[2 * t + 1];
-      const double xj = qs[jj * P + p], xk = qs[kk * P + p];
+      const double xj = qb.value(jj, qb.load(jj, P, p)), xk = qb.value(kk, qb.load(kk, P, p));
       const_as<double>* Qh = qpair + (int64_t)t * A * A;
 #pragma unroll
       for (int b = 0; b < A; ++b)
@@ -585,6 +599,13 @@ k_eval_fp64(const EvalArgs args) {
   eval_wave<A, AR, PM, DBG>(args, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
 }
 
+// the same on float offsets from a per-mode centre (pib_eval_block_flow_offsets)
+template <int A>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A, false>::waves)))
+k_eval_fp64_offsets(const EvalArgs args) {
+  eval_wave<A, A, false, false, true>(args, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+}
+
 LaneProb lane_prob(const EvalArgs& a) {
   LaneProb p{};
   p.q = a.q;
@@ -707,6 +728,30 @@ hipError_t launch_eval_fp64(int A, int AR, int pm, const EvalArgs& a, hipStream_
 #define PIB_CASE(a_, ar_) if (A == a_ && AR == ar_) return launch_a<a_, ar_>(pm, a, s);
   PIB_EVAL_CASES(PIB_CASE)
 #undef PIB_CASE
+  return hipErrorInvalidValue;
+}
+
+template <int A>
+static hipError_t launch_offsets_a(const EvalArgs& a, hipStream_t s) {
+  const int spw = kWave / a.G;
+  const int64_t waves = (a.B + spw - 1) / spw;
+  const int threads = 256;
+  const dim3 grid((unsigned)((waves * kWave + threads - 1) / threads));
+  hipLaunchKernelGGL((k_eval_fp64_offsets<A>), grid, dim3(threads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_eval_fp64_offsets(int A, const EvalArgs& a, hipStream_t s) {
+  switch (A) {
+    case 1: return launch_offsets_a<1>(a, s);
+    case 2: return launch_offsets_a<2>(a, s);
+    case 3: return launch_offsets_a<3>(a, s);
+    case 4: return launch_offsets_a<4>(a, s);
+    case 5: return launch_offsets_a<5>(a, s);
+    case 6: return launch_offsets_a<6>(a, s);
+    case 7: return launch_offsets_a<7>(a, s);
+    case 8: return launch_offsets_a<8>(a, s);
+  }
   return hipErrorInvalidValue;
 }
 

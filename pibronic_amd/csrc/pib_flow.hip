@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(256) k_whiten(BasisArgs b, const void* in, voi
   float* Xs = wsm + P * L::ldu + wave * 16 * L::ldx;  // this wave's 16 input rows
   float* Ys = Xs + 64 * L::ldx;                     // this wave's 16 output rows
   const int N = b.N, NP = N * P, Ph = P / 2, H = NP / 2;
-  constexpr bool inv = MODE == FLOW_UNWHITEN;
+  constexpr bool inv = MODE == FLOW_UNWHITEN || MODE == FLOW_UNWHITEN_OFFSETS;
   for (int e = tid; e < P * P; e += 256) {
     const int r = e / P, c = e - r * P;
     if (inv) Us[c * L::ldu + r] = U[e];  // B[k][c] = U[c][k]
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(256) k_whiten(BasisArgs b, const void* in, voi
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       const int64_t gr = gr0 + i;
-      if (MODE != FLOW_GRAD) r.m[i] = b.m[j];
+      if (MODE == FLOW_WHITEN || MODE == FLOW_UNWHITEN) r.m[i] = b.m[j];
 #pragma unroll
       for (int t = 0; t < NO; ++t) {
         const int o = lane + 64 * t;
@@ -210,7 +210,9 @@ __global__ void __launch_bounds__(256) k_whiten(BasisArgs b, const void* in, voi
 #pragma unroll
       for (int t = 0; t < NO; ++t) {
         const int o = lane + 64 * t;
-        if (inv) {
+        if (MODE == FLOW_UNWHITEN_OFFSETS) {
+          ((float*)out)[gr * P + o] = Ys[i * L::ldx + o];
+        } else if (inv) {
           ((double*)out)[gr * P + o] = (double)Ys[i * L::ldx + o] + cur.m[i];
         } else {
           const int h = o / Ph, kk = o - h * Ph, k = 2 * kk + h;
@@ -245,6 +247,7 @@ static hipError_t launch_whiten_p(int mode, const BasisArgs& b, const void* in, 
                                   int num_cu, hipStream_t s) {
   if (mode == FLOW_WHITEN) return launch_whiten_pm<P, FLOW_WHITEN>(b, in, out, U, num_cu, s);
   if (mode == FLOW_UNWHITEN) return launch_whiten_pm<P, FLOW_UNWHITEN>(b, in, out, U, num_cu, s);
+  if (mode == FLOW_UNWHITEN_OFFSETS) return launch_whiten_pm<P, FLOW_UNWHITEN_OFFSETS>(b, in, out, U, num_cu, s);
   return launch_whiten_pm<P, FLOW_GRAD>(b, in, out, U, num_cu, s);
 }
 

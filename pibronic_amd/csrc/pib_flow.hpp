@@ -60,7 +60,8 @@ struct BasisArgs {
 enum : int {
   FLOW_WHITEN = 0,   // R (double) -> X = R - m (float) ; Y -> zp = perm(Y / sigma)
   FLOW_UNWHITEN = 1, // zp -> Y = unperm(zp) * sigma ; X -> R (double) = X + m
-  FLOW_GRAD = 2      // dR (float) -> X = dR ; Y -> dzp = perm(Y * sigma)
+  FLOW_GRAD = 2,     // dR (float) -> X = dR ; Y -> dzp = perm(Y * sigma)
+  FLOW_UNWHITEN_OFFSETS = 3  // as FLOW_UNWHITEN, X -> dR (float) = X (R = dR + m)
 };
 
 hipError_t launch_flow_pre(int mode, const BasisArgs& b, const void* in, float* out, hipStream_t s);

@@ -41,6 +41,8 @@ enum : int { OUT_RHO = 0, OUT_G = 1, OUT_GP = 2, OUT_GM = 3, OUT_LOGS = 4, NUM_O
 
 struct EvalArgs {
   const double* q;      // (B, N, P) bead coordinates R, sample-major
+  const float* qf;      // or (pib_eval_block_flow_offsets) float offsets: R = qcentre[j] + qf
+  const double* qcentre;  // N
   int64_t B;
   int N, P;
   int G;                // lanes per sample (power of two, divides 64)
@@ -168,6 +170,7 @@ struct BatchTile {
 
 // launchers (return hipError_t of the launch)
 hipError_t launch_eval_fp64(int A, int AR, int pm, const EvalArgs& a, hipStream_t s);
+hipError_t launch_eval_fp64_offsets(int A, const EvalArgs& a, hipStream_t s);
 hipError_t launch_eval_wide(int pm, const EvalArgs& a, hipStream_t s, bool bf16x3 = false);
 hipError_t launch_eval_fp32(int A, int AR, int pm, const EvalArgs& a, hipStream_t s);
 hipError_t launch_eval_batched(int A, int AR, int pm, const LaneProbs& probs, const BatchTile* tiles,

@@ -392,11 +392,21 @@ class Flow:
                           beta1, beta2, eps, self.step_count, stream)
 
     # ------------------------------------------------------------------ proposal draws
-    def draw(self, B, seed, offset, stream=None, layer_events=None):
+    def offsets_ok(self):
+        """Whether draws can be returned as float offsets from the basis centre and evaluated
+        as such (pib_eval_block_flow_offsets: the FP64 lanes kernel, A_rho = A <= 8; the fused
+        unwhitening: P = 64 or 128)."""
+        d = self.data
+        return self.P in (64, 128) and d.states <= 8 and d.rho.states == d.states
+
+    def draw(self, B, seed, offset, stream=None, layer_events=None, offsets=False):
         """B proposal samples: R (B, N, P) double and log q(R) (B,). The base sample is drawn
         directly in the flow's coordinates with its log rho (pib_flow_draw_base).
         layer_events: optional pair of torch.cuda.Event recorded on `stream` around the
         coupling layers (to time them apart from the base draw and the unwhitening).
+        offsets: return float offsets dR (B, N, P) from the basis centre m instead of R
+        (R = m + dR exactly as the double path forms it; BoxData.evaluate_offsets_device takes
+        them), half the bytes written and read.
 
         With draw_precision "bf16" the layers' networks run as single bf16 products. log q is
         still exact for the map applied: a coupling layer passes its conditioning half through
@@ -414,8 +424,12 @@ class Flow:
         self._layers_forward(zp, B, logdet, stream, single_bf16=self.draw_precision == "bf16")
         if layer_events is not None:
             layer_events[1].record(stream)
-        R = torch.empty((B, self.N, self.P), dtype=torch.float64, device=self.dev)
         work = self._buf("work", (2 * B * self.D,))
+        if offsets:
+            dR = torch.empty((B, self.N, self.P), dtype=torch.float32, device=self.dev)
+            self.ctx.flow_whiten(_native.FLOW_UNWHITEN_OFFSETS, B, zp, work, dR, stream)
+            return dR, logrho - logdet
+        R = torch.empty((B, self.N, self.P), dtype=torch.float64, device=self.dev)
         self.ctx.flow_whiten(_native.FLOW_UNWHITEN, B, zp, work, R, stream)
         return R, logrho - logdet
 

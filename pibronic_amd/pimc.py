@@ -454,6 +454,15 @@ class BoxData:
             stream = torch.cuda.current_stream(self.device)
         self.context().eval_block(q, B, flags, out, out_stride, debug, stream)
 
+    def evaluate_offsets_device(self, dq, B, out, out_stride, noscale=False, stream=None):
+        """pib_eval_block_flow_offsets: the FP64 evaluation (no +-dbeta) of coordinates given as
+        float offsets from the flow basis centre (Flow.draw(offsets=True))."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        self.context().eval_block_flow_offsets(dq, B, _native.EVAL_NOSCALE if noscale else 0, out, out_stride,
+                                               stream)
+
 
 class BoxDataPM(BoxData):
     """BoxData with the beta +- delta_beta path (pibronic/pimc/pimc.py:695-738)."""

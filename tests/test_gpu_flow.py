@@ -672,3 +672,35 @@ def test_block_sums_with_logq_matches_host(model):
     assert np.allclose(np.log(np.abs(r)), lw.cpu().numpy(), rtol=0, atol=1e-9)
     with pytest.raises(_native.NativeError):
         data.context().block_sums_q(out, B, B, False, 1.0, None, sums)
+
+
+@pytest.mark.parametrize("which", ["A3_N2_P64", "A8_N16_P64"])
+def test_offset_draws_match_double_draws(tmp_path, model, which):
+    """Flow.draw(offsets=True) writes float offsets dR with m + dR (float64) equal to the double
+    path's R bit for bit, the same log q, and pib_eval_block_flow_offsets on dR gives
+    pib_eval_block's outputs on R bit for bit (the lanes kernel at A = 3 and at A = 8);
+    +-dbeta is refused before any launch."""
+    _announce(f"flow offset draws {which}")
+    if which == "A3_N2_P64":
+        data = model
+    else:
+        FS = synthetic.write_workload(str(tmp_path), "A8_N16_P64_T300")
+        data = make_data(FS, 300.0, 64, 256, 256, seed=4)
+    fl = flow.Flow(data, layers=2, hidden=64, smax=2.0, seed=3)
+    assert fl.offsets_ok()
+    B = 256
+    R, lq1 = fl.draw(B, 5, 1000)
+    dq, lq2 = fl.draw(B, 5, 1000, offsets=True)
+    assert dq.dtype == torch.float32 and dq.shape == R.shape
+    m = torch.as_tensor(np.asarray(data.rho.state_weight) @ np.asarray(data.rho.state_shift), dtype=torch.float64)
+    assert torch.equal(lq1.cpu(), lq2.cpu())
+    assert torch.equal(R.cpu(), m[None, :, None] + dq.cpu().double())
+    out1 = torch.empty((_native.NUM_OUT, B), dtype=torch.float64, device=DEV)
+    out2 = torch.full((_native.NUM_OUT, B), float("nan"), dtype=torch.float64, device=DEV)
+    data.evaluate_device(R, B, False, out1, B)
+    data.evaluate_offsets_device(dq, B, out2, B)
+    torch.cuda.synchronize()
+    rows = [_native.OUT_RHO, _native.OUT_G, _native.OUT_LOGS]
+    assert torch.equal(out1[rows].cpu(), out2[rows].cpu())
+    with pytest.raises(_native.NativeError, match="flags"):
+        data.context().eval_block_flow_offsets(dq, B, _native.EVAL_PM, out2, B)
