@@ -98,7 +98,10 @@ template <int P>
 struct WhitenLds {
   static constexpr int ldu = P + 16;  // basis rows 16 floats apart mod 64 banks
   static constexpr int ldx = P + 4;   // row tiles 4 floats apart
-  static constexpr size_t bytes = sizeof(float) * ((size_t)P * ldu + 2 * 64 * (size_t)ldx);
+  // the basis, the four waves' input rows and (all modes but the offsets) their output rows
+  static constexpr size_t bytes(int mode) {
+    return sizeof(float) * ((size_t)P * ldu + (mode == 3 ? 1 : 2) * 64 * (size_t)ldx);
+  }
 };
 
 template <int P, int MODE>
@@ -188,7 +191,10 @@ __global__ void __launch_bounds__(256) k_whiten(BasisArgs b, const void* in, voi
     if ((rw + rstep) * 16 < rows) load_rows(rw + rstep, nxt);
     // one column tile at a time, each a single in-place accumulator chain written to Ys at
     // once (keeping all P/16 tiles live let the compiler rename accumulators between unrolled
-    // steps with a destination that partly overlaps the C operand, which gave wrong sums)
+    // steps with a destination that partly overlaps the C operand, which gave wrong sums). The
+    // offsets go from the accumulators straight to memory (register r of lane l holds row
+    // 4(l/16) + r, column 16c + l%16: 64-byte row pieces), with no output rows in LDS: 38
+    // instead of 55 KB of LDS per work-group, three work-groups per CU instead of two.
 #pragma unroll 1
     for (int c = 0; c < P / 16; ++c) {
       f32x4w acc = {0.f, 0.f, 0.f, 0.f};
@@ -198,21 +204,26 @@ __global__ void __launch_bounds__(256) k_whiten(BasisArgs b, const void* in, voi
         const float bb = Us[(k0 + (lane >> 4)) * L::ldu + 16 * c + (lane & 15)];
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb, acc, 0, 0, 0);
       }
+      if constexpr (MODE == FLOW_UNWHITEN_OFFSETS) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Ys[(4 * (lane >> 4) + r) * L::ldx + 16 * c + (lane & 15)] = acc[r];
+        for (int r = 0; r < 4; ++r)
+          ((float*)out)[(gr0 + 4 * (lane >> 4) + r) * P + 16 * c + (lane & 15)] = acc[r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Ys[(4 * (lane >> 4) + r) * L::ldx + 16 * c + (lane & 15)] = acc[r];
+      }
     }
     wave_lds_sync();
     int64_t s = s0;
     int j = j0;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
+      if constexpr (MODE == FLOW_UNWHITEN_OFFSETS) break;  // stored from the accumulators
       const int64_t gr = gr0 + i;
 #pragma unroll
       for (int t = 0; t < NO; ++t) {
         const int o = lane + 64 * t;
-        if (MODE == FLOW_UNWHITEN_OFFSETS) {
-          ((float*)out)[gr * P + o] = Ys[i * L::ldx + o];
-        } else if (inv) {
+        if (inv) {
           ((double*)out)[gr * P + o] = (double)Ys[i * L::ldx + o] + cur.m[i];
         } else {
           const int h = o / Ph, kk = o - h * Ph, k = 2 * kk + h;
@@ -233,7 +244,7 @@ bool whiten_fused_supported(int P) { return P == 64 || P == 128; }
 template <int P, int MODE>
 static hipError_t launch_whiten_pm(const BasisArgs& b, const void* in, void* out, const float* U, int num_cu,
                                    hipStream_t s) {
-  const size_t lds = WhitenLds<P>::bytes;
+  const size_t lds = WhitenLds<P>::bytes(MODE);
   static unsigned long long lds_set = 0;  // more than the default 64 KB of dynamic LDS when P = 128
   if (hipError_t e = ensure_dynamic_lds((const void*)k_whiten<P, MODE>, lds, lds_set)) return e;
   const int64_t blocks = (b.B * (int64_t)b.N + 63) / 64;  // 4 waves x 16 rows
