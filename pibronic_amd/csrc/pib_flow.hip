@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(256) k_whiten(BasisArgs b, const void* in, voi
     // steps with a destination that partly overlaps the C operand, which gave wrong sums). The
     // offsets go from the accumulators straight to memory (register r of lane l holds row
     // 4(l/16) + r, column 16c + l%16: 64-byte row pieces), with no output rows in LDS: 38
-    // instead of 55 KB of LDS per work-group, three work-groups per CU instead of two.
+    // instead of 55 KB of LDS per work-group, four work-groups per CU instead of two.
 #pragma unroll 1
     for (int c = 0; c < P / 16; ++c) {
       f32x4w acc = {0.f, 0.f, 0.f, 0.f};
