@@ -36,7 +36,8 @@ namespace pib {
 //
 // The reference forms M = U diag(exp(-tau lambda)) U^T from eigh(V) (pimc.py:1167-1187); M is
 // the only consumer of the decomposition, so it is computed directly: scaling and squaring of
-// the degree-12 Taylor polynomial T_12, evaluated by Paterson-Stockmeyer in X^3:
+// a Taylor polynomial, in one of three evaluation schemes (expm_lane picks one per kernel). The
+// first is the degree-12 Taylor polynomial T_12, evaluated by Paterson-Stockmeyer in X^3:
 //   T_12 = ((B3 X^3 + B2) X^3 + B1) X^3 + B0,  B_n = c_3n I + c_3n+1 X + c_3n+2 X^2 (n < 3),
 //   B3 = c9 I + c10 X + c11 X^2 + c12 X^3,
 // five products (X^2, X^3, three Horner steps) against seven for Horner's rule at degree 8.
@@ -241,17 +242,127 @@ __device__ __forceinline__ void expm_lane_s8(double (&X)[A][A], double (&T)[A][A
   square_times<A>(T, s);
 }
 
-// Which scheme each surface count uses (index A): Paterson-Stockmeyer degree 12 where its two
-// extra live matrices cost no occupancy that the saved products do not buy back, else the
-// three-product degree-8 form. Measured on MI355X (eval ms, Horner-8 against PS-12, same GPU):
-// A=4, N=8, P=64, 65,536 samples 0.209 against 0.197; A=6, N=12, P=128, 1e5 samples 1.70
-// against 2.16 (PS-12 needs 318 VGPRs there: one wave per SIMD instead of two); A=8, N=16,
-// P=64, 65,536 samples 1.51 against 1.57 (both at one wave per SIMD, PS-12 spilling).
-constexpr bool kExpmPS12[9] = {true, true, true, true, true, false, false, false, false};
+// theta_15+ = 0.6764: the largest |X| at which the form below has a relative backward error
+// sum_{k>=16} |h_k| |X|^(k-1) <= 2^-53, h(x) = log(e^-x p(x)) (h_16 = -2.17e-14, h_17 = 1.89e-14)
+constexpr double kLog2Theta15 = -0.5640051;  // log2(0.67642)
 
+// Degree 16 in four products, exact Taylor coefficients through x^15 (x^16 at 0.546 / 16!), in
+// the evaluation form of Sastre (2018) one level up from the degree-8 form:
+//   y02 = X2 (c1 X2 + c2 X),
+//   y12 = (y02 + c3 X2 + c4 X)(y02 + c5 X2) + c6 y02 + c7 X2,
+//   T   = (y12 + c8 X2 + c9 X)(y12 + c10 y02 + c11 X) + c12 y12 + c13 y02 + c14 X2 + X + I.
+// The fourteen coefficients solve the fourteen equations that make the x^2..x^15 terms 1/k! (of
+// the real solutions, one with the smallest coefficients). To keep four matrices live at most, y02
+// is held as G = y02 + c5 X2 (the second factor of y12's product; the later terms in y02 are
+// rewritten in G and X2), and y12's first factor is formed one row at a time. alpha = |X^2|_F^(1/2)
+// as in the degree-8 form; on the A = 8 workload matrices a wave squares 2 times instead of 5-6.
 template <int A>
+__device__ __forceinline__ void expm_lane_s15(double (&X)[A][A], double (&T)[A][A]) {
+  constexpr double c1 = -4.01876161020103546293293287596e-4, c2 = -2.94553144027968298047549126357e-3;
+  constexpr double c3 = 8.70906657683767596765346131523e-3, c4 = -4.01756844067356780152362494573e-1;
+  constexpr double c5 = -3.23076288812231208197540935626e-2, c6 = 2.33731940471157926649827413344e-2;
+  constexpr double c7 = 2.61492797729811710806033238185e-1, c8 = -2.38107037387098722469262858218e-1;
+  constexpr double c9 = -4.1302763659297829110328428344e-2, c10 = -5.79236170707326052179805145605e0;
+  constexpr double c11 = 2.22420917249637356119032703879e0, c12 = 1.04080173523135436461532671084e1;
+  constexpr double c13 = 3.03012340073871230596564458406e0, c14 = -2.12975559049643578453357460382e0;
+  // the same polynomial in G = y02 + c5 X2
+  constexpr double d3 = c3 - c5, e7 = c7 - c6 * c5, f10 = -c10 * c5, g14 = c14 - c13 * c5;
+  double X2[A][A], Gm[A][A];
+  sym_prod_lane<A>(X, X, 0.0, X2);
+  double n2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < A; ++i) {
+    n2 = fma(X2[i][i], X2[i][i], n2);
+#pragma unroll
+    for (int k = i + 1; k < A; ++k) n2 = fma(2.0 * X2[i][k], X2[i][k], n2);
+  }
+  const double sd = 0.25 * log2_coarse(n2) - kLog2Theta15;
+  int s = 0;
+  if (sd > 0.0) s = sd < 64.0 ? (int)ceil(sd) : 64;
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = i; k < A; ++k) {
+      X[i][k] = ldexp(X[i][k], -s);
+      X[k][i] = X[i][k];
+      X2[i][k] = ldexp(X2[i][k], -2 * s);
+      X2[k][i] = X2[i][k];
+    }
+  // T <- c1 X2 + c2 X; G <- X2 T + c5 X2
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = i; k < A; ++k) {
+      T[i][k] = fma(c1, X2[i][k], c2 * X[i][k]);
+      T[k][i] = T[i][k];
+    }
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = i; k < A; ++k) {
+      double acc = c5 * X2[i][k];
+#pragma unroll
+      for (int c = 0; c < A; ++c) acc = fma(X2[i][c], T[c][k], acc);
+      Gm[i][k] = acc;
+      Gm[k][i] = acc;
+    }
+  // T <- y12 = F G + c6 G + e7 X2, F = G + d3 X2 + c4 X taken one row at a time
+#pragma unroll
+  for (int i = 0; i < A; ++i) {
+    double f[A];
+#pragma unroll
+    for (int c = 0; c < A; ++c) f[c] = fma(c4, X[i][c], fma(d3, X2[i][c], Gm[i][c]));
+#pragma unroll
+    for (int k = i; k < A; ++k) {
+      double acc = fma(c6, Gm[i][k], e7 * X2[i][k]);
+#pragma unroll
+      for (int c = 0; c < A; ++c) acc = fma(f[c], Gm[c][k], acc);
+      T[i][k] = acc;
+      T[k][i] = acc;
+    }
+  }
+  // factors in place: X <- y12 + c8 X2 + c9 X, X2 <- y12 + c10 G + c11 X + f10 X2,
+  // T <- c12 y12 + c13 G + g14 X2 + X + I
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = i; k < A; ++k) {
+      const double x = X[i][k], x2 = X2[i][k], g = Gm[i][k], y = T[i][k];
+      double r = fma(c12, y, fma(c13, g, fma(g14, x2, x)));
+      if (i == k) r += 1.0;
+      X[i][k] = fma(c8, x2, fma(c9, x, y));
+      X[k][i] = X[i][k];
+      X2[i][k] = fma(c10, g, fma(c11, x, fma(f10, x2, y)));
+      X2[k][i] = X2[i][k];
+      T[i][k] = r;
+      T[k][i] = r;
+    }
+  // T <- X X2 + T: each output entry reads only its own entry of T
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = i; k < A; ++k) {
+      double acc = T[i][k];
+#pragma unroll
+      for (int c = 0; c < A; ++c) acc = fma(X[i][c], X2[c][k], acc);
+      T[i][k] = acc;
+      T[k][i] = acc;
+    }
+  square_times<A>(T, s);
+}
+
+// Which scheme each kernel uses: the four-product degree-15+ form without +-dbeta (its four
+// live matrices fit beside one ring state), except at A = 6, whose kernel holds two waves per
+// SIMD in 256 VGPRs and spills with four live matrices; else Paterson-Stockmeyer degree 12 for
+// A <= 4 and the three-product degree-8 form above (the +-dbeta kernels carry three ring states).
+// Measured on MI355X (eval ms, degree 15+ against the previous scheme, 1e5 samples, same GPU):
+// A=4 N=8 P=64 0.308 / 0.315 (PS-12), A=5 N=10 P=64 0.472 / 0.508, A=6 N=12 P=128 1.758 / 1.427
+// (spilling), A=7 N=14 P=64 1.173 / 1.260, A=8 N=16 P=64 1.723 / 1.846 (4.05 / 4.43 at 262,144).
+template <int A, bool PM>
 __device__ __forceinline__ void expm_lane(double (&X)[A][A], double (&T)[A][A]) {
-  if constexpr (kExpmPS12[A])
+  if constexpr (!PM && A != 6)
+    expm_lane_s15<A>(X, T);
+  else if constexpr (A <= 4)
     expm_lane_ps12<A>(X, T);
   else
     expm_lane_s8<A>(X, T);
@@ -456,7 +567,7 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
           X[b][c] = -args.tau * (v[b][c] + Ev[b * A + c]);
           X[c][b] = X[b][c];
         }
-      expm_lane<A>(X, M);
+      expm_lane<A, PM>(X, M);
     }
 
     if (DBG) {

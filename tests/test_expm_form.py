@@ -6,11 +6,13 @@ The kernels use a scaled and squared Taylor polynomial T_m: X = -tau V is scaled
 that a bound alpha on its spectral radius satisfies alpha 2^-s <= theta_m, T_m is evaluated
 and squared s times. For a symmetric X every intermediate is a polynomial in X, so the
 2-norm error is the largest scalar error over the eigenvalues. Schemes:
-  * per-lane FP64 kernel, A <= 4: m = 12 by Paterson-Stockmeyer in X^3,
+  * per-lane FP64 kernel without +-dbeta (A != 6): m = 15+ (degree 16, Taylor through x^15) in
+    four products, alpha = |X^2|_F^(1/2), theta_15+ = 0.6764;
+  * per-lane FP64 kernel with +-dbeta, A <= 4: m = 12 by Paterson-Stockmeyer in X^3,
     alpha = min(|X|_F, |X^3|_F^(1/3)), theta_12 = 0.2996;
-  * per-lane FP64 kernel, 5 <= A <= 8: m = 8 in three products (Sastre's form),
-    alpha = |X^2|_F^(1/2), theta_8 = 0.049 (Horner's rule at degree 8 with alpha = |X|_F, the
-    earlier scheme, is kept as a comparison);
+  * per-lane FP64 kernel with +-dbeta for 5 <= A <= 8, and A = 6 without: m = 8 in three
+    products (Sastre's form), alpha = |X^2|_F^(1/2), theta_8 = 0.049 (Horner's rule at degree 8
+    with alpha = |X|_F, the earlier scheme, is kept as a comparison);
   * per-lane FP32 kernel: the same three-product form in float32, theta_8 = 0.619;
   * one-wave FP64 kernel (A <= 16): m = 12 by Paterson-Stockmeyer in X^4,
     alpha = max(|X^3|_F^(1/3), |X^4|_F^(1/4)), theta_12 = 0.2996.
@@ -123,6 +125,38 @@ def expm_s8(X, theta=0.049, dtype=np.float64):
     return T, s
 
 
+S15_COEF = (-4.01876161020103546293293287596e-4, -2.94553144027968298047549126357e-3,
+            8.70906657683767596765346131523e-3, -4.01756844067356780152362494573e-1,
+            -3.23076288812231208197540935626e-2, 2.33731940471157926649827413344e-2,
+            2.61492797729811710806033238185e-1, -2.38107037387098722469262858218e-1,
+            -4.1302763659297829110328428344e-2, -5.79236170707326052179805145605e0,
+            2.22420917249637356119032703879e0, 1.04080173523135436461532671084e1,
+            3.03012340073871230596564458406e0, -2.12975559049643578453357460382e0)
+THETA15 = 2.0 ** -0.5640051
+
+
+def expm_s15(X, theta=THETA15):
+    """Per-lane FP64 scheme without +-dbeta: alpha = |X^2|_F^(1/2), degree 16 in four products
+    (Taylor through x^15): y02 = X2 (c1 X2 + c2 X), y12 = (y02 + c3 X2 + c4 X)(y02 + c5 X2)
+    + c6 y02 + c7 X2, T = (y12 + c8 X2 + c9 X)(y12 + c10 y02 + c11 X) + c12 y12 + c13 y02
+    + c14 X2 + X + I, evaluated as the kernel does with y02 held as G = y02 + c5 X2."""
+    X = np.asarray(X, dtype=np.float64)
+    n = X.shape[-1]
+    X2 = X @ X
+    s = _squarings(np.linalg.norm(X2, axis=(-2, -1)) ** 0.5, theta)
+    f = np.ldexp(1.0, -s)[..., None, None]
+    x1, x2 = X * f, X2 * f ** 2
+    c1, c2, c3, c4, c5, c6, c7, c8, c9, c10, c11, c12, c13, c14 = S15_COEF
+    G = x2 @ (c1 * x2 + c2 * x1) + c5 * x2
+    y12 = (G + (c3 - c5) * x2 + c4 * x1) @ G + c6 * G + (c7 - c6 * c5) * x2
+    F1 = y12 + c8 * x2 + c9 * x1
+    F2 = y12 + c10 * G + c11 * x1 - c10 * c5 * x2
+    T = F1 @ F2 + (c12 * y12 + c13 * G + (c14 - c13 * c5) * x2 + x1 + np.eye(n))
+    for k in range(int(s.max()) if s.size else 0):
+        T = np.where((s > k)[..., None, None], T @ T, T)
+    return T, s
+
+
 def expm_eigh(X):
     lam, U = np.linalg.eigh(np.asarray(X, dtype=np.float64))
     return np.einsum("...ik,...k,...jk->...ij", U, np.exp(lam), U)
@@ -132,6 +166,7 @@ SCHEMES = {  # (function, unit roundoff, degree)
     "lanes_fp64": (expm_ps12_x3, U64, 12),
     "lanes_fp64_h8": (lambda X: expm_horner(X, 8, 0.049, np.float64), U64, 8),
     "lanes_fp64_s8": (expm_s8, U64, 8),
+    "lanes_fp64_s15": (expm_s15, U64, 16),
     "lanes_fp32": (lambda X: expm_s8(X, 0.619, np.float32), U32, 8),
     "wide_fp64": (expm_ps12, U64, 12),
 }
@@ -164,9 +199,9 @@ def _workload_X(A, N, P, T, off, B=24):
 
 
 @pytest.mark.parametrize("shape,schemes", [
-    ((4, 8, 64, 300.0, None), ["lanes_fp64", "lanes_fp32", "wide_fp64"]),
-    ((8, 16, 64, 300.0, None), ["lanes_fp64_s8", "lanes_fp64_h8", "lanes_fp32", "wide_fp64"]),
-    ((12, 24, 128, 250.0, 0.1), ["wide_fp64"]),
+    ((4, 8, 64, 300.0, None), ["lanes_fp64", "lanes_fp64_s15", "lanes_fp32", "wide_fp64"]),
+    ((8, 16, 64, 300.0, None), ["lanes_fp64_s15", "lanes_fp64_s8", "lanes_fp64_h8", "lanes_fp32", "wide_fp64"]),
+    ((12, 24, 128, 250.0, 0.1), ["lanes_fp64_s15", "wide_fp64"]),
 ])
 def test_workload_matrices(shape, schemes):
     X = _workload_X(*shape)
@@ -230,3 +265,71 @@ def test_s8_coefficients_give_the_taylor_polynomial():
         c = list(S8_COEF)
         c[i] *= 1.0 + 1e-6
         assert np.max(np.abs(expand(c) / taylor - 1.0)) > 1e-8, i
+
+
+def _s15_expand(c, one):
+    """Coefficients of the four-product form as a polynomial in x (degree 16), in the
+    arithmetic of `one` (float or mpmath)."""
+    deg = 17
+
+    def mul(a, b):
+        r = [one * 0] * deg
+        for i in range(deg):
+            for j in range(deg - i):
+                r[i + j] += a[i] * b[j]
+        return r
+
+    def lin(*terms):
+        r = [one * 0] * deg
+        for k, p in terms:
+            for i in range(deg):
+                r[i] += k * p[i]
+        return r
+
+    mono = lambda k: [one if i == k else one * 0 for i in range(deg)]  # noqa: E731
+    c1, c2, c3, c4, c5, c6, c7, c8, c9, c10, c11, c12, c13, c14 = c
+    X, X2, I = mono(1), mono(2), mono(0)
+    y02 = mul(X2, lin((c1, X2), (c2, X)))
+    y12 = lin((one, mul(lin((one, y02), (c3, X2), (c4, X)), lin((one, y02), (c5, X2)))), (c6, y02), (c7, X2))
+    return lin((one, mul(lin((one, y12), (c8, X2), (c9, X)), lin((one, y12), (c10, y02), (c11, X)))),
+               (c12, y12), (c13, y02), (c14, X2), (one, X), (one, I))
+
+
+def test_s15_coefficients_give_the_taylor_polynomial():
+    """The four-product form expands to 1 + x + ... + x^15/15! (each coefficient to 1e-14
+    relative from the double coefficients), its x^16 coefficient is 0.5457/16!, and each of its
+    fourteen coefficients matters (a 1e-6 relative change moves some Taylor coefficient by
+    more than 1e-9 relative)."""
+    p = _s15_expand(S15_COEF, 1.0)
+    rel = [abs(p[k] * math.factorial(k) - 1.0) for k in range(16)]
+    assert max(rel) < 1e-14, rel
+    assert abs(p[16] * math.factorial(16) - 0.5457435) < 1e-6
+    for i in range(14):
+        c = list(S15_COEF)
+        c[i] *= 1.0 + 1e-6
+        q = _s15_expand(c, 1.0)
+        assert max(abs(q[k] * math.factorial(k) - 1.0) for k in range(16)) > 1e-9, i
+
+
+def test_s15_theta_is_the_backward_error_bound():
+    """theta_15+ is where the relative backward error series of the form reaches the unit
+    roundoff: h(x) = log(e^-x p(x)) = sum_{k>=16} h_k x^k (the truncation; the rounding of the
+    coefficients to double leaves terms below x^16 of the size of rounding errors), with
+    sum |h_k| theta^(k-1) <= 2^-53 at the kernel's theta and above it 1% further out."""
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 40
+    one = mp.mpf(1)
+    p = _s15_expand([mp.mpf(c) for c in S15_COEF], one)
+    n = 60
+    P = p + [mp.mpf(0)] * (n - len(p))
+    q = [sum(((-1) ** i / mp.factorial(i)) * P[k - i] for i in range(k + 1)) for k in range(n)]
+    q[0] -= 1  # e^-x p(x) - 1: zero through x^15 but for the rounding of the double coefficients
+    for k in range(16):  # that rounding: at most ~u relative at x^2, 1.2e-12 relative at x^15
+        assert abs(q[k]) < 1e-11 / mp.factorial(k) + mp.mpf(10) ** -30
+        q[k] = mp.mpf(0)
+    q2 = [sum(q[i] * q[k - i] for i in range(k + 1)) for k in range(n)]  # from x^32
+    h = [q[k] - q2[k] / 2 for k in range(n)]  # log(1 + q) through x^47 (q^3 starts at x^48)
+    berr = lambda t: sum(abs(h[k]) * t ** (k - 1) for k in range(16, 48))  # noqa: E731
+    u = mp.mpf(2) ** -53
+    assert berr(mp.mpf(THETA15)) <= u
+    assert berr(mp.mpf(THETA15) * 1.01) > u
