@@ -369,7 +369,10 @@ __global__ void __launch_bounds__(kThreads) k_flow_layer(FusedLayerArgs a) {
 // matrix cores, would bound the layer). The blocks stream from L2 into a register ring
 // kRing - 1 blocks ahead of use; the biases sit in LDS and the coupling's z is loaded ahead of
 // its epilogue, because vector loads complete in order: a load issued behind the ring and
-// waited for at once would wait for the whole ring. The activations of all SB samples are
+// waited for at once would wait for the whole ring. z moves in whole 128-byte row segments,
+// transposed through per-wave LDS slots. The log-det partials are added to the result by a
+// no-return atomic (a load at the end would wait for every z store of the wave to complete).
+// The activations of all SB samples are
 // staged in LDS as bf16 rows in natural order -- the conditioning half (at D = 1024 in two
 // parts of 256 inputs, the second behind a barrier pair halfway through the first product),
 // then h1, then h2 over the conditioning half -- and every wave reads them as B operands one block ahead, each
@@ -470,15 +473,39 @@ __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
   };
   uvec4f bcur[NT], bnxt[NT];
   read_b(0, bcur);
-  // z of the transformed half for the coupling: four float4 (features 32q + 8j + 4h, j = 0..3)
-  // of one sample tile, loaded kZAhead blocks before the end of a pass (tile 0) or while the
-  // previous tile is processed
+  // z of the transformed half for the coupling, one tile (32 samples x the 32 features of block
+  // q) at a time: loaded in whole 128-byte row segments (lane L: row 8i + L / 8, 16-byte chunk
+  // (L % 8) ^ (L / 8), i = 0..3) kZAhead blocks before the end of a pass (tile 0) or while the
+  // previous tile is processed, then transposed through a 4 KB LDS slot of the wave (bufB, free
+  // once step 1 is done; row r at 32 r floats, chunk k at position k ^ (r % 8): conflict-free
+  // reads of one chunk by 32 rows) into the accumulator layout. The coupled tile goes back into
+  // its slot and out as whole row segments through a buffer resource (scalar row offsets).
   constexpr int kZAhead = 4;
-  float4 zq[4];
-  auto load_z = [&](int n, int q, float4(&dst)[4]) {
-    const float* zp = a.z + (s0 + 32 * n + l32) * D + a.b_off + 32 * q + 4 * h;
+  float* zslot = (float*)bufB + w * 2048;
+  const int wu = __builtin_amdgcn_readfirstlane(w);  // the wave index in a scalar register
+  const int zr = lane >> 3, zk = (lane & 7) ^ zr;
+  const unsigned zoff = (unsigned)(zr * D + 4 * zk) * 4u;
+  const __amdgpu_buffer_rsrc_t zrsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)(a.z + s0 * D), (short)0, SB * D * 4, 0x00020000);
+  auto zsoff = [&](int n, int q, int i) { return ((32 * n + 8 * i) * D + a.b_off + 32 * q) * 4; };
+  uvec4f zq[4];
+  auto load_z = [&](int n, int q, uvec4f(&dst)[4]) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) dst[j] = *(const float4*)(zp + 8 * j);
+    for (int i = 0; i < 4; ++i) dst[i] = __builtin_amdgcn_raw_buffer_load_b128(zrsrc, (int)zoff, zsoff(n, q, i), 0);
+  };
+  // The slot accesses of one wave are ordered by compiler fences (no barrier separates them): at
+  // D = 512 the scheduler moved the stores' slot reads above the coupled values' write-back.
+#define PIB_FENCE() do { asm volatile("" ::: "memory"); __builtin_amdgcn_sched_barrier(0); } while (0)
+  auto put_z = [&](int t, const uvec4f(&src)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) *(float4*)(zslot + 1024 * t + 256 * i + 4 * lane) = __builtin_bit_cast(float4, src[i]);
+  };
+  auto z_store = [&](int n, int q, int t) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      __builtin_amdgcn_raw_buffer_store_b128(
+          __builtin_bit_cast(uvec4f, *(const float4*)(zslot + 1024 * t + 256 * i + 4 * lane)), zrsrc, (int)zoff,
+          zsoff(n, q, i), 0);
   };
 #pragma clang loop unroll(full)
   for (int b = 0; b < G::NBLK; ++b) {
@@ -492,7 +519,7 @@ __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
     const bool restage = step == 0 && c % (G::XH / 16) == G::XH / 16 - 1 && c != last;
     const bool ahead = b + 1 < G::NBLK && (c != last || step == 2) && !restage;
     if (ahead) read_b(b + 1, bnxt);
-    if (step == 2 && c == last - kZAhead) load_z(0, G::P3 * w + t3 / KB2, zq);
+    if (step == 2 && c == last - kZAhead) load_z(0, G::P3 * wu + t3 / KB2, zq);
     // keep the loads here: the scheduler would otherwise sink them next to their use, leaving
     // one weight block in flight and the LDS latency exposed
     __builtin_amdgcn_sched_barrier(0);
@@ -546,18 +573,20 @@ __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
     } else {
       // coupling of features 32q + 8j + 4h + 0..3 of sample 32n + l32 (the arithmetic of
       // k_flow_layer's single-bf16 epilogue)
-      const int q = G::P3 * w + t3 / KB2;
+      const int q = G::P3 * wu + t3 / KB2;
 #pragma unroll
       for (int n = 0; n < NT; ++n) {
-        float4 zc[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) zc[j] = zq[j];
+        const int t = n & 1;
+        put_z(t, zq);
+        PIB_FENCE();
         if (n + 1 < NT) load_z(n + 1, q, zq);
-        float* zp = a.z + (s0 + 32 * n + l32) * D + a.b_off + 32 * q + 4 * h;
+        float* zs = zslot + 1024 * t + 32 * l32;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int f0 = 32 * q + 8 * j + 4 * h;
-          float zz[4] = {zc[j].x, zc[j].y, zc[j].z, zc[j].w};
+          float* zsj = zs + 4 * ((2 * j + h) ^ (l32 & 7));
+          const float4 zc = *(const float4*)zsj;
+          float zz[4] = {zc.x, zc.y, zc.z, zc.w};
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const float sh = acc[1][n][4 * j + i] + bl3[H + f0 + i];
@@ -568,8 +597,13 @@ __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
             zz[i] = a.inverse ? (zz[i] - sh) * es : fmaf(zz[i], es, sh);
             part[n] += (double)sc;
           }
-          *(float4*)(zp + 8 * j) = float4{zz[0], zz[1], zz[2], zz[3]};
+          *(float4*)zsj = float4{zz[0], zz[1], zz[2], zz[3]};
         }
+        PIB_FENCE();
+        z_store(n, q, t);
+        // the log-det partial materialised here: left to the scheduler, the FP64 additions sank to
+        // the kernel's end and the 64 scales per lane they need were spilled
+        asm volatile("" : "+v"(part[n]));
       }
     }
   }
@@ -585,9 +619,11 @@ __global__ void __launch_bounds__(512) k_flow_layer_ws(FusedLayerArgs a) {
     double t = 0.0;
 #pragma unroll
     for (int v = 0; v < 8; ++v) t += red[v * SB + tid];
-    a.logdet[s0 + tid] += a.inverse ? -t : t;
+    // a no-return atomic add: a load here would wait for every z store of the wave to complete
+    unsafeAtomicAdd(a.logdet + s0 + tid, a.inverse ? -t : t);
   }
 }
+#undef PIB_FENCE
 
 template <int H, int SB>
 hipError_t launch_ws(const FusedLayerArgs& a, hipStream_t s) {

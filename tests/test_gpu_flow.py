@@ -704,3 +704,27 @@ def test_offset_draws_match_double_draws(tmp_path, model, which):
     assert torch.equal(out1[rows].cpu(), out2[rows].cpu())
     with pytest.raises(_native.NativeError, match="flags"):
         data.context().eval_block_flow_offsets(dq, B, _native.EVAL_PM, out2, B)
+
+
+@pytest.mark.parametrize("name,B", [("A8_N16_P64_T300", 262144), ("A4_N8_P64_T300", 131072)])
+def test_fragment_layer_draws_repeat_bit_for_bit(tmp_path, name, B):
+    """Single-bf16 draws through k_flow_layer_ws at full size: the same draw twice gives the
+    same R and log q bit for bit, and every R finite. The layer moves z through LDS by DMA with
+    hand-counted waits; a wait one too short lets a tile read a slot before its DMA has filled
+    it, which shows as a few hundred changed samples per million that differ from run to run."""
+    _announce(f"fragment-layer draws repeat {name} B={B}")
+    FS = synthetic.write_workload(str(tmp_path), name)
+    cfg = synthetic.WORKLOADS[name]
+    data = make_data(FS, cfg["T"], cfg["P"], 256, 256, seed=4)
+    fl = flow.Flow(data, layers=8, hidden=256, seed=3, gemm="bf16x3")
+    g = torch.Generator().manual_seed(11)
+    fl.params += 0.02 * torch.randn(fl.params.shape, generator=g).to(fl.params.device, fl.params.dtype)
+    fl._version += 1
+    fl.draw_precision = "bf16"
+    assert fl.use_frag
+    R1, lq1 = fl.draw(B, 5, 0)
+    R2, lq2 = fl.draw(B, 5, 0)
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(R1).all())
+    same = (R1 == R2).all(dim=2).all(dim=1) & (lq1 == lq2)
+    assert bool(same.all()), f"{int((~same).sum())} of {B} samples differ between two identical draws"
