@@ -15,15 +15,12 @@
 //    k of Y) as the B operand. Every product below chains in registers that way.
 //  * M = exp(-tau V) directly, without eigenvectors: the reference forms
 //    M = U diag(exp(-tau lambda)) U^T from eigh(V) (pimc.py:1167-1187), and M is the only
-//    consumer of the decomposition. For X = -tau V, alpha = max(|X^3|_F^(1/3), |X^4|_F^(1/4))
-//    bounds the spectral radius (Al-Mohy and Higham, SIAM J. Matrix Anal. Appl. 31 (2009) 970);
-//    the powers are scaled by 2^-s so that alpha 2^-s <= theta_12 = 0.2996, where the degree-12
-//    Taylor polynomial T_12 has a backward error below the FP64 unit roundoff (Sastre et al.,
-//    SIAM J. Sci. Comput. 37 (2015) A439); T_12 is evaluated by Paterson-Stockmeyer in the
-//    fourth power (five products in all) and squared s times. The four beads of a chunk go
-//    through it in lockstep (one s, the largest they need), so four independent product
-//    chains fill the matrix-core pipeline. V is symmetric, so every power and polynomial of
-//    it is, and the accumulator layout of each product feeds the next one.
+//    consumer of the decomposition. X = -tau V is scaled by 2^-s, a degree-16 polynomial equal
+//    to the Taylor polynomial through X^15 is evaluated in four products (expm_sym) and
+//    squared s times. The four beads of a chunk go through it in lockstep (one s, the largest
+//    they need), so four independent product chains fill the matrix-core pipeline. V is
+//    symmetric, so every power and polynomial of it is, and the accumulator layout of each
+//    product feeds the next one.
 //  * The ring product runs on the same FP64 matrix cores, carried transposed,
 //    R^T <- diag(O_p / S_p) M_p R^T, whose trace equals the reference's tr(prod_p M_p O_p)
 //    because M_p is symmetric and O_p diagonal.
@@ -113,47 +110,67 @@ __device__ __forceinline__ d4 sym_mul(const d4& X, const d4& Y) {
   return sym_mul<NR>(X, Y, d4{0.0, 0.0, 0.0, 0.0});
 }
 
-constexpr double kLog2Theta12 = -1.738798;  // log2(0.2996158)
+constexpr double kLog2Theta15 = -0.5640051;  // log2(0.67642)
 
 // exp(X) for a symmetric X in the FP64 accumulator layout, zero outside its leading n x n
-// block, n <= 4 NR (scaling and squaring of the degree-12 Taylor polynomial; see the file
-// header). The result is zero outside the leading block.
+// block, n <= 4 NR: scaling and squaring of the degree-16 polynomial that is the Taylor
+// polynomial through X^15, in four products (the per-lane kernel's form, after Sastre 2018):
+//   y02 = X2 (c1 X2 + c2 X),  y12 = (y02 + c3 X2 + c4 X)(y02 + c5 X2) + c6 y02 + c7 X2,
+//   T = (y12 + c8 X2 + c9 X)(y12 + c10 y02 + c11 X) + c12 y12 + c13 y02 + c14 X2 + X + I,
+// with y02 held as G = y02 + c5 X2. alpha = |X^2|_F^(1/2) >= the spectral radius of a symmetric
+// X; alpha 2^-s <= theta_15+ = 0.6764, where the form's relative backward error is below the
+// FP64 unit roundoff. On the A = 12 workload matrices a chunk of four beads squares 2.1 times
+// instead of 3.1 with degree-12 Paterson-Stockmeyer in X^4: 6.1 products per bead instead of
+// 8.1. Every factor is a polynomial in X, hence symmetric, as sym_mul needs of its first
+// operand; only the identity term touches the diagonal, and only in the leading block, so the
+// result is zero outside it.
 template <int NR>
 __device__ __forceinline__ d4 expm_sym(const d4& X, int n, int g, int j) {
+  constexpr double c1 = -4.01876161020103546293293287596e-4, c2 = -2.94553144027968298047549126357e-3;
+  constexpr double c3 = 8.70906657683767596765346131523e-3, c4 = -4.01756844067356780152362494573e-1;
+  constexpr double c5 = -3.23076288812231208197540935626e-2, c6 = 2.33731940471157926649827413344e-2;
+  constexpr double c7 = 2.61492797729811710806033238185e-1, c8 = -2.38107037387098722469262858218e-1;
+  constexpr double c9 = -4.1302763659297829110328428344e-2, c10 = -5.79236170707326052179805145605e0;
+  constexpr double c11 = 2.22420917249637356119032703879e0, c12 = 1.04080173523135436461532671084e1;
+  constexpr double c13 = 3.03012340073871230596564458406e0, c14 = -2.12975559049643578453357460382e0;
+  constexpr double d3 = c3 - c5, e7 = c7 - c6 * c5, f10 = -c10 * c5, g14 = c14 - c13 * c5;
   const bool dl = ((j & 3) == g) && (j < n);  // register j / 4 of this lane holds X[j][j]
   const int dr = j >> 2;
   const d4 X2 = sym_mul<NR>(X, X);
-  const d4 X3 = sym_mul<NR>(X2, X);
-  d4 X4 = sym_mul<NR>(X2, X2);
-  double n3 = 0.0, n4 = 0.0;
+  double n2 = 0.0;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    n3 = fma(X3[r], X3[r], n3);
-    n4 = fma(X4[r], X4[r], n4);
-  }
-  n3 = wave_sum_dpp(n3);
-  n4 = wave_sum_dpp(n4);
-  // smallest s >= 0 with |X^3|^(1/3) 2^-s and |X^4|^(1/4) 2^-s <= theta_12 (log2 |X^k|_F =
-  // log2(n_k) / 2); a zero matrix gives -inf, a NaN s = 0, the count is capped at 64
-  const double sd = fmax(log2_coarse(n3) * (1.0 / 6.0), log2_coarse(n4) * 0.125) - kLog2Theta12;
+  for (int r = 0; r < 4; ++r) n2 = fma(X2[r], X2[r], n2);
+  n2 = wave_sum_dpp(n2);
+  // smallest s >= 0 with |X^2|_F^(1/2) 2^-s <= theta_15+; a zero matrix gives -inf, a NaN s = 0,
+  // the count is capped at 64
+  const double sd = 0.25 * log2_coarse(n2) - kLog2Theta15;
   int s = 0;
   if (sd > 0.0) s = sd < 64.0 ? (int)ceil(sd) : 64;
   s = __builtin_amdgcn_readfirstlane(s);  // identical on every lane
-  constexpr double c3 = 1.0 / 6, c4 = 1.0 / 24, c5 = 1.0 / 120, c6 = 1.0 / 720, c7 = 1.0 / 5040,
-                   c8 = 1.0 / 40320, c9 = 1.0 / 362880, c10 = 1.0 / 3628800,
-                   c11 = 1.0 / 39916800, c12 = 1.0 / 479001600;
-  d4 B0, B1, B2;
+  d4 x1, x2, H;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const double x1 = ldexp(X[r], -s), x2 = ldexp(X2[r], -2 * s), x3 = ldexp(X3[r], -3 * s);
-    X4[r] = ldexp(X4[r], -4 * s);
-    const double I = (dl && r == dr) ? 1.0 : 0.0;
-    // T_12 = B0 + X^4 (B1 + X^4 B2), B_i cubic (B2 also takes the X^4 / 12! term)
-    B2[r] = fma(c12, X4[r], fma(c11, x3, fma(c10, x2, fma(c9, x1, c8 * I))));
-    B1[r] = fma(c7, x3, fma(c6, x2, fma(c5, x1, c4 * I)));
-    B0[r] = fma(c3, x3, fma(0.5, x2, x1 + I));
+    x1[r] = ldexp(X[r], -s);
+    x2[r] = ldexp(X2[r], -2 * s);
+    H[r] = fma(c1, x2[r], c2 * x1[r]);
   }
-  d4 T = sym_mul<NR>(X4, sym_mul<NR>(X4, B2, B1), B0);
+  const d4 G = sym_mul<NR>(x2, H, c5 * x2);  // y02 + c5 X2
+  d4 F, acc;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    F[r] = fma(c4, x1[r], fma(d3, x2[r], G[r]));  // y02 + c3 X2 + c4 X
+    acc[r] = fma(c6, G[r], e7 * x2[r]);
+  }
+  const d4 Y = sym_mul<NR>(F, G, acc);  // y12
+  d4 F1, F2, R;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const double I = (dl && r == dr) ? 1.0 : 0.0;
+    F1[r] = fma(c8, x2[r], fma(c9, x1[r], Y[r]));
+    F2[r] = fma(c10, G[r], fma(c11, x1[r], fma(f10, x2[r], Y[r])));
+    R[r] = fma(c12, Y[r], fma(c13, G[r], fma(g14, x2[r], x1[r]))) + I;
+  }
+  d4 T = sym_mul<NR>(F1, F2, R);
   for (int k = 0; k < s; ++k) T = sym_mul<NR>(T, T);
   return T;
 }
