@@ -115,6 +115,25 @@ def test_surface_counts(tmp_path, A):
     _parity(FS, 300.0, 9, 200, True, seed=14)
 
 
+@pytest.mark.parametrize("A", [1, 3, 5, 6, 7, 8])
+def test_surface_counts_plain(tmp_path, A):
+    """The plain kernels (no +-dbeta) at every compiled surface count: the four-product degree-15+
+    exponential (the degree-8 form at A = 6), odd bead count, full (j != k) quadratic couplings."""
+    _announce(f"surfaces A={A} N=3 P=9 full Q, plain")
+    FS = stage_dict_models(tmp_path, random_model(A, 3, seed=A, quad="full"))
+    _parity(FS, 300.0, 9, 200, False, seed=14)
+
+
+@pytest.mark.parametrize("A,pm", [(3, False), (5, False), (8, False), (5, True)])
+def test_parity_many_squarings(tmp_path, A, pm):
+    """A cold, coarse ring (T = 10 K, P = 4, tau = 290 / eV) so that the exponentials square four
+    to five times in the plain kernels' degree-15+ form (seven to eight in the +-dbeta kernels'
+    degree-8 form), with g up to 1e124."""
+    _announce(f"many squarings A={A} pm={pm}")
+    FS = stage_dict_models(tmp_path, random_model(A, 3, seed=20 + A, quad="full"))
+    _parity(FS, 10.0, 4, 200, pm, seed=16)
+
+
 @pytest.mark.parametrize("A,AR", [(2, 1), (4, 2), (2, 4), (4, 8)])
 def test_sampling_model_with_other_surface_count(tmp_path, A, AR):
     """A_rho != A: the sampling model's O-matrices use a separate coordinate copy
