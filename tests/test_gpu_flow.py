@@ -709,9 +709,9 @@ def test_offset_draws_match_double_draws(tmp_path, model, which):
 @pytest.mark.parametrize("name,B", [("A8_N16_P64_T300", 262144), ("A4_N8_P64_T300", 131072)])
 def test_fragment_layer_draws_repeat_bit_for_bit(tmp_path, name, B):
     """Single-bf16 draws through k_flow_layer_ws at full size: the same draw twice gives the
-    same R and log q bit for bit, and every R finite. The layer moves z through LDS by DMA with
-    hand-counted waits; a wait one too short lets a tile read a slot before its DMA has filled
-    it, which shows as a few hundred changed samples per million that differ from run to run."""
+    same R and log q bit for bit, and every R finite. The layer moves z through per-wave LDS
+    slots with no barrier between its phases; a read of a slot ahead of the write that fills it
+    shows as a few changed samples per million that differ from run to run."""
     _announce(f"fragment-layer draws repeat {name} B={B}")
     FS = synthetic.write_workload(str(tmp_path), name)
     cfg = synthetic.WORKLOADS[name]
