@@ -337,6 +337,16 @@ def learned_leg(data, args, log):
         "frac": tf / BF16_PEAK_TFLOPS, "matrix_core_frac": (1 if single else 3) * tf / BF16_PEAK_TFLOPS,
         "layers_ms": main["layers_ms"], "flops_per_draw": mlp_flops,
         "note": "HIP events around the coupling layers of each timed draw"}
+    if single:
+        # a one-launch layer reads both halves of z (B x D float) and writes the transformed half,
+        # adds to the log-det (read and write of B doubles) and reads its bf16 weights once
+        layer_bytes = B * (3 * H * 4 + 16) + 2 * (H * hdn + hdn * hdn + hdn * D)
+        hbm = fl.layers * layer_bytes / (main["layers_ms"] * 1e-3)
+        main["roofline_layers"].update({"hbm_frac": hbm / HBM_PEAK_BPS, "bytes_per_draw": fl.layers * layer_bytes})
+        if layer_bytes / HBM_PEAK_BPS > mlp_flops / fl.layers / (BF16_PEAK_TFLOPS * 1e12):
+            # HBM is the larger floor: name it as the bound (the matrix-core fraction stays beside it)
+            main["roofline_layers"].update({"bound": "hbm", "achieved": hbm / 1e9, "peak": HBM_PEAK_BPS / 1e9,
+                                            "unit": "GB/s", "frac": hbm / HBM_PEAK_BPS})
     variants = []
     if args.variants:
         Kv = max(1, min(K, args.variant_steps))
