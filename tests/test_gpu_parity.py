@@ -107,6 +107,28 @@ def test_lane_split_invariance(tmp_path, lanes):
     _parity(FS, 300.0, 37, 300, True, seed=13, lanes=lanes)
 
 
+@pytest.mark.parametrize("pm", [False, True])
+def test_empty_and_single_sample_blocks(tmp_path, pm):
+    """Edge sizes: a block of no samples is a no-op (the evaluation and the sampler return 0 and
+    write nothing, with null coordinate pointers too); block sums over no samples are refused
+    (B < 1); a block of one sample matches the oracle under several lanes-per-sample splits."""
+    import torch
+    _announce(f"empty and single-sample blocks pm={pm}")
+    FS = stage_dict_models(tmp_path, synthetic.generate(4, 8, quadratic=True, seed=0))
+    data = make_data(FS, 300.0, 64, 4, 4, pm=pm)
+    ctx = data.context()
+    out = torch.full((_native.NUM_OUT, 4), 7.0, dtype=torch.float64, device="cuda")
+    ctx.eval_block(0, 0, _native.EVAL_PM if pm else 0, out, 0)
+    ctx.sample_block(1, 0, 0, 0)
+    torch.cuda.synchronize()
+    assert bool((out == 7.0).all())
+    sums = torch.zeros(_native.NUM_SUMS, dtype=torch.float64, device="cuda")
+    with pytest.raises(_native.NativeError, match="B < 1"):
+        ctx.block_sums(out, 4, 0, pm, 1e-4, sums)
+    for lanes in (1, 8, 64):
+        _parity(FS, 300.0, 64, 1, pm, seed=21, lanes=lanes)
+
+
 @pytest.mark.parametrize("A", [1, 2, 3, 5, 6, 7, 8])
 def test_surface_counts(tmp_path, A):
     """Every compiled surface count, odd bead count, full (j != k) quadratic couplings."""
