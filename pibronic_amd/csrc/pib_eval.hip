@@ -457,9 +457,16 @@ struct EvalOccupancy {
 };
 
 // The samples of one wave: wave w evaluates samples [w * 64/G, (w + 1) * 64/G) of `args`.
-// QF: coordinates as float offsets from a per-mode centre (args.qf, args.qcentre)
-template <int A, int AR, bool PM, bool DBG, bool QF = false>
+// QF: coordinates as float offsets from a per-mode centre (args.qf, args.qcentre).
+// CH (with PM): 0 accumulates the three ring chains of g, g+ and g- together; 1, 2, 3 only the
+// chain of g (writing rho, g and log S), of g+ or of g- (writing that output alone). Every pass
+// forms the same M and scaling as the three-chain form, so three passes give its outputs bit
+// for bit, with a third of the accumulators live (at A = 8 the three chains do not fit in the
+// registers of one lane).
+template <int A, int AR, bool PM, bool DBG, bool QF = false, int CH = 0>
 __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
+  static_assert(CH == 0 || (PM && !DBG && !QF), "single-chain passes are +-dbeta passes without debug records");
+  constexpr bool ALL = PM && CH == 0;  // the g+ and g- chains beside g's
   const int lane = threadIdx.x & (kWave - 1);
   const int G = args.G, K = args.K, N = args.N, P = args.P;
   const int gl = lane & (G - 1);
@@ -494,7 +501,7 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
 #pragma unroll
     for (int k = 0; k < A; ++k) {
       Ng[i][k] = (i == k) ? 1.0 : 0.0;
-      if (PM) { Np[i][k] = Ng[i][k]; Nm[i][k] = Ng[i][k]; }
+      if (ALL) { Np[i][k] = Ng[i][k]; Nm[i][k] = Ng[i][k]; }
     }
   // log-domain products: lr[a] = sum_p log(O_rho,a(p) / S_p), ls = sum_p log S_p
   double lr[AR];
@@ -549,10 +556,8 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
     for (int a = 0; a < A; ++a) {
       lOv[a] = fma(-0.5, (tv + ev[a]) + sv.k2[a], sv.lpref[a]);
       lS = fmax(lS, lOv[a]);
-      if (PM) {
-        lOp[a] = fma(-0.5, (tp + ep[a]) + sp.k2[a], sp.lpref[a]);
-        lOm[a] = fma(-0.5, (tm + em[a]) + sm.k2[a], sm.lpref[a]);
-      }
+      if (ALL || CH == 2) lOp[a] = fma(-0.5, (tp + ep[a]) + sp.k2[a], sp.lpref[a]);
+      if (ALL || CH == 3) lOm[a] = fma(-0.5, (tm + em[a]) + sm.k2[a], sm.lpref[a]);
     }
     if (args.noscale) lS = 0.0;
 
@@ -590,8 +595,9 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
     for (int a = 0; a < AR; ++a) lr[a] += lOr[a] - lS;
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-      Ov[a] = exp(lOv[a] - lS);
-      if (PM) {
+      // in a single-chain pass Ov holds the factors of that pass's chain
+      Ov[a] = exp((CH == 2 ? lOp[a] : (CH == 3 ? lOm[a] : lOv[a])) - lS);
+      if (ALL) {
         Op[a] = exp(lOp[a] - lS);
         Om[a] = exp(lOm[a] - lS);
       }
@@ -603,7 +609,7 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
 #pragma unroll
       for (int c = 0; c < A; ++c) {
         rg[c] = Ng[i][c];
-        if (PM) { rpl[c] = Np[i][c]; rmi[c] = Nm[i][c]; }
+        if (ALL) { rpl[c] = Np[i][c]; rmi[c] = Nm[i][c]; }
       }
 #pragma unroll
       for (int k = 0; k < A; ++k) {
@@ -611,7 +617,7 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
 #pragma unroll
         for (int c = 1; c < A; ++c) s = fma(rg[c], M[c][k], s);
         Ng[i][k] = s * Ov[k];
-        if (PM) {
+        if (ALL) {
           double sp_ = rpl[0] * M[0][k], sm_ = rmi[0] * M[0][k];
 #pragma unroll
           for (int c = 1; c < A; ++c) {
@@ -649,7 +655,7 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
         }
       }
     }
-    if (PM) {
+    if (ALL) {
 #pragma unroll
       for (int w = 0; w < 2; ++w) {
         double (&X)[A][A] = (w == 0) ? Np : Nm;
@@ -683,7 +689,13 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
     if (leader) ls += tl;
   }
 
-  if (live && gl == 0) {
+  if (live && gl == 0 && CH >= 2) {  // the g+ or g- pass: the trace of its chain alone
+    double t = Ng[0][0];
+#pragma unroll
+    for (int i = 1; i < A; ++i) t += Ng[i][i];
+    args.out[(CH == 2 ? OUT_GP : OUT_GM) * args.out_stride + sample] = t;
+  }
+  if (live && gl == 0 && CH < 2) {
     const int64_t st = args.out_stride;
     double rho = exp(lr[0]);
 #pragma unroll
@@ -693,7 +705,7 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
     for (int i = 1; i < A; ++i) g += Ng[i][i];
     args.out[OUT_RHO * st + sample] = rho;
     args.out[OUT_G * st + sample] = g;
-    if (PM) {
+    if (ALL) {
       double gp = Np[0][0], gm = Nm[0][0];
 #pragma unroll
       for (int i = 1; i < A; ++i) { gp += Np[i][i]; gm += Nm[i][i]; }
@@ -704,10 +716,10 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
   }
 }
 
-template <int A, int AR, bool PM, bool DBG>
+template <int A, int AR, bool PM, bool DBG, int CH = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A, PM>::waves)))
 k_eval_fp64(const EvalArgs args) {
-  eval_wave<A, AR, PM, DBG>(args, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  eval_wave<A, AR, PM, DBG, false, CH>(args, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
 }
 
 // the same on float offsets from a per-mode centre (pib_eval_block_flow_offsets)
@@ -802,6 +814,18 @@ static hipError_t launch_a(int pm, const EvalArgs& a, hipStream_t s) {
   // while this kernel is bound by dependent FP64 arithmetic, not by memory.
   const EvalArgs& b = a;
   const size_t lds = 0;
+  if constexpr (A == 8 && AR == 8) {
+    // +-dbeta at A = 8: the three ring chains in three passes of one chain each (the three
+    // together spill 1.5 KB per lane to scratch: 5.78 ms against 3.55 ms for the passes at
+    // 65,536 samples, N = 16, P = 64; at A = 7 one pass is still faster, 2.00 ms against about
+    // three times the plain kernel's 0.77); the debug records keep the one-pass form
+    if (pm && !dbg) {
+      hipLaunchKernelGGL((k_eval_fp64<A, AR, true, false, 1>), grid, dim3(threads), lds, s, b);
+      hipLaunchKernelGGL((k_eval_fp64<A, AR, true, false, 2>), grid, dim3(threads), lds, s, b);
+      hipLaunchKernelGGL((k_eval_fp64<A, AR, true, false, 3>), grid, dim3(threads), lds, s, b);
+      return hipGetLastError();
+    }
+  }
   if (pm && dbg)
     hipLaunchKernelGGL((k_eval_fp64<A, AR, true, true>), grid, dim3(threads), lds, s, b);
   else if (pm)

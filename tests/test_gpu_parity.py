@@ -129,6 +129,24 @@ def test_empty_and_single_sample_blocks(tmp_path, pm):
         _parity(FS, 300.0, 64, 1, pm, seed=21, lanes=lanes)
 
 
+@pytest.mark.parametrize("quad", ["diag", "full"])
+def test_pm_passes_match_one_pass_bit_for_bit(tmp_path, quad):
+    """+-dbeta at A = 8 runs as three single-chain passes (g, g+, g-); the debug path keeps the
+    kernel that carries the three chains together. Both form the same M and scaling, so every
+    output agrees bit for bit, and the passes match the oracle."""
+    _announce(f"A=8 +-dbeta passes vs one pass, {quad} Q")
+    FS = stage_dict_models(tmp_path, random_model(8, 4, seed=31, quad=quad))
+    data = make_data(FS, 300.0, 16, 300, 300, pm=True)
+    q, _ = data.draw_host_sample(np.random.default_rng(32), 300)
+    r3 = pimc.evaluate_samples(data, q, pm=True)
+    r1 = pimc.evaluate_samples(data, q, pm=True, debug=True)
+    for k in ("s_rho", "s_g", "s_gP", "s_gM", "log_scale"):
+        assert np.array_equal(r3[k], r1[k]), k
+    o = po.evaluate(oracle_setup(FS, 300.0, 16), q, pm=True)
+    for k in ("s_rho", "s_g", "s_gP", "s_gM"):
+        assert rel_err(r3[k], o[k]) < RTOL, k
+
+
 @pytest.mark.parametrize("A", [1, 2, 3, 5, 6, 7, 8])
 def test_surface_counts(tmp_path, A):
     """Every compiled surface count, odd bead count, full (j != k) quadratic couplings."""
