@@ -421,6 +421,24 @@ def n_quad_pairs(data):
     return int(np.count_nonzero(np.abs(data.vib.quadratic).sum(axis=(2, 3))))
 
 
+def pm_leg(FS, cfg, B, steps, W, seed, log, label):
+    """The same harmonic blocks with g at beta and beta +- dbeta (PIB_EVAL_PM, the evaluation of
+    block_compute_pm, the reference's job path), timed like the plain leg."""
+    from pibronic_amd import _native
+    A, N, P = cfg["A"], cfg["N"], cfg["P"]
+    data = setup_data(FS, cfg, B, steps + W, seed, pm=True)
+    h = harmonic_leg(data, B, steps, W, seed, _native.EVAL_PM, True)
+    sbs = h["samples"] * P / h["seconds"]
+    kname = ("k_eval_fp64_chain<8, 8, 1..3> (three single-chain passes)" if A == 8
+             else f"k_eval_fp64<{A}, {A}, true, false>" if A <= 8 else "k_eval_wide (+-dbeta)")
+    log(f"[bench] FP64 +-dbeta {label}: {sbs:.3e} samples*beads/s, eval {h['eval_kernel_ms']:.3f} ms")
+    return {"samples_beads_per_s": sbs, "ms_per_step": h["ms_per_step"], "steps": h["steps"], "block": B,
+            "eval_kernel_ms": h["eval_kernel_ms"],
+            "roofline": eval_roofline(A, N, P, n_quad_pairs(data), True, B, h["eval_kernel_ms"], kname),
+            "note": "harmonic blocks with g at beta and beta +- dbeta (block_compute_pm's evaluation); "
+                    "algorithmic flops count M once and three ring products"}
+
+
 def fp64_line(FS, name, cfg, args, cpu, log):
     """The plain FP64 kernel on one workload: harmonic blocks of 1e5, samples*beads/s and the
     evaluation kernel's roofline (traffic from the committed counter profile when there is one)."""
@@ -451,7 +469,8 @@ def fp64_line(FS, name, cfg, args, cpu, log):
             "ess_fraction": h["ess_fraction"],
             "roofline": eval_roofline(A, N, P, n_quad_pairs(data), False, B, h["eval_kernel_ms"], kname,
                                       traffic, tdetail),
-            "cpu_baseline": cpu_line(cpu, sbs, f"the {name} model") if cpu else None}
+            "cpu_baseline": cpu_line(cpu, sbs, f"the {name} model") if cpu else None,
+            "pm": pm_leg(FS, cfg, B, args.fp64_steps, args.warmup, args.seed + 3, log, name)}
     # the reference-API path: block_compute (device sampling, evaluation, sums, and the copy of
     # every block's per-sample results to the host arrays of BoxResult)
     api = setup_data(FS, cfg, B, args.fp64_steps, args.seed + 1)
@@ -653,6 +672,7 @@ def main():
     log(f"[bench] harmonic {name}: {nblocks} blocks, ess fraction {harm['ess_fraction']:.3e}")
     nq = n_quad_pairs(data)
     kname = f"k_eval_fp64<{A}, {A}, false, false>" if A <= 8 else "k_eval_wide"
+    harm_pm = pm_leg(FS, cfg, args.block, max(1, min(nblocks, args.fp64_steps)), W, args.seed + 4, log, name)
 
     learned = fl = None
     if not args.no_learned and P % 64 == 0:
@@ -708,6 +728,7 @@ def main():
         "tail_diagnostics": tails,
         "samples_beads_per_s": harm_sbs,
         "samples_beads_per_s_note": f"harmonic FP64 step on {name} (blocks of {args.block})",
+        "harmonic_pm": harm_pm,
         "roofline": roof,
         "cpu_baseline": cpu_line(cpu, harm_sbs, f"the {name} model") if cpu else None,
         "fp64_kernel": fp64,

@@ -716,10 +716,17 @@ __device__ __forceinline__ void eval_wave(const EvalArgs& args, int64_t wave) {
   }
 }
 
-template <int A, int AR, bool PM, bool DBG, int CH = 0>
+template <int A, int AR, bool PM, bool DBG>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A, PM>::waves)))
 k_eval_fp64(const EvalArgs args) {
-  eval_wave<A, AR, PM, DBG, false, CH>(args, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  eval_wave<A, AR, PM, DBG>(args, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+}
+
+// one ring chain of the +-dbeta evaluation (CH 1: g with rho and log S, 2: g+, 3: g-)
+template <int A, int AR, int CH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EvalOccupancy<A, true>::waves)))
+k_eval_fp64_chain(const EvalArgs args) {
+  eval_wave<A, AR, true, false, false, CH>(args, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
 }
 
 // the same on float offsets from a per-mode centre (pib_eval_block_flow_offsets)
@@ -820,9 +827,9 @@ static hipError_t launch_a(int pm, const EvalArgs& a, hipStream_t s) {
     // 65,536 samples, N = 16, P = 64; at A = 7 one pass is still faster, 2.00 ms against about
     // three times the plain kernel's 0.77); the debug records keep the one-pass form
     if (pm && !dbg) {
-      hipLaunchKernelGGL((k_eval_fp64<A, AR, true, false, 1>), grid, dim3(threads), lds, s, b);
-      hipLaunchKernelGGL((k_eval_fp64<A, AR, true, false, 2>), grid, dim3(threads), lds, s, b);
-      hipLaunchKernelGGL((k_eval_fp64<A, AR, true, false, 3>), grid, dim3(threads), lds, s, b);
+      hipLaunchKernelGGL((k_eval_fp64_chain<A, AR, 1>), grid, dim3(threads), lds, s, b);
+      hipLaunchKernelGGL((k_eval_fp64_chain<A, AR, 2>), grid, dim3(threads), lds, s, b);
+      hipLaunchKernelGGL((k_eval_fp64_chain<A, AR, 3>), grid, dim3(threads), lds, s, b);
       return hipGetLastError();
     }
   }
