@@ -824,8 +824,10 @@ static hipError_t launch_a(int pm, const EvalArgs& a, hipStream_t s) {
   if constexpr (A == 8 && AR == 8) {
     // +-dbeta at A = 8: the three ring chains in three passes of one chain each (the three
     // together spill 1.5 KB per lane to scratch: 5.78 ms against 3.55 ms for the passes at
-    // 65,536 samples, N = 16, P = 64; at A = 7 one pass is still faster, 2.00 ms against about
-    // three times the plain kernel's 0.77); the debug records keep the one-pass form
+    // 65,536 samples, N = 16, P = 64, timed alone; in a block loop after evaluations without
+    // +-dbeta both forms have been measured at ~1.6x their cycles, the passes never slower; at
+    // A = 7 one pass is still faster, 2.00 ms against about three times the plain kernel's
+    // 0.77); the debug records keep the one-pass form
     if (pm && !dbg) {
       hipLaunchKernelGGL((k_eval_fp64_chain<A, AR, 1>), grid, dim3(threads), lds, s, b);
       hipLaunchKernelGGL((k_eval_fp64_chain<A, AR, 2>), grid, dim3(threads), lds, s, b);
